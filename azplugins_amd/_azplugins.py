@@ -1,0 +1,167 @@
+"""Stand-in for the reference's pybind11 module ``_azplugins`` (pair part).
+
+The reference registers one C++ class per evaluator and device
+(``module.cc:142-150,171-179``): ``PotentialPair<Name>``,
+``PotentialPair<Name>GPU``, ``PotentialPairDPDThermo<Name>[GPU]``,
+``PotentialPairConservative<Name>`` and ``AnisoPotentialPair<Name>[GPU]``.  The
+Python classes in ``pair.py`` select one by ``_cpp_class_name`` (+ "GPU").
+Here every name resolves to the same gfx950 implementation behind the C ABI
+(``azp_pair_*``); there is no CPU runtime path.
+"""
+
+import ctypes
+
+from . import _azp
+
+
+class PotentialPairGPUBase:
+    """Owns one ``azp_pair`` handle (a PotentialPair*GPU object)."""
+
+    cpp_class_name = None
+
+    def __init__(self, ntypes):
+        self._h = ctypes.c_void_p()
+        _azp.call("azp_pair_create", ctypes.byref(self._h), self.cpp_class_name.encode(), int(ntypes))
+        ev = ctypes.c_int()
+        drv = ctypes.c_int()
+        _azp.call("azp_class_lookup", self.cpp_class_name.encode(), ctypes.byref(ev), ctypes.byref(drv))
+        self.evaluator = ev.value
+        self.driver = drv.value
+        self.ntypes = int(ntypes)
+        self.nvalues = _azp.lib().azp_param_nvalues(self.evaluator)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                _azp.lib().azp_pair_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def setParams(self, ti, tj, values):
+        arr = _azp.dbl_array(values)
+        _azp.call("azp_pair_set_params", self._h, ti, tj, arr)
+
+    def getParams(self, ti, tj):
+        arr = (ctypes.c_double * self.nvalues)()
+        _azp.call("azp_pair_get_params", self._h, ti, tj, arr)
+        return list(arr)
+
+    def setRCut(self, ti, tj, r_cut):
+        _azp.call("azp_pair_set_r_cut", self._h, ti, tj, float(r_cut))
+
+    def setROn(self, ti, tj, r_on):
+        _azp.call("azp_pair_set_r_on", self._h, ti, tj, float(r_on))
+
+    def setShiftMode(self, mode):
+        _azp.call("azp_pair_set_shift_mode", self._h, _azp.SHIFT_MODES[mode])
+
+    def setPrecision(self, precision):
+        _azp.call("azp_pair_set_precision", self._h, _azp.PRECISIONS[precision])
+
+    def setThermostat(self, seed, dt, kT):
+        _azp.call("azp_pair_set_thermostat", self._h, int(seed) & 0xFFFF, float(dt), float(kT))
+
+    def getRCutMatrix(self):
+        arr = (ctypes.c_double * (self.ntypes * self.ntypes))()
+        _azp.call("azp_pair_get_r_cut", self._h, arr)
+        return list(arr)
+
+    def compute(self, nlist_handle, particles, outputs, timestep, compute_virial=True, stream=None):
+        _azp.call(
+            "azp_pair_compute",
+            self._h,
+            nlist_handle,
+            ctypes.byref(particles),
+            ctypes.byref(outputs),
+            int(timestep),
+            1 if compute_virial else 0,
+            _azp.stream_ptr(stream),
+        )
+
+
+_NAMES = (
+    "PotentialPairColloid",
+    "PotentialPairExpandedYukawa",
+    "PotentialPairHertz",
+    "PotentialPairPerturbedLennardJones",
+    "PotentialPairDPDThermoGeneralWeight",
+    "PotentialPairConservativeGeneralWeight",
+    "AnisoPotentialPairTwoPatchMorse",
+)
+
+for _n in _NAMES:
+    for _suffix in ("", "GPU"):
+        _cls = type(_n + _suffix, (PotentialPairGPUBase,), {"cpp_class_name": _n + _suffix})
+        globals()[_n + _suffix] = _cls
+del _n, _suffix, _cls
+
+
+class NeighborListGPU:
+    """Owns one ``azp_nlist`` handle (cell list + full neighbor list)."""
+
+    def __init__(self, r_buff=0.4):
+        self._h = ctypes.c_void_p()
+        _azp.call("azp_nlist_create", ctypes.byref(self._h))
+        self.set_r_buff(r_buff)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                _azp.lib().azp_nlist_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_r_buff(self, r_buff):
+        _azp.call("azp_nlist_set_r_buff", self._h, float(r_buff))
+
+    def set_r_cut(self, ntypes, r_cut_matrix):
+        arr = _azp.dbl_array(r_cut_matrix)
+        _azp.call("azp_nlist_set_r_cut", self._h, int(ntypes), arr)
+
+    def update(self, particles, force_rebuild=False, stream=None):
+        rebuilt = ctypes.c_int(0)
+        _azp.call(
+            "azp_nlist_update",
+            self._h,
+            ctypes.byref(particles),
+            1 if force_rebuild else 0,
+            ctypes.byref(rebuilt),
+            _azp.stream_ptr(stream),
+        )
+        return bool(rebuilt.value)
+
+    def get(self):
+        nn = ctypes.c_void_p()
+        nl = ctypes.c_void_p()
+        pitch = ctypes.c_uint32()
+        mx = ctypes.c_uint32()
+        _azp.call("azp_nlist_get", self._h, ctypes.byref(nn), ctypes.byref(nl), ctypes.byref(pitch), ctypes.byref(mx))
+        return nn.value, nl.value, pitch.value, mx.value
+
+    def download(self, N):
+        """(n_neigh[N], list[max_neighbors, pitch]) as numpy arrays."""
+        import numpy as np
+
+        _, _, pitch, mx = self.get()
+        nn = np.zeros(max(N, 1), dtype=np.uint32)
+        lst = np.zeros(max(mx * pitch, 1), dtype=np.uint32)
+        _azp.call("azp_nlist_download", self._h, int(N), nn.ctypes.data_as(ctypes.c_void_p),
+                  lst.ctypes.data_as(ctypes.c_void_p), lst.size)
+        return nn[:N], lst[: mx * pitch].reshape(mx, pitch) if pitch else lst[:0].reshape(0, 0)
+
+    def num_builds(self):
+        n = ctypes.c_uint64()
+        _azp.call("azp_nlist_num_builds", self._h, ctypes.byref(n))
+        return n.value

@@ -1,0 +1,580 @@
+// Cell list and full neighbor list on gfx950.
+//
+// Replaces the upstream CellListGPU + NeighborListGPUBinned pair that the
+// azplugins pair classes reach through their `nlist` argument (pair.py:110-111).
+//
+// Build (all kernels, no host round trip except the overflow / rebuild flags):
+//   1. bin:     cell of each particle, atomically counted per cell
+//   2. scan:    exclusive prefix sum of the counts (one workgroup)
+//   3. scatter: members written into per-cell ranges, then each cell's members
+//               sorted by index so the list order is deterministic
+//   4. walk:    one lane per particle scans its 27 neighbor cells (duplicates
+//               removed for grids narrower than 3 cells) and appends j with
+//               rsq <= r_list(ti,tj)^2 — the same FP64 predicate, evaluated
+//               without contraction, as the host oracle, so the pair sets
+//               agree exactly.  Entry k of particle i goes to
+//               nlist[k * pitch + i] (coalesced for one-lane-per-particle
+//               consumers).  Lanes that find more than max_neighbors entries
+//               keep counting and report the largest count; the host grows
+//               the list and reruns the walk.
+// Rebuild check: a particle moved farther than r_buff/2 since the last build.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "device_common.h"
+
+namespace azp
+{
+
+struct CellGrid
+    {
+    int n[3];
+    double w[3];
+    };
+
+__device__ __forceinline__ int cell_coord(double x, double L, double Linv, int n)
+    {
+    // fractional coordinate in [0, 1) for x in [-L/2, L/2)
+    double f = (x + 0.5 * L) * Linv;
+    int c = (int)floor(f * (double)n);
+    c %= n;
+    if (c < 0)
+        c += n;
+    return c;
+    }
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n)
+    {
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n)
+        p[k] = v;
+    }
+
+__global__ void k_bin(const double4* __restrict__ pos,
+                      uint32_t N,
+                      DevBox box,
+                      CellGrid g,
+                      uint32_t* __restrict__ cell_of,
+                      uint32_t* __restrict__ slot_of,
+                      uint32_t* __restrict__ count)
+    {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N)
+        return;
+    const double4 p = pos[i];
+    const int cx = cell_coord(p.x, box.L[0], box.Linv[0], g.n[0]);
+    const int cy = cell_coord(p.y, box.L[1], box.Linv[1], g.n[1]);
+    const int cz = cell_coord(p.z, box.L[2], box.Linv[2], g.n[2]);
+    const uint32_t c = (uint32_t)((cz * g.n[1] + cy) * g.n[0] + cx);
+    cell_of[i] = c;
+    slot_of[i] = atomicAdd(&count[c], 1u);
+    }
+
+// Exclusive scan of n counts by a single workgroup of 1024 lanes.
+__global__ void __launch_bounds__(1024) k_scan(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, uint32_t n)
+    {
+    __shared__ uint32_t s[1024];
+    const uint32_t t = threadIdx.x;
+    const uint32_t per = (n + 1023u) / 1024u;
+    const uint32_t b = t * per;
+    const uint32_t e = min(b + per, n);
+    uint32_t local = 0;
+    for (uint32_t k = b; k < e; ++k)
+        local += in[k];
+    s[t] = local;
+    __syncthreads();
+    // Hillis-Steele inclusive scan over the 1024 partial sums
+    for (uint32_t off = 1; off < 1024; off <<= 1)
+        {
+        uint32_t v = (t >= off) ? s[t - off] : 0u;
+        __syncthreads();
+        s[t] += v;
+        __syncthreads();
+        }
+    uint32_t run = s[t] - local;
+    for (uint32_t k = b; k < e; ++k)
+        {
+        out[k] = run;
+        run += in[k];
+        }
+    if (t == 1023)
+        out[n] = s[1023];
+    }
+
+__global__ void k_scatter(uint32_t N,
+                          const uint32_t* __restrict__ cell_of,
+                          const uint32_t* __restrict__ slot_of,
+                          const uint32_t* __restrict__ start,
+                          uint32_t* __restrict__ members)
+    {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N)
+        return;
+    members[start[cell_of[i]] + slot_of[i]] = i;
+    }
+
+// Insertion sort of each cell's members by particle index.
+__global__ void k_sort_cells(uint32_t ncell, const uint32_t* __restrict__ start, uint32_t* __restrict__ members)
+    {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell)
+        return;
+    const uint32_t b = start[c], e = start[c + 1];
+    for (uint32_t k = b + 1; k < e; ++k)
+        {
+        const uint32_t v = members[k];
+        uint32_t m = k;
+        while (m > b && members[m - 1] > v)
+            {
+            members[m] = members[m - 1];
+            --m;
+            }
+        members[m] = v;
+        }
+    }
+
+struct WalkArgs
+    {
+    const double4* pos;
+    uint32_t N;
+    DevBox box;
+    CellGrid g;
+    const uint32_t* cell_of;
+    const uint32_t* start;
+    const uint32_t* members;
+    const double* r_listsq;
+    uint32_t ntypes;
+    uint32_t* n_neigh;
+    uint32_t* nlist;
+    uint32_t pitch;
+    uint32_t max_neighbors;
+    uint32_t* overflow; // [0] largest count seen when > max_neighbors, [2] bad type seen
+    };
+
+// Type index of a particle, or ntypes when pos.w is not a valid type index.
+__device__ __forceinline__ uint32_t checked_type(double w, uint32_t ntypes)
+    {
+    return (w >= 0.0 && w < (double)ntypes) ? (uint32_t)w : ntypes;
+    }
+
+__global__ void __launch_bounds__(256) k_walk(const WalkArgs a)
+    {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.N)
+        return;
+    const double4 pi = a.pos[i];
+    const uint32_t ti = checked_type(pi.w, a.ntypes);
+    if (ti == a.ntypes)
+        {
+        // no neighbors for a particle whose type has no parameters
+        a.n_neigh[i] = 0;
+        atomicOr(a.overflow + 2, 1u);
+        return;
+        }
+    const uint32_t c = a.cell_of[i];
+    const int cx = (int)(c % (uint32_t)a.g.n[0]);
+    const int cy = (int)((c / (uint32_t)a.g.n[0]) % (uint32_t)a.g.n[1]);
+    const int cz = (int)(c / ((uint32_t)a.g.n[0] * (uint32_t)a.g.n[1]));
+    // neighbor offsets per axis, deduplicated when the grid is narrower than 3
+    int ox[3], oy[3], oz[3];
+    int nx = 0, ny = 0, nz = 0;
+    for (int d = -1; d <= 1; ++d)
+        {
+        const int x = (cx + d + a.g.n[0]) % a.g.n[0];
+        bool dup = false;
+        for (int m = 0; m < nx; ++m)
+            dup |= (ox[m] == x);
+        if (!dup)
+            ox[nx++] = x;
+        const int y = (cy + d + a.g.n[1]) % a.g.n[1];
+        dup = false;
+        for (int m = 0; m < ny; ++m)
+            dup |= (oy[m] == y);
+        if (!dup)
+            oy[ny++] = y;
+        const int z = (cz + d + a.g.n[2]) % a.g.n[2];
+        dup = false;
+        for (int m = 0; m < nz; ++m)
+            dup |= (oz[m] == z);
+        if (!dup)
+            oz[nz++] = z;
+        }
+    uint32_t count = 0;
+    for (int iz = 0; iz < nz; ++iz)
+        for (int iy = 0; iy < ny; ++iy)
+            for (int ix = 0; ix < nx; ++ix)
+                {
+                const uint32_t nb = (uint32_t)((oz[iz] * a.g.n[1] + oy[iy]) * a.g.n[0] + ox[ix]);
+                const uint32_t b = a.start[nb], e = a.start[nb + 1];
+                for (uint32_t m = b; m < e; ++m)
+                    {
+                    const uint32_t j = a.members[m];
+                    if (j == i)
+                        continue;
+                    const double4 pj = a.pos[j];
+                    const uint32_t tj = checked_type(pj.w, a.ntypes);
+                    if (tj == a.ntypes)
+                        continue;
+                    const double rl = a.r_listsq[ti * a.ntypes + tj];
+                    if (!(rl > 0.0))
+                        continue;
+                    const double dx = min_image_1d(pi.x - pj.x, a.box.L[0], a.box.Linv[0]);
+                    const double dy = min_image_1d(pi.y - pj.y, a.box.L[1], a.box.Linv[1]);
+                    const double dz = min_image_1d(pi.z - pj.z, a.box.L[2], a.box.Linv[2]);
+                    const double rsq = rsq3_nofma(dx, dy, dz);
+                    if (rsq <= rl)
+                        {
+                        if (count < a.max_neighbors)
+                            a.nlist[(size_t)count * a.pitch + i] = j;
+                        ++count;
+                        }
+                    }
+                }
+    a.n_neigh[i] = count;
+    if (count > a.max_neighbors)
+        atomicMax(a.overflow, count);
+    }
+
+// Sets *flag when some particle moved farther than sqrt(thresh_sq).
+__global__ void k_displacement(const double4* __restrict__ pos,
+                               const double4* __restrict__ last,
+                               uint32_t N,
+                               DevBox box,
+                               double thresh_sq,
+                               uint32_t* __restrict__ flag)
+    {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N)
+        return;
+    const double4 p = pos[i];
+    const double4 q = last[i];
+    const double dx = min_image_1d(p.x - q.x, box.L[0], box.Linv[0]);
+    const double dy = min_image_1d(p.y - q.y, box.L[1], box.Linv[1]);
+    const double dz = min_image_1d(p.z - q.z, box.L[2], box.Linv[2]);
+    if (rsq3_nofma(dx, dy, dz) > thresh_sq)
+        atomicOr(flag, 1u);
+    }
+
+__global__ void k_copy_d4(const double4* __restrict__ in, double4* __restrict__ out, uint32_t N)
+    {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < N)
+        out[i] = in[i];
+    }
+
+} // namespace azp
+
+using namespace azp;
+
+struct azp_nlist
+    {
+    double r_buff = 0.4;
+    uint32_t ntypes = 0;
+    std::vector<double> r_cut;
+    bool params_dirty = true;
+
+    DeviceArray<double> r_listsq;
+    DeviceArray<uint32_t> n_neigh;
+    DeviceArray<uint32_t> list;
+    DeviceArray<double4> last_pos;
+    DeviceArray<uint32_t> cell_of, slot_of, count, start, members;
+    DeviceArray<uint32_t> flags; // [0] overflow, [1] rebuild, [2] bad type
+    uint32_t* h_flags = nullptr;  // pinned
+
+    uint32_t pitch = 0;
+    uint32_t max_neighbors = 0;
+    uint32_t built_N = 0;
+    azp_box built_box = {0, 0, 0};
+    bool have_list = false;
+    uint64_t n_builds = 0;
+
+    ~azp_nlist()
+        {
+        if (h_flags)
+            (void)hipHostFree(h_flags);
+        }
+
+    double r_list_max() const
+        {
+        double m = 0.0;
+        for (double rc : r_cut)
+            if (rc > 0.0)
+                m = std::max(m, rc + r_buff);
+        return m;
+        }
+    };
+
+static int nlist_build(azp_nlist* nl, const azp_particles* p, hipStream_t s)
+    {
+    const uint32_t N = p->N;
+    const DevBox box = make_box(p->box);
+    const double rmax = nl->r_list_max();
+    CellGrid g;
+    const double L[3] = {p->box.Lx, p->box.Ly, p->box.Lz};
+    size_t ncell = 1;
+    for (int k = 0; k < 3; ++k)
+        {
+        int n = (rmax > 0.0) ? (int)std::floor(L[k] / (rmax * (1.0 + 1e-9))) : 1;
+        n = std::max(1, std::min(n, 1024));
+        g.n[k] = n;
+        g.w[k] = L[k] / n;
+        ncell *= (size_t)n;
+        }
+    // cap the grid at ~4 cells per particle (dilute systems)
+    while (ncell > 4 * (size_t)std::max<uint32_t>(N, 64) + 64)
+        {
+        int kk = 0;
+        for (int k = 1; k < 3; ++k)
+            if (g.n[k] > g.n[kk])
+                kk = k;
+        g.n[kk] = std::max(1, g.n[kk] / 2);
+        g.w[kk] = L[kk] / g.n[kk];
+        ncell = (size_t)g.n[0] * g.n[1] * g.n[2];
+        }
+
+    AZP_TRY(nl->cell_of.reserve(N));
+    AZP_TRY(nl->slot_of.reserve(N));
+    AZP_TRY(nl->count.reserve(ncell));
+    AZP_TRY(nl->start.reserve(ncell + 1));
+    AZP_TRY(nl->members.reserve(N));
+    AZP_TRY(nl->n_neigh.reserve(N));
+    AZP_TRY(nl->last_pos.reserve(N));
+    AZP_TRY(nl->flags.reserve(3));
+
+    const unsigned B = 256;
+    const double4* pos = reinterpret_cast<const double4*>(p->d_pos);
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(ncell, B)), dim3(B), 0, s, nl->count.ptr, 0u, ncell);
+    hipLaunchKernelGGL(k_bin, dim3(grid_for(N, B)), dim3(B), 0, s, pos, N, box, g,
+                       nl->cell_of.ptr, nl->slot_of.ptr, nl->count.ptr);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, nl->count.ptr, nl->start.ptr, (uint32_t)ncell);
+    hipLaunchKernelGGL(k_scatter, dim3(grid_for(N, B)), dim3(B), 0, s, N, nl->cell_of.ptr,
+                       nl->slot_of.ptr, nl->start.ptr, nl->members.ptr);
+    hipLaunchKernelGGL(k_sort_cells, dim3(grid_for(ncell, B)), dim3(B), 0, s, (uint32_t)ncell,
+                       nl->start.ptr, nl->members.ptr);
+    AZP_LAUNCH_CHECK();
+
+    // initial capacity from the density
+    const double vol = L[0] * L[1] * L[2];
+    if (nl->max_neighbors == 0)
+        {
+        const double est = (double)N / vol * 4.18879020478639 * rmax * rmax * rmax;
+        nl->max_neighbors = (uint32_t)std::min(1.3 * est + 16.0, (double)N);
+        nl->max_neighbors = (nl->max_neighbors + 7u) & ~7u;
+        if (nl->max_neighbors == 0)
+            nl->max_neighbors = 8;
+        }
+    nl->pitch = wave_pitch(N);
+
+    for (int attempt = 0; attempt < 4; ++attempt)
+        {
+        AZP_TRY(nl->list.reserve((size_t)nl->max_neighbors * nl->pitch));
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, nl->flags.ptr, 0u, (size_t)3);
+        WalkArgs wa;
+        wa.pos = pos;
+        wa.N = N;
+        wa.box = box;
+        wa.g = g;
+        wa.cell_of = nl->cell_of.ptr;
+        wa.start = nl->start.ptr;
+        wa.members = nl->members.ptr;
+        wa.r_listsq = nl->r_listsq.ptr;
+        wa.ntypes = nl->ntypes;
+        wa.n_neigh = nl->n_neigh.ptr;
+        wa.nlist = nl->list.ptr;
+        wa.pitch = nl->pitch;
+        wa.max_neighbors = nl->max_neighbors;
+        wa.overflow = nl->flags.ptr;
+        hipLaunchKernelGGL(k_walk, dim3(grid_for(N, B)), dim3(B), 0, s, wa);
+        AZP_LAUNCH_CHECK();
+        AZP_HIP(hipMemcpyAsync(nl->h_flags, nl->flags.ptr, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        AZP_HIP(hipStreamSynchronize(s));
+        if (nl->h_flags[2])
+            return set_error(AZP_ERR_INVALID,
+                             "a particle type index is outside [0, %u) (pos.w must hold the type)",
+                             nl->ntypes);
+        const uint32_t need = nl->h_flags[0];
+        if (need == 0)
+            {
+            hipLaunchKernelGGL(k_copy_d4, dim3(grid_for(N, B)), dim3(B), 0, s, pos, nl->last_pos.ptr, N);
+            AZP_LAUNCH_CHECK();
+            nl->built_N = N;
+            nl->built_box = p->box;
+            nl->have_list = true;
+            nl->n_builds++;
+            return AZP_OK;
+            }
+        nl->max_neighbors = ((uint32_t)(need * 1.1) + 8u) & ~7u;
+        }
+    return set_error(AZP_ERR_STATE, "neighbor list did not converge after growing its capacity");
+    }
+
+extern "C" int azp_nlist_create(azp_nlist** out)
+    {
+    if (!out)
+        return set_error(AZP_ERR_INVALID, "null out");
+    azp_nlist* nl = new azp_nlist();
+    if (hipHostMalloc((void**)&nl->h_flags, 4 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess)
+        {
+        nl->h_flags = nullptr;
+        delete nl;
+        return set_error(AZP_ERR_NOMEM, "pinned host allocation failed");
+        }
+    *out = nl;
+    return AZP_OK;
+    }
+
+extern "C" int azp_nlist_destroy(azp_nlist* nl)
+    {
+    delete nl;
+    return AZP_OK;
+    }
+
+extern "C" int azp_nlist_set_r_buff(azp_nlist* nl, double r_buff)
+    {
+    if (!nl || !(r_buff >= 0.0))
+        return set_error(AZP_ERR_INVALID, "r_buff must be >= 0");
+    if (r_buff != nl->r_buff)
+        nl->params_dirty = true;
+    nl->r_buff = r_buff;
+    return AZP_OK;
+    }
+
+extern "C" int azp_nlist_set_r_cut(azp_nlist* nl, uint32_t ntypes, const double* r_cut)
+    {
+    if (!nl || ntypes == 0 || !r_cut)
+        return set_error(AZP_ERR_INVALID, "bad r_cut arguments");
+    std::vector<double> rc(r_cut, r_cut + (size_t)ntypes * ntypes);
+    for (uint32_t a = 0; a < ntypes; ++a)
+        for (uint32_t b = 0; b < ntypes; ++b)
+            if (rc[a * ntypes + b] != rc[b * ntypes + a])
+                return set_error(AZP_ERR_INVALID, "r_cut must be symmetric");
+    if (ntypes != nl->ntypes || rc != nl->r_cut)
+        nl->params_dirty = true;
+    nl->ntypes = ntypes;
+    nl->r_cut = rc;
+    return AZP_OK;
+    }
+
+extern "C" int azp_nlist_update(azp_nlist* nl, const azp_particles* p, int force_rebuild, int* rebuilt, void* stream)
+    {
+    if (rebuilt)
+        *rebuilt = 0;
+    if (!nl || !p)
+        return set_error(AZP_ERR_INVALID, "null argument");
+    if (nl->ntypes == 0)
+        return set_error(AZP_ERR_STATE, "set r_cut before building the neighbor list");
+    if (!box_valid(p->box))
+        return set_error(AZP_ERR_BOX, "box lengths must be positive");
+    if (p->N > 0 && !p->d_pos)
+        return set_error(AZP_ERR_INVALID, "null positions");
+    const double rmax = nl->r_list_max();
+    const double L[3] = {p->box.Lx, p->box.Ly, p->box.Lz};
+    for (int k = 0; k < 3; ++k)
+        if (L[k] < 2.0 * rmax)
+            return set_error(AZP_ERR_BOX,
+                             "box length %g is smaller than twice the neighbor range %g: particles "
+                             "would interact with their own images",
+                             L[k],
+                             rmax);
+    hipStream_t s = (hipStream_t)stream;
+    if (nl->params_dirty)
+        {
+        const size_t nt2 = (size_t)nl->ntypes * nl->ntypes;
+        std::vector<double> rl(nt2);
+        for (size_t k = 0; k < nt2; ++k)
+            {
+            const double r = nl->r_cut[k] + nl->r_buff;
+            rl[k] = nl->r_cut[k] > 0.0 ? r * r : 0.0;
+            }
+        AZP_TRY(nl->r_listsq.reserve(nt2));
+        AZP_HIP(hipMemcpyAsync(nl->r_listsq.ptr, rl.data(), nt2 * sizeof(double), hipMemcpyHostToDevice, s));
+        }
+    bool need = force_rebuild || !nl->have_list || nl->params_dirty || p->N != nl->built_N
+                || p->box.Lx != nl->built_box.Lx || p->box.Ly != nl->built_box.Ly
+                || p->box.Lz != nl->built_box.Lz;
+    if (!need && p->N > 0)
+        {
+        const unsigned B = 256;
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, nl->flags.ptr + 1, 0u, (size_t)1);
+        const double half = 0.5 * nl->r_buff;
+        hipLaunchKernelGGL(k_displacement, dim3(grid_for(p->N, B)), dim3(B), 0, s,
+                           reinterpret_cast<const double4*>(p->d_pos), nl->last_pos.ptr, p->N,
+                           make_box(p->box), half * half, nl->flags.ptr + 1);
+        AZP_LAUNCH_CHECK();
+        AZP_HIP(hipMemcpyAsync(nl->h_flags + 1, nl->flags.ptr + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        AZP_HIP(hipStreamSynchronize(s));
+        need = nl->h_flags[1] != 0;
+        }
+    nl->params_dirty = false;
+    if (!need)
+        return AZP_OK;
+    if (p->N == 0)
+        {
+        nl->built_N = 0;
+        nl->built_box = p->box;
+        nl->have_list = true;
+        nl->pitch = 0;
+        AZP_TRY(nl->n_neigh.reserve(1));
+        AZP_TRY(nl->list.reserve(1));
+        if (rebuilt)
+            *rebuilt = 1;
+        return AZP_OK;
+        }
+    AZP_TRY(nlist_build(nl, p, s));
+    if (rebuilt)
+        *rebuilt = 1;
+    return AZP_OK;
+    }
+
+extern "C" int azp_nlist_get(const azp_nlist* nl,
+                             const uint32_t** d_n_neigh,
+                             const uint32_t** d_nlist,
+                             uint32_t* pitch,
+                             uint32_t* max_neighbors)
+    {
+    if (!nl)
+        return set_error(AZP_ERR_INVALID, "null nlist");
+    if (!nl->have_list)
+        return set_error(AZP_ERR_STATE, "neighbor list not built");
+    if (d_n_neigh)
+        *d_n_neigh = nl->n_neigh.ptr;
+    if (d_nlist)
+        *d_nlist = nl->list.ptr;
+    if (pitch)
+        *pitch = nl->pitch;
+    if (max_neighbors)
+        *max_neighbors = nl->max_neighbors;
+    return AZP_OK;
+    }
+
+extern "C" int azp_nlist_num_builds(const azp_nlist* nl, uint64_t* n)
+    {
+    if (!nl || !n)
+        return set_error(AZP_ERR_INVALID, "null argument");
+    *n = nl->n_builds;
+    return AZP_OK;
+    }
+
+extern "C" int azp_nlist_download(const azp_nlist* nl,
+                                  uint32_t N,
+                                  uint32_t* h_n_neigh,
+                                  uint32_t* h_nlist,
+                                  uint64_t nlist_capacity)
+    {
+    if (!nl || !h_n_neigh || !h_nlist)
+        return set_error(AZP_ERR_INVALID, "null argument");
+    if (!nl->have_list)
+        return set_error(AZP_ERR_STATE, "neighbor list not built");
+    if (N != nl->built_N)
+        return set_error(AZP_ERR_INVALID, "N %u differs from the list's %u", N, nl->built_N);
+    const uint64_t need = (uint64_t)nl->max_neighbors * nl->pitch;
+    if (nlist_capacity < need)
+        return set_error(AZP_ERR_INVALID, "host list needs %llu entries", (unsigned long long)need);
+    if (N == 0)
+        return AZP_OK;
+    AZP_HIP(hipDeviceSynchronize());
+    AZP_HIP(hipMemcpy(h_n_neigh, nl->n_neigh.ptr, (size_t)N * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    AZP_HIP(hipMemcpy(h_nlist, nl->list.ptr, (size_t)need * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    return AZP_OK;
+    }

@@ -1,0 +1,403 @@
+// Isotropic pair forces over a full neighbor list (gfx950).
+//
+// Replaces the launcher instantiated per evaluator at
+// PotentialPairGPUKernel.cu.inc:25-28 (gpu_compute_pair_forces<Eval>).
+//
+// Layout: one lane per particle i, walking its full neighbor list
+// nlist[k * pitch + i] — consecutive lanes read consecutive words, so the
+// list stream is fully coalesced.  Per-type-pair constants (cutoff, hoisted
+// energy shift, parameter products) live in LDS; with one particle type they
+// are held in registers.  Force, energy and the six virial components are
+// accumulated in FP64 registers and written once (no atomics; every lane owns
+// its particle).
+//
+// Two precision forms:
+//   plain (FP64):  FP64 positions, min image, evaluator and accumulation.
+//   mixed:         positions as wrapped 32-bit fixed point (16-byte gather),
+//                  dx exact in integers then FP32, FP32 evaluator, FP64
+//                  accumulation.  Pairs whose FP32 rsq lies within a relative
+//                  band of 1e-5 around rcutsq are re-tested on the FP64
+//                  positions, so the in/out cutoff decision is the FP64 one.
+#include <type_traits>
+
+#include "evaluators.h"
+
+namespace azp
+{
+
+constexpr float kCutBand = 1e-5f;
+
+struct PairKernelArgs
+    {
+    const double4* pos;
+    const int4* pos_fixed;
+    uint32_t N;
+    DevBox box;
+    float fx_scale[3]; // L / 2^32 per axis
+    const uint32_t* n_neigh;
+    const uint32_t* nlist;
+    uint32_t nlist_pitch;
+    const void* params;
+    const double* rcutsq;
+    const double* ronsq;
+    uint32_t ntypes;
+    int shift_mode;
+    double4* force;
+    double* virial;
+    uint32_t vpitch;
+    };
+
+template<typename R> struct TypePairExtra
+    {
+    double rcsq64;
+    R rcsq;
+    R rcsq_lo; // rcsq (1 - band)
+    R rcsq_hi; // rcsq (1 + band)
+    R ronsq;
+    };
+
+template<class Eval, typename R>
+__device__ __forceinline__ void prepare_pair(const PairKernelArgs& a,
+                                             uint32_t t,
+                                             typename Eval::template Prep<R>& q,
+                                             TypePairExtra<R>& x)
+    {
+    const auto* params = static_cast<const typename Eval::param_type*>(a.params);
+    const double rc = a.rcutsq[t];
+    const double ron = a.ronsq ? a.ronsq[t] : 0.0;
+    const bool shift
+        = a.shift_mode == AZP_SHIFT_SHIFT || (a.shift_mode == AZP_SHIFT_XPLOR && ron > rc);
+    q = Eval::template prepare<R>(params[t], rc, shift);
+    x.rcsq64 = rc;
+    x.rcsq = (R)rc;
+    x.rcsq_lo = (R)(rc * (1.0 - (double)kCutBand));
+    x.rcsq_hi = (R)(rc * (1.0 + (double)kCutBand));
+    x.ronsq = (R)ron;
+    }
+
+// xplor smoothing (upstream PotentialPair, restated in the oracle):
+// S = (rc2 - r2)^2 (rc2 + 2 r2 - 3 ron2) / (rc2 - ron2)^3 for ron2 <= r2 < rc2.
+template<typename R>
+__device__ __forceinline__ void apply_xplor(R rsq, R rcutsq, R ronsq, R& fdivr, R& e)
+    {
+    if (rsq >= ronsq && rsq < rcutsq)
+        {
+        const R old_e = e, old_f = fdivr;
+        const R denom = (rcutsq - ronsq) * (rcutsq - ronsq) * (rcutsq - ronsq);
+        const R denom_inv = f_rcp(denom);
+        const R d = rsq - rcutsq;
+        const R S = d * d * (rcutsq + R(2) * rsq - R(3) * ronsq) * denom_inv;
+        const R dS_dr_divr = R(12) * (rsq - ronsq) * d * denom_inv;
+        e = old_e * S;
+        fdivr = S * old_f - dS_dr_divr * old_e;
+        }
+    }
+
+struct Acc
+    {
+    double fx = 0, fy = 0, fz = 0, e = 0;
+    double v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0, v5 = 0;
+    };
+
+template<bool VIRIAL>
+__device__ __forceinline__ void accumulate(Acc& acc, double fd, double dx, double dy, double dz, double e)
+    {
+    const double px = fd * dx, py = fd * dy, pz = fd * dz;
+    acc.fx += px;
+    acc.fy += py;
+    acc.fz += pz;
+    acc.e += e;
+    if (VIRIAL)
+        {
+        acc.v0 = fma(px, dx, acc.v0);
+        acc.v1 = fma(px, dy, acc.v1);
+        acc.v2 = fma(px, dz, acc.v2);
+        acc.v3 = fma(py, dy, acc.v3);
+        acc.v4 = fma(py, dz, acc.v4);
+        acc.v5 = fma(pz, dz, acc.v5);
+        }
+    }
+
+template<bool VIRIAL>
+__device__ __forceinline__ void store(const PairKernelArgs& a, uint32_t i, const Acc& acc)
+    {
+    a.force[i] = make_double4(acc.fx, acc.fy, acc.fz, 0.5 * acc.e);
+    if (VIRIAL)
+        {
+        const size_t p = a.vpitch;
+        a.virial[0 * p + i] = 0.5 * acc.v0;
+        a.virial[1 * p + i] = 0.5 * acc.v1;
+        a.virial[2 * p + i] = 0.5 * acc.v2;
+        a.virial[3 * p + i] = 0.5 * acc.v3;
+        a.virial[4 * p + i] = 0.5 * acc.v4;
+        a.virial[5 * p + i] = 0.5 * acc.v5;
+        }
+    }
+
+// FP64 displacement with minimum image, no contraction (matches the oracle).
+__device__ __forceinline__ double exact_rsq(const PairKernelArgs& a, uint32_t i, uint32_t j)
+    {
+    const double4 pi = a.pos[i];
+    const double4 pj = a.pos[j];
+    const double dx = min_image_1d(pi.x - pj.x, a.box.L[0], a.box.Linv[0]);
+    const double dy = min_image_1d(pi.y - pj.y, a.box.L[1], a.box.Linv[1]);
+    const double dz = min_image_1d(pi.z - pj.z, a.box.L[2], a.box.Linv[2]);
+    return rsq3_nofma(dx, dy, dz);
+    }
+
+template<class Eval, bool MIXED, bool VIRIAL, bool SINGLE>
+__global__ void __launch_bounds__(512) k_pair(const PairKernelArgs a)
+    {
+    using R = std::conditional_t<MIXED, float, double>;
+    using Prep = typename Eval::template Prep<R>;
+    extern __shared__ double smem[];
+    const uint32_t nt = a.ntypes;
+    const uint32_t nt2 = nt * nt;
+    Prep* s_prep = reinterpret_cast<Prep*>(smem);
+    TypePairExtra<R>* s_x = reinterpret_cast<TypePairExtra<R>*>(s_prep + nt2);
+
+    Prep q0;
+    TypePairExtra<R> x0;
+    if (SINGLE)
+        {
+        prepare_pair<Eval, R>(a, 0, q0, x0);
+        }
+    else
+        {
+        for (uint32_t t = threadIdx.x; t < nt2; t += blockDim.x)
+            prepare_pair<Eval, R>(a, t, s_prep[t], s_x[t]);
+        __syncthreads();
+        }
+
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.N)
+        return;
+
+    Acc acc;
+    const uint32_t nn = a.n_neigh[i];
+    const bool xplor = a.shift_mode == AZP_SHIFT_XPLOR;
+
+    if constexpr (!MIXED)
+        {
+        const double4 pi = a.pos[i];
+        const uint32_t ti = (uint32_t)pi.w;
+        for (uint32_t k = 0; k < nn; ++k)
+            {
+            const uint32_t j = a.nlist[(size_t)k * a.nlist_pitch + i];
+            const double4 pj = a.pos[j];
+            const double dx = min_image_1d(pi.x - pj.x, a.box.L[0], a.box.Linv[0]);
+            const double dy = min_image_1d(pi.y - pj.y, a.box.L[1], a.box.Linv[1]);
+            const double dz = min_image_1d(pi.z - pj.z, a.box.L[2], a.box.Linv[2]);
+            const double rsq = dx * dx + dy * dy + dz * dz;
+            const uint32_t tp = SINGLE ? 0u : ti * nt + (uint32_t)pj.w;
+            const double rcsq = SINGLE ? x0.rcsq64 : s_x[tp].rcsq64;
+            if (rsq < rcsq)
+                {
+                const Prep& q = SINGLE ? q0 : s_prep[tp];
+                if (Eval::active(q))
+                    {
+                    double fdivr, e;
+                    Eval::compute(q, rsq, fdivr, e);
+                    if (xplor)
+                        apply_xplor<double>(rsq, rcsq, SINGLE ? x0.ronsq : s_x[tp].ronsq, fdivr, e);
+                    accumulate<VIRIAL>(acc, fdivr, dx, dy, dz, e);
+                    }
+                }
+            }
+        }
+    else
+        {
+        const int4 pi = a.pos_fixed[i];
+        const uint32_t ti = (uint32_t)pi.w;
+        const float sx = a.fx_scale[0], sy = a.fx_scale[1], sz = a.fx_scale[2];
+        for (uint32_t k = 0; k < nn; ++k)
+            {
+            const uint32_t j = a.nlist[(size_t)k * a.nlist_pitch + i];
+            const int4 pj = a.pos_fixed[j];
+            const float dx = (float)fixed_delta(pi.x, pj.x) * sx;
+            const float dy = (float)fixed_delta(pi.y, pj.y) * sy;
+            const float dz = (float)fixed_delta(pi.z, pj.z) * sz;
+            float rsq = dx * dx + dy * dy + dz * dz;
+            const uint32_t tp = SINGLE ? 0u : ti * nt + (uint32_t)pj.w;
+            const TypePairExtra<R>& x = SINGLE ? x0 : s_x[tp];
+            if (rsq < x.rcsq_hi)
+                {
+                bool in = rsq < x.rcsq_lo;
+                if (!in)
+                    {
+                    const double r64 = exact_rsq(a, i, j);
+                    in = r64 < x.rcsq64;
+                    rsq = (float)r64;
+                    }
+                const Prep& q = SINGLE ? q0 : s_prep[tp];
+                if (in && Eval::active(q))
+                    {
+                    float fdivr, e;
+                    Eval::compute(q, rsq, fdivr, e);
+                    if (xplor)
+                        apply_xplor<float>(rsq, x.rcsq, x.ronsq, fdivr, e);
+                    accumulate<VIRIAL>(acc, (double)fdivr, (double)dx, (double)dy, (double)dz, (double)e);
+                    }
+                }
+            }
+        }
+    store<VIRIAL>(a, i, acc);
+    }
+
+__global__ void k_pack_fixed(const double4* __restrict__ pos, uint32_t N, DevBox box, int4* __restrict__ out)
+    {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N)
+        return;
+    const double4 p = pos[i];
+    out[i] = make_int4(to_fixed(p.x, box.Linv[0]),
+                       to_fixed(p.y, box.Linv[1]),
+                       to_fixed(p.z, box.Linv[2]),
+                       (int)p.w);
+    }
+
+template<class Eval, bool MIXED, bool VIRIAL, bool SINGLE>
+static int launch_t(const PairKernelArgs& ka, unsigned block, hipStream_t stream)
+    {
+    using R = std::conditional_t<MIXED, float, double>;
+    const size_t nt2 = (size_t)ka.ntypes * ka.ntypes;
+    const size_t shmem
+        = SINGLE ? 0 : nt2 * (sizeof(typename Eval::template Prep<R>) + sizeof(TypePairExtra<R>));
+    if (shmem > 64 * 1024)
+        return set_error(AZP_ERR_UNSUPPORTED, "too many particle types (%u) for LDS staging", ka.ntypes);
+    const unsigned grid = grid_for(ka.N, block);
+    hipLaunchKernelGGL((k_pair<Eval, MIXED, VIRIAL, SINGLE>), dim3(grid), dim3(block), shmem, stream, ka);
+    AZP_LAUNCH_CHECK();
+    return AZP_OK;
+    }
+
+template<class Eval, bool MIXED>
+static int launch_eval(const PairKernelArgs& ka, bool virial, unsigned block, hipStream_t s)
+    {
+    const bool single = ka.ntypes == 1;
+    if (virial)
+        return single ? launch_t<Eval, MIXED, true, true>(ka, block, s)
+                      : launch_t<Eval, MIXED, true, false>(ka, block, s);
+    return single ? launch_t<Eval, MIXED, false, true>(ka, block, s)
+                  : launch_t<Eval, MIXED, false, false>(ka, block, s);
+    }
+
+int pack_fixed(const double* d_pos, uint32_t N, const azp_box& box, int32_t* d_out, hipStream_t stream)
+    {
+    if (N == 0)
+        return AZP_OK;
+    const unsigned block = 256;
+    hipLaunchKernelGGL(k_pack_fixed,
+                       dim3(grid_for(N, block)),
+                       dim3(block),
+                       0,
+                       stream,
+                       reinterpret_cast<const double4*>(d_pos),
+                       N,
+                       make_box(box),
+                       reinterpret_cast<int4*>(d_out));
+    AZP_LAUNCH_CHECK();
+    return AZP_OK;
+    }
+
+int validate_common(const azp_pair_args* args)
+    {
+    if (!args)
+        return set_error(AZP_ERR_INVALID, "null args");
+    const azp_particles& p = args->particles;
+    if (p.N == 0)
+        return AZP_OK;
+    if (!p.d_pos || !args->out.d_force || !args->d_n_neigh || !args->d_nlist || !args->d_rcutsq)
+        return set_error(AZP_ERR_INVALID, "missing device array in pair args");
+    if (!box_valid(p.box))
+        return set_error(AZP_ERR_BOX, "box lengths must be positive");
+    if (args->ntypes == 0)
+        return set_error(AZP_ERR_INVALID, "ntypes must be >= 1");
+    if (args->nlist_pitch < p.N)
+        return set_error(AZP_ERR_INVALID, "nlist pitch %u < N %u", args->nlist_pitch, p.N);
+    if (args->compute_virial && (!args->out.d_virial || args->out.virial_pitch < p.N))
+        return set_error(AZP_ERR_INVALID, "virial requested without a virial array of pitch >= N");
+    if (args->block_size != 0
+        && (args->block_size % 64 != 0 || args->block_size > 512))
+        return set_error(AZP_ERR_INVALID, "block size %u must be a multiple of 64, <= 512", args->block_size);
+    return AZP_OK;
+    }
+
+} // namespace azp
+
+using namespace azp;
+
+extern "C" int azp_gpu_pack_fixed(const double* d_pos,
+                                  uint32_t N,
+                                  const azp_box* box,
+                                  int32_t* d_pos_fixed,
+                                  void* stream)
+    {
+    if (!box || !box_valid(*box))
+        return set_error(AZP_ERR_BOX, "invalid box");
+    if (N > 0 && (!d_pos || !d_pos_fixed))
+        return set_error(AZP_ERR_INVALID, "null pointer");
+    return pack_fixed(d_pos, N, *box, d_pos_fixed, (hipStream_t)stream);
+    }
+
+extern "C" int azp_gpu_compute_pair_forces(const azp_pair_args* args, int evaluator, const void* d_params)
+    {
+    AZP_TRY(validate_common(args));
+    const azp_particles& p = args->particles;
+    if (p.N == 0)
+        return AZP_OK;
+    if (!d_params)
+        return set_error(AZP_ERR_INVALID, "null params");
+    const bool mixed = args->precision == AZP_PREC_MIXED;
+    if (mixed && !p.d_pos_fixed)
+        return set_error(AZP_ERR_INVALID, "mixed precision needs d_pos_fixed (azp_gpu_pack_fixed)");
+    if (args->shift_mode == AZP_SHIFT_XPLOR && !args->d_ronsq)
+        return set_error(AZP_ERR_INVALID, "xplor needs d_ronsq");
+
+    PairKernelArgs ka;
+    ka.pos = reinterpret_cast<const double4*>(p.d_pos);
+    ka.pos_fixed = reinterpret_cast<const int4*>(p.d_pos_fixed);
+    ka.N = p.N;
+    ka.box = make_box(p.box);
+    for (int k = 0; k < 3; ++k)
+        ka.fx_scale[k] = (float)(ka.box.L[k] / 4294967296.0);
+    ka.n_neigh = args->d_n_neigh;
+    ka.nlist = args->d_nlist;
+    ka.nlist_pitch = args->nlist_pitch;
+    ka.params = d_params;
+    ka.rcutsq = args->d_rcutsq;
+    ka.ronsq = args->d_ronsq;
+    ka.ntypes = args->ntypes;
+    ka.shift_mode = args->shift_mode;
+    ka.force = reinterpret_cast<double4*>(args->out.d_force);
+    ka.virial = args->out.d_virial;
+    ka.vpitch = args->out.virial_pitch;
+    const unsigned block = args->block_size ? args->block_size : 256;
+    const bool vir = args->compute_virial != 0;
+    hipStream_t s = (hipStream_t)args->stream;
+
+    switch (evaluator)
+        {
+    case AZP_EVAL_HERTZ:
+        return mixed ? launch_eval<EvalHertz, true>(ka, vir, block, s)
+                     : launch_eval<EvalHertz, false>(ka, vir, block, s);
+    case AZP_EVAL_EXPANDED_YUKAWA:
+        return mixed ? launch_eval<EvalExpandedYukawa, true>(ka, vir, block, s)
+                     : launch_eval<EvalExpandedYukawa, false>(ka, vir, block, s);
+    case AZP_EVAL_PERTURBED_LJ:
+        return mixed ? launch_eval<EvalPerturbedLJ, true>(ka, vir, block, s)
+                     : launch_eval<EvalPerturbedLJ, false>(ka, vir, block, s);
+    case AZP_EVAL_COLLOID:
+        if (mixed)
+            return set_error(AZP_ERR_UNSUPPORTED,
+                             "Colloid runs in the plain FP64 form only (its colloid-colloid "
+                             "branch cancels catastrophically in FP32)");
+        return launch_eval<EvalColloid, false>(ka, vir, block, s);
+    case AZP_EVAL_DPD_GENERAL_WEIGHT:
+        // PotentialPairConservative<DPDPairEvaluatorGeneralWeight>
+        return mixed ? launch_eval<EvalDPDConservative, true>(ka, vir, block, s)
+                     : launch_eval<EvalDPDConservative, false>(ka, vir, block, s);
+    default:
+        return set_error(AZP_ERR_INVALID, "evaluator %d is not an isotropic pair evaluator", evaluator);
+        }
+    }

@@ -1,0 +1,114 @@
+"""Minimal HOOMD-compatible simulation harness on the gfx950 library.
+
+HOOMD-blue (which the reference plugs into) is not required: this harness
+provides the subset the pair-force path and its tests use — ``Snapshot``,
+``Simulation`` with ``create_state_from_snapshot``/``run``/``timestep``/
+``seed``, ``md.nlist.Cell``, ``md.Integrator``, ``md.methods.ConstantVolume``
+and ``md.methods.Langevin``, ``md.compute.ThermodynamicQuantities``,
+``filter.All`` and ``variant.Constant``.  Particle data lives in HBM as torch
+tensors (device memory only; all arithmetic runs in libazp.so kernels).
+
+Conventions that differ from HOOMD-blue are documented where they occur:
+boxes are orthorhombic and periodic; angular momenta are space-frame vectors
+of spherical tops (isotropic moment of inertia).
+"""
+
+import types as _types
+import warnings
+
+import numpy as np
+
+from . import filter, variant  # noqa: F401
+from .state import Box, Snapshot, State  # noqa: F401
+
+
+class _Communicator:
+    rank = 0
+    num_ranks = 1
+
+
+class Device:
+    """Single-GPU device descriptor (one process per GPU)."""
+
+    def __init__(self):
+        self.communicator = _Communicator()
+
+
+class GPU(Device):
+    pass
+
+
+class Operations:
+    def __init__(self, sim):
+        self._sim = sim
+        self._integrator = None
+        self.computes = []
+
+    @property
+    def integrator(self):
+        return self._integrator
+
+    @integrator.setter
+    def integrator(self, value):
+        self._integrator = value
+        self._sim._attached = False
+
+
+class Simulation:
+    """Owns the state, the operations and the timestep counter."""
+
+    def __init__(self, device=None, seed=None):
+        self.device = device if device is not None else GPU()
+        self._seed = None if seed is None else int(seed) & 0xFFFF
+        self.timestep = 0
+        self.state = None
+        self.operations = Operations(self)
+        self._attached = False
+
+    @property
+    def seed(self):
+        return 0 if self._seed is None else self._seed
+
+    @seed.setter
+    def seed(self, v):
+        self._seed = int(v) & 0xFFFF
+
+    def _warn_if_seed_unset(self):
+        if self._seed is None:
+            warnings.warn("Simulation.seed is not set, using default seed=0", stacklevel=3)
+
+    def create_state_from_snapshot(self, snapshot):
+        if self.state is not None:
+            raise RuntimeError("state already created")
+        self.state = State(self, snapshot)
+        return self.state
+
+    def _attach(self):
+        if self.state is None:
+            raise RuntimeError("create the state before running")
+        integ = self.operations.integrator
+        if integ is not None:
+            integ._attach(self)
+        for c in self.operations.computes:
+            c._attach(self)
+        self._attached = True
+
+    def run(self, steps):
+        """Advance `steps` timesteps; forces are (re)computed at the start."""
+        if not self._attached:
+            self._attach()
+        integ = self.operations.integrator
+        if integ is not None:
+            integ._prepare_run(self.timestep)
+        for _ in range(int(steps)):
+            if integ is not None:
+                integ._update(self.timestep)
+            self.timestep += 1
+
+
+from . import md  # noqa: E402
+from .. import pair as _pair  # noqa: E402
+
+# ``hoomd.azplugins.pair`` spelling
+azplugins = _types.SimpleNamespace(pair=_pair)
+_ = np

@@ -1,0 +1,63 @@
+"""Force base class with HOOMD's per-particle output properties."""
+
+import numpy as np
+
+from ... import _azp
+
+torch = _azp.torch
+
+
+class Force:
+    def __init__(self):
+        self._attached = False
+        self._sim = None
+        self._force = None
+        self._virial = None
+        self._torque = None
+
+    def _allocate(self, N):
+        dev = torch.device("cuda")
+        self._pitch = max(64, (N + 63) // 64 * 64)
+        self._force = torch.zeros((max(N, 1), 4), dtype=torch.float64, device=dev)
+        self._virial = torch.zeros((6, self._pitch), dtype=torch.float64, device=dev)
+        self._torque = torch.zeros((max(N, 1), 4), dtype=torch.float64, device=dev)
+
+    def _outputs(self):
+        o = _azp.Outputs()
+        o.d_force = self._force.data_ptr()
+        o.d_virial = self._virial.data_ptr()
+        o.virial_pitch = self._pitch
+        o.d_torque = self._torque.data_ptr()
+        return o
+
+    def _require(self):
+        if not self._attached:
+            raise RuntimeError("the force is not attached to a running simulation")
+
+    @property
+    def forces(self):
+        self._require()
+        N = self._sim.state.N
+        return self._force[:N, :3].cpu().numpy()
+
+    @property
+    def energies(self):
+        self._require()
+        N = self._sim.state.N
+        return self._force[:N, 3].cpu().numpy()
+
+    @property
+    def energy(self):
+        return float(np.sum(self.energies))
+
+    @property
+    def virials(self):
+        self._require()
+        N = self._sim.state.N
+        return self._virial[:, :N].T.cpu().numpy()
+
+    @property
+    def torques(self):
+        self._require()
+        N = self._sim.state.N
+        return self._torque[:N, :3].cpu().numpy()

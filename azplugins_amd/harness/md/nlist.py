@@ -1,0 +1,61 @@
+"""Neighbor lists (``md.nlist.Cell``) backed by the library's cell list."""
+
+from ... import _azplugins
+
+
+class NeighborList:
+    """Verlet-buffered full neighbor list shared by the pair potentials.
+
+    ``buffer`` is r_buff: the list holds pairs within r_cut(ti,tj) + r_buff
+    and is rebuilt when a particle has moved more than r_buff / 2.
+    """
+
+    def __init__(self, buffer, exclusions=("bond",), rebuild_check_delay=1, check_dist=True):
+        self.buffer = float(buffer)
+        self.exclusions = tuple(exclusions)
+        self.rebuild_check_delay = int(rebuild_check_delay)
+        self.check_dist = bool(check_dist)
+        self._cpp = None
+        self._sim = None
+        self._users = []
+        self._last = None
+
+    def _attach(self, sim):
+        if self._cpp is None or self._sim is not sim:
+            self._cpp = _azplugins.NeighborListGPU(self.buffer)
+            self._sim = sim
+            self._users = []
+            self._last = None
+
+    def _register(self, pair):
+        if pair not in self._users:
+            self._users.append(pair)
+
+    @property
+    def handle(self):
+        return self._cpp.handle
+
+    def _r_cut_matrix(self):
+        nt = len(self._sim.state.types)
+        m = [0.0] * (nt * nt)
+        for u in self._users:
+            for k, v in enumerate(u._r_cut_matrix()):
+                m[k] = max(m[k], v)
+        return nt, m
+
+    def _update(self, timestep):
+        if self._last == timestep:
+            return
+        nt, m = self._r_cut_matrix()
+        self._cpp.set_r_buff(self.buffer)
+        self._cpp.set_r_cut(nt, m)
+        self._cpp.update(self._sim.state.particles())
+        self._last = timestep
+
+    @property
+    def num_builds(self):
+        return self._cpp.num_builds() if self._cpp is not None else 0
+
+
+class Cell(NeighborList):
+    """Cell-list-accelerated neighbor list (HOOMD's ``md.nlist.Cell``)."""

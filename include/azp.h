@@ -1,0 +1,332 @@
+/*
+ * azp.h — C ABI of the MI355X (gfx950) short-range pair-force library
+ * (azplugins_amd/lib/libazp.so).
+ *
+ * Plain C types only: device pointers, sizes and POD structs.  Every entry
+ * point returns AZP_OK (0) or an error code; azp_last_error() returns the
+ * message of the last failure on the calling thread.  No exception crosses the
+ * boundary.  Device arrays are owned by the caller unless an object (neighbor
+ * list, pair potential) says otherwise; the caller's device pointers need only
+ * be valid for the duration of the call.  All work is enqueued on the HIP
+ * stream passed as `stream` (NULL = the default stream).
+ *
+ * Particle layout in HBM (structure of arrays, N particles):
+ *   pos          double4[N]  x, y, z, type (type index stored as a double)
+ *   pos_fixed    int4[N]     x, y, z as wrapped 32-bit fixed point of the box
+ *                            (x / L * 2^32), type   (mixed precision only)
+ *   vel          double4[N]  vx, vy, vz, mass        (DPD)
+ *   tag          uint32[N]   particle tags           (DPD random stream)
+ *   orientation  double4[N]  quaternion s, x, y, z   (anisotropic)
+ *   force        double4[N]  fx, fy, fz, energy      (output, overwritten)
+ *   virial       double[6*pitch] xx, xy, xz, yy, yz, zz, component-major
+ *   torque       double4[N]  tx, ty, tz, 0           (anisotropic output)
+ *   neighbor list: n_neigh uint32[N]; nlist uint32[max_neighbors * pitch],
+ *                  entry k of particle i at nlist[k * pitch + i] (full list).
+ * The box is orthorhombic and periodic in x, y, z; positions in [-L/2, L/2).
+ */
+#ifndef AZP_H
+#define AZP_H
+
+#include <stdbool.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AZP_ABI_VERSION 1
+
+enum azp_status
+    {
+    AZP_OK = 0,
+    AZP_ERR_INVALID = 1,     /* bad argument */
+    AZP_ERR_HIP = 2,         /* HIP runtime error */
+    AZP_ERR_BOX = 3,         /* box too small for the interaction range */
+    AZP_ERR_UNSUPPORTED = 4, /* feature not available for this evaluator */
+    AZP_ERR_NOMEM = 5,       /* device allocation failed */
+    AZP_ERR_STATE = 6        /* object used before it was configured */
+    };
+
+/* Evaluators: the six azplugins functors on the pair path. */
+enum azp_evaluator
+    {
+    AZP_EVAL_HERTZ = 0,             /* PairEvaluatorHertz.h */
+    AZP_EVAL_EXPANDED_YUKAWA = 1,   /* PairEvaluatorExpandedYukawa.h */
+    AZP_EVAL_PERTURBED_LJ = 2,      /* PairEvaluatorPerturbedLennardJones.h */
+    AZP_EVAL_COLLOID = 3,           /* PairEvaluatorColloid.h */
+    AZP_EVAL_DPD_GENERAL_WEIGHT = 4, /* DPDPairEvaluatorGeneralWeight.h */
+    AZP_EVAL_TWO_PATCH_MORSE = 5    /* AnisoPairEvaluatorTwoPatchMorse.h */
+    };
+
+/* Force drivers (the upstream HOOMD templates the evaluators plug into). */
+enum azp_driver
+    {
+    AZP_DRIVER_PAIR = 0,         /* PotentialPair<Eval>[GPU] */
+    AZP_DRIVER_DPD_THERMO = 1,   /* PotentialPairDPDThermo<Eval>[GPU] */
+    AZP_DRIVER_CONSERVATIVE = 2, /* PotentialPairConservative<Eval> */
+    AZP_DRIVER_ANISO = 3         /* AnisoPotentialPair<Eval>[GPU] */
+    };
+
+enum azp_shift_mode
+    {
+    AZP_SHIFT_NONE = 0,
+    AZP_SHIFT_SHIFT = 1,
+    AZP_SHIFT_XPLOR = 2
+    };
+
+enum azp_precision
+    {
+    AZP_PREC_FP64 = 0, /* plain form: FP64 everywhere */
+    AZP_PREC_MIXED = 1 /* FP32 pair arithmetic, FP64 accumulation */
+    };
+
+typedef struct
+    {
+    double Lx, Ly, Lz;
+    } azp_box;
+
+/* ---- param_type PODs: same fields, order and alignment as the reference ---- */
+
+/* PairEvaluatorHertz.h:41-47 */
+typedef struct
+    {
+    double epsilon;
+    } __attribute__((aligned(8))) azp_param_hertz;
+
+/* PairEvaluatorExpandedYukawa.h:45-53 */
+typedef struct
+    {
+    double epsilon;
+    double kappa;
+    double delta;
+    } __attribute__((aligned(32))) azp_param_expanded_yukawa;
+
+/* PairEvaluatorPerturbedLennardJones.h:57-66 (stored derived fields) */
+typedef struct
+    {
+    double sigma_6;
+    double epsilon_x_4;
+    double attraction_scale_factor;
+    double rwcasq;
+    } __attribute__((aligned(32))) azp_param_perturbed_lj;
+
+/* PairEvaluatorColloid.h:48-57 */
+typedef struct
+    {
+    double A;
+    double a_1;
+    double a_2;
+    double sigma_3;
+    } __attribute__((aligned(32))) azp_param_colloid;
+
+/* DPDPairEvaluatorGeneralWeight.h:54-62 */
+typedef struct
+    {
+    double A;
+    double gamma;
+    double s;
+    } __attribute__((aligned(32))) azp_param_dpd_general_weight;
+
+/* AnisoPairEvaluatorTwoPatchMorse.h:63-68 */
+typedef struct
+    {
+    double M_d;
+    double M_rinv;
+    double r_eq;
+    double omega;
+    double alpha;
+    bool repulsion;
+    } azp_param_two_patch_morse;
+
+/* ---- particle data and outputs (device pointers) ---- */
+typedef struct
+    {
+    uint32_t N;
+    azp_box box;
+    const double* d_pos;
+    const int32_t* d_pos_fixed; /* optional (mixed); filled internally when NULL */
+    const double* d_vel;
+    const uint32_t* d_tag;
+    const double* d_orientation;
+    } azp_particles;
+
+typedef struct
+    {
+    double* d_force;
+    double* d_virial; /* may be NULL when compute_virial == 0 */
+    uint32_t virial_pitch;
+    double* d_torque;
+    } azp_outputs;
+
+/* ---- kernel launcher arguments (replaces HOOMD's pair_args_t,
+ * dpd_pair_args_t and a_pair_args_t) ---- */
+typedef struct
+    {
+    azp_particles particles;
+    azp_outputs out;
+    const uint32_t* d_n_neigh;
+    const uint32_t* d_nlist;
+    uint32_t nlist_pitch;
+    uint32_t max_neighbors;
+    const double* d_rcutsq; /* ntypes^2 */
+    const double* d_ronsq;  /* ntypes^2, xplor only (may be NULL otherwise) */
+    uint32_t ntypes;
+    int32_t shift_mode;
+    int32_t compute_virial;
+    int32_t precision;
+    uint32_t block_size; /* 0: library default / autotuned value */
+    /* DPD thermostat */
+    uint16_t seed;
+    uint64_t timestep;
+    double dt;
+    double kT;
+    void* stream;
+    } azp_pair_args;
+
+/* ---- library ---- */
+const char* azp_last_error(void);
+int azp_abi_version(void);
+
+/* ---- parameters: dict values in the reference's Python key order <-> POD ---- */
+int azp_param_nvalues(int evaluator);
+int azp_param_size(int evaluator);
+/* Replaces PairParameters<Name>(pybind11::dict) (e.g. PairEvaluatorPerturbedLennardJones.h:35-44). */
+int azp_param_pack(int evaluator, const double* values, void* pod);
+/* Replaces param_type::asDict()/toPython() (e.g. PairEvaluatorPerturbedLennardJones.h:47-54). */
+int azp_param_unpack(int evaluator, const void* pod, double* values);
+/* Registry of the reference's pybind11 class names (module.cc:142-150,171-179),
+ * e.g. "PotentialPairPerturbedLennardJonesGPU" -> (AZP_EVAL_PERTURBED_LJ, AZP_DRIVER_PAIR). */
+int azp_class_lookup(const char* cpp_class_name, int* evaluator, int* driver);
+
+/* ---- kernel launchers: replace the explicit template instantiations
+ * gpu_compute_pair_forces<Eval>        (PotentialPairGPUKernel.cu.inc:25-28),
+ * gpu_compute_dpd_forces<Eval>         (PotentialPairDPDThermoGPUKernel.cu.inc:21-24),
+ * gpu_compute_pair_aniso_forces<Eval>  (AnisoPotentialPairGPUKernel.cu.inc:21-25).
+ * d_params points to ntypes^2 PODs of the evaluator's param type in device memory. */
+int azp_gpu_compute_pair_forces(const azp_pair_args* args, int evaluator, const void* d_params);
+int azp_gpu_compute_dpd_forces(const azp_pair_args* args, const void* d_params);
+int azp_gpu_compute_pair_aniso_forces(const azp_pair_args* args, int evaluator, const void* d_params);
+/* pos (double4) -> pos_fixed (int4) for the mixed-precision form */
+int azp_gpu_pack_fixed(const double* d_pos, uint32_t N, const azp_box* box, int32_t* d_pos_fixed, void* stream);
+/* DPD random variates alpha_k = U(-1,1) for (tag_i[k], tag_j[k]) at (seed, timestep). */
+int azp_gpu_dpd_alpha(uint16_t seed,
+                      uint64_t timestep,
+                      uint64_t n,
+                      const uint32_t* d_tag_i,
+                      const uint32_t* d_tag_j,
+                      double* d_alpha,
+                      void* stream);
+
+/* ---- integration around the force path (replace the upstream
+ * TwoStepConstantVolume / TwoStepLangevin kernels reached from the reference
+ * tests, test_pair.py:325-326, test_pair_dpd.py:189-200) ----
+ * accel: double4[N] (ax, ay, az, 0), image: int4[N] or NULL. */
+int azp_gpu_integrate_vv_one(double* d_pos,
+                             double* d_vel,
+                             const double* d_accel,
+                             int32_t* d_image,
+                             uint32_t N,
+                             const azp_box* box,
+                             double dt,
+                             void* stream);
+/* a = (F [- gamma v + sqrt(6 gamma kT / dt) U(-1,1)]) / m;  v += dt a / 2 */
+int azp_gpu_integrate_vv_two(double* d_vel,
+                             double* d_accel,
+                             const double* d_force,
+                             const uint32_t* d_tag,
+                             uint32_t N,
+                             double dt,
+                             int langevin,
+                             double gamma,
+                             double kT,
+                             uint16_t seed,
+                             uint64_t timestep,
+                             void* stream);
+/* spherical tops: angmom double4[N] space-frame L, inertia double[N] (isotropic I) */
+int azp_gpu_integrate_rot_one(double* d_orientation,
+                              double* d_angmom,
+                              const double* d_torque,
+                              const double* d_inertia,
+                              uint32_t N,
+                              double dt,
+                              void* stream);
+int azp_gpu_integrate_rot_two(double* d_angmom,
+                              const double* d_torque,
+                              const double* d_inertia,
+                              const uint32_t* d_tag,
+                              uint32_t N,
+                              double dt,
+                              int langevin,
+                              double gamma_r,
+                              double kT,
+                              uint16_t seed,
+                              uint64_t timestep,
+                              void* stream);
+/* FP64 sums (deterministic order): out[10] = translational KE, rotational KE,
+ * potential energy, virial xx xy xz yy yz zz, N.  d_work holds
+ * azp_thermo_workspace_size() doubles. */
+int azp_thermo_workspace_size(void);
+int azp_gpu_thermo_reduce(const double* d_vel,
+                          const double* d_force,
+                          const double* d_virial,
+                          uint32_t virial_pitch,
+                          const double* d_angmom,
+                          const double* d_inertia,
+                          uint32_t N,
+                          double* d_work,
+                          double* d_out,
+                          void* stream);
+
+/* ---- neighbor list (replaces upstream CellListGPU + NeighborListGPUBinned;
+ * reached from the pair classes through `nlist`, pair.py:110-111) ---- */
+typedef struct azp_nlist azp_nlist;
+int azp_nlist_create(azp_nlist** out);
+int azp_nlist_destroy(azp_nlist* nl);
+int azp_nlist_set_r_buff(azp_nlist* nl, double r_buff);
+/* ntypes^2 host array, r_cut <= 0 excludes the type pair */
+int azp_nlist_set_r_cut(azp_nlist* nl, uint32_t ntypes, const double* r_cut);
+/* Rebuild when forced, when N/box/r_cut changed, or when a particle moved more
+ * than r_buff/2 since the last build; grows capacity and retries on overflow. */
+int azp_nlist_update(azp_nlist* nl, const azp_particles* p, int force_rebuild, int* rebuilt, void* stream);
+int azp_nlist_get(const azp_nlist* nl,
+                  const uint32_t** d_n_neigh,
+                  const uint32_t** d_nlist,
+                  uint32_t* pitch,
+                  uint32_t* max_neighbors);
+int azp_nlist_num_builds(const azp_nlist* nl, uint64_t* n);
+/* Copy n_neigh[N] and the list (max_neighbors * pitch entries) to host memory
+ * (synchronizes the device). */
+int azp_nlist_download(const azp_nlist* nl,
+                       uint32_t N,
+                       uint32_t* h_n_neigh,
+                       uint32_t* h_nlist,
+                       uint64_t nlist_capacity);
+
+/* ---- pair potential objects (replace PotentialPair<Name>GPU,
+ * PotentialPairDPDThermo<Name>GPU, AnisoPotentialPair<Name>GPU) ---- */
+typedef struct azp_pair azp_pair;
+int azp_pair_create(azp_pair** out, const char* cpp_class_name, uint32_t ntypes);
+int azp_pair_destroy(azp_pair* pp);
+int azp_pair_set_params(azp_pair* pp, uint32_t ti, uint32_t tj, const double* values);
+int azp_pair_get_params(const azp_pair* pp, uint32_t ti, uint32_t tj, double* values);
+int azp_pair_set_r_cut(azp_pair* pp, uint32_t ti, uint32_t tj, double r_cut);
+int azp_pair_set_r_on(azp_pair* pp, uint32_t ti, uint32_t tj, double r_on);
+int azp_pair_set_shift_mode(azp_pair* pp, int mode);
+int azp_pair_set_precision(azp_pair* pp, int precision);
+int azp_pair_set_thermostat(azp_pair* pp, uint16_t seed, double dt, double kT);
+/* r_cut matrix (ntypes^2) the neighbor list must cover */
+int azp_pair_get_r_cut(const azp_pair* pp, double* r_cut);
+int azp_pair_compute(azp_pair* pp,
+                     const azp_nlist* nl,
+                     const azp_particles* p,
+                     const azp_outputs* out,
+                     uint64_t timestep,
+                     int compute_virial,
+                     void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AZP_H */

@@ -1,0 +1,122 @@
+"""CPU-side checks of the C ABI: the library loads, exports every symbol the
+header declares, and its host-only entry points (parameter packing, class
+registry, argument validation) behave like the reference's param_type
+constructors, asDict() and class registration.  No kernel is launched."""
+
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle
+from azplugins_amd import _azp, _azplugins
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    with open(os.path.join(ROOT, "include", "azp.h")) as fh:
+        text = fh.read()
+    return sorted(set(re.findall(r"^(?:int|const char\*) (azp_[a-z0-9_]+)\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = _azp.lib()
+    names = _declared()
+    assert len(names) >= 30
+    for n in names:
+        assert hasattr(L, n), n
+    bound = {s[0] for s in _azp.SIGNATURES}
+    assert set(names) == bound, set(names) ^ bound
+    assert L.azp_abi_version() == 1
+
+
+KEYS = oracle.PARAM_KEYS
+
+CASES = {
+    0: [{"epsilon": 2.0}, {"epsilon": 0.0}],
+    1: [{"epsilon": 1.0, "kappa": 1.0, "delta": 3.0}],
+    2: [{"epsilon": 2.0, "sigma": s, "attraction_scale_factor": 0.5} for s in (1.05, 0.5, 0.85, 1.0)],
+    3: [{"A": 100.0, "a_1": 1.5, "a_2": 0.0, "sigma": s} for s in (2.0, 1.05)],
+    4: [{"A": 2.0, "gamma": 4.5, "s": 0.5}],
+    5: [{"M_d": 1.8341, "M_r": 0.0302, "r_eq": 1.0043, "omega": 5.0, "alpha": 0.4, "repulsion": False}],
+}
+
+
+@pytest.mark.parametrize("ev", sorted(CASES))
+def test_param_pack_matches_oracle_and_round_trips(ev):
+    size = _azp.lib().azp_param_size(ev)
+    for v in CASES[ev]:
+        vals = [float(v[k]) for k in KEYS[ev]]
+        pod = (ctypes.c_char * size)()
+        _azp.call("azp_param_pack", ev, _azp.dbl_array(vals), pod)
+        stored = np.frombuffer(bytes(pod)[: 8 * min(5, len(oracle.pack(ev, v)))], dtype=np.float64)
+        # stored fields equal the oracle's restatement of the param_type constructor
+        np.testing.assert_array_equal(stored, oracle.pack(ev, v)[: stored.shape[0]])
+        back = (ctypes.c_double * 8)()
+        _azp.call("azp_param_unpack", ev, pod, back)
+        got = list(back)[: len(vals)]
+        if ev == 5:
+            assert np.allclose(got, vals)  # 1/(1/M_r) is not exact (test_pair_aniso.py:147-149)
+        else:
+            assert got == vals  # exact, as test_pair.py:349 requires
+
+
+def test_class_registry():
+    ev = ctypes.c_int()
+    drv = ctypes.c_int()
+    for name, e, d in [
+        ("PotentialPairPerturbedLennardJonesGPU", 2, 0),
+        ("PotentialPairColloid", 3, 0),
+        ("PotentialPairDPDThermoGeneralWeightGPU", 4, 1),
+        ("PotentialPairConservativeGeneralWeight", 4, 2),
+        ("AnisoPotentialPairTwoPatchMorseGPU", 5, 3),
+    ]:
+        _azp.call("azp_class_lookup", name.encode(), ctypes.byref(ev), ctypes.byref(drv))
+        assert (ev.value, drv.value) == (e, d)
+    with pytest.raises(_azp.AzpError):
+        _azp.call("azp_class_lookup", b"PotentialPairLJ", ctypes.byref(ev), ctypes.byref(drv))
+
+
+def test_pair_object_host_side_validation():
+    pp = _azplugins.PotentialPairDPDThermoGeneralWeightGPU(2)
+    with pytest.raises(_azp.AzpError, match="mode 'none' only"):
+        pp.setShiftMode("shift")
+    with pytest.raises(_azp.AzpError, match="FP64 form only"):
+        pp.setPrecision("mixed")
+    with pytest.raises(_azp.AzpError, match="out of range"):
+        pp.setParams(2, 0, [1.0, 1.0, 1.0])
+    with pytest.raises(_azp.AzpError, match="not set"):
+        pp.getParams(0, 1)
+    pp.setParams(0, 1, [25.0, 4.5, 0.5])
+    assert pp.getParams(1, 0) == [25.0, 4.5, 0.5]  # symmetric
+    pa = _azplugins.AnisoPotentialPairTwoPatchMorseGPU(1)
+    with pytest.raises(_azp.AzpError):
+        pa.setShiftMode("xplor")
+    pc = _azplugins.PotentialPairColloidGPU(1)
+    with pytest.raises(_azp.AzpError, match="FP64 form only"):
+        pc.setPrecision("mixed")
+
+
+def test_harness_api_shape():
+    """Class names, constructor signatures and keys of hoomd.azplugins.pair."""
+    import inspect
+
+    from azplugins_amd import pair
+
+    assert sorted(pair.__all__) == ["Colloid", "DPDGeneralWeight", "ExpandedYukawa", "Hertz",
+                                    "PerturbedLennardJones", "TwoPatchMorse"]
+    sig = lambda c: list(inspect.signature(c.__init__).parameters)[1:]
+    assert sig(pair.Hertz) == ["nlist", "default_r_cut", "default_r_on", "mode"]
+    assert sig(pair.DPDGeneralWeight) == ["nlist", "kT", "default_r_cut"]
+    assert sig(pair.TwoPatchMorse) == ["nlist", "default_r_cut", "mode"]
+    assert pair.DPDGeneralWeight._cpp_class_name == "PotentialPairDPDThermoGeneralWeight"
+    import azplugins_amd.harness as hoomd
+
+    p = hoomd.azplugins.pair.TwoPatchMorse(nlist=hoomd.md.nlist.Cell(buffer=0.4), default_r_cut=2.0)
+    with pytest.raises(KeyError):
+        p.params[("A", "A")] = dict(M_d=1.0)
+    p.params[("A", "A")] = dict(M_d=1, M_r=0.2, r_eq=1.0, omega=20, alpha=0.4, repulsion=True)
+    assert p.params[("A", "A")]["repulsion"] is True

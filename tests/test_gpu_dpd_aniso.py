@@ -1,0 +1,191 @@
+"""GPU parity of the DPD thermostat and the anisotropic (TwoPatchMorse) path."""
+
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle
+from azplugins_amd import _azp
+from gpu_util import GpuSystem, gpu_pair, jittered_lattice, nlist_pairs, rel_err, say
+
+torch = _azp.torch
+pytestmark = pytest.mark.gpu
+
+DPD_PARAMS = {(0, 0): {"A": 25.0, "gamma": 4.5, "s": 0.5},
+              (0, 1): {"A": 40.0, "gamma": 4.5, "s": 0.5},
+              (1, 1): {"A": 25.0, "gamma": 3.0, "s": 2.0}}
+
+
+def test_dpd_alpha_bit_exact():
+    rng = np.random.default_rng(7)
+    n = 200000
+    ti = rng.integers(0, 2**32 - 1, size=n, dtype=np.uint64).astype(np.uint32)
+    tj = rng.integers(0, 2**32 - 1, size=n, dtype=np.uint64).astype(np.uint32)
+    for seed, t in [(0, 0), (12345, 777), (65535, 4294967295)]:
+        say(f"[alpha] seed={seed} t={t}")
+        dti = torch.tensor(ti.astype(np.int64).astype(np.int32), device="cuda")
+        dtj = torch.tensor(tj.astype(np.int64).astype(np.int32), device="cuda")
+        out = torch.zeros(n, dtype=torch.float64, device="cuda")
+        _azp.call("azp_gpu_dpd_alpha", seed, t, n, _azp.dptr(dti), _azp.dptr(dtj), _azp.dptr(out),
+                  _azp.stream_ptr())
+        torch.cuda.synchronize()
+        g = out.cpu().numpy()
+        o = oracle.dpd_alpha(seed, t, ti, tj)
+        assert np.array_equal(g.view(np.uint64), o.view(np.uint64))
+
+
+def _isolated_pairs(npairs, L, rc, seed):
+    """Pairs of particles at random separations < rc, pairs > 2 (rc + r_buff) apart."""
+    rng = np.random.default_rng(seed)
+    m = int(np.ceil(npairs ** (1 / 3)))
+    a = L / m
+    assert a > 2 * (rc + 0.4) + 1.0
+    g = (np.arange(m) + 0.5) * a - L / 2
+    x, y, z = np.meshgrid(g, g, g, indexing="ij")
+    c = np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1)[:npairs]
+    d = rng.normal(size=(npairs, 3))
+    d *= (rng.uniform(0.05, 0.999 * rc, size=npairs) / np.linalg.norm(d, axis=1))[:, None]
+    p = np.concatenate([c - 0.5 * d, c + 0.5 * d])
+    pos4 = np.zeros((2 * npairs, 4))
+    pos4[:, :3] = p - L * np.floor(p / L + 0.5)
+    pos4[:, 3] = rng.integers(0, 2, size=2 * npairs)
+    vel4 = np.zeros((2 * npairs, 4))
+    vel4[:, :3] = rng.normal(size=(2 * npairs, 3))
+    vel4[:, 3] = 1.0
+    tags = rng.permutation(10**6)[: 2 * npairs]
+    return pos4, vel4, tags
+
+
+def test_dpd_isolated_pairs_bit_exact():
+    """Each particle feels exactly one pair force: GPU == oracle bit for bit."""
+    L = 60.0
+    pos4, vel4, tags = _isolated_pairs(3000, L, 1.0, 11)
+    Lv = np.array([L, L, L])
+    seed, t, dt, kT = 4321, 98765, 0.02, 1.3
+    say("[dpd isolated]")
+    r = gpu_pair("PotentialPairDPDThermoGeneralWeight", GpuSystem(pos4, Lv, vel4, tag=tags), 2,
+                 DPD_PARAMS, 1.0, thermostat=(seed, dt, kT), timestep=t)
+    pairs = oracle.nlist(pos4, Lv, 2, oracle.r_listsq_matrix([1.0] * 4, 0.4), "brute")
+    assert len(pairs[0]) == 3000
+    fo, vo = oracle.dpd_forces(pos4, vel4, tags, Lv, pairs, 2, oracle.pack_matrix(4, DPD_PARAMS, 2),
+                               np.ones(4), seed, t, dt, kT, portable_pow=True)
+    assert np.array_equal(r.force.view(np.uint64), fo.view(np.uint64))
+    assert np.array_equal(r.virial.view(np.uint64), vo.view(np.uint64))
+    # the reference's libm pow differs from the shared portable pow by ulps only
+    fl, _ = oracle.dpd_forces(pos4, vel4, tags, Lv, pairs, 2, oracle.pack_matrix(4, DPD_PARAMS, 2),
+                              np.ones(4), seed, t, dt, kT, portable_pow=False)
+    assert rel_err(r.force, fl) < 1e-13
+
+
+def test_dpd_many_body():
+    L = 12.0
+    rng = np.random.default_rng(5)
+    N = int(3.0 * L**3)
+    pos4 = np.zeros((N, 4))
+    pos4[:, :3] = rng.uniform(-L / 2, L / 2, size=(N, 3))
+    pos4[:, 3] = rng.integers(0, 2, size=N)
+    vel4 = np.zeros((N, 4))
+    vel4[:, :3] = rng.normal(size=(N, 3))
+    vel4[:, 3] = 1.0
+    Lv = np.array([L, L, L])
+    seed, t, dt, kT = 3, 1000, 0.02, 1.0
+    say(f"[dpd many-body] N={N}")
+    r = gpu_pair("PotentialPairDPDThermoGeneralWeight", GpuSystem(pos4, Lv, vel4), 2, DPD_PARAMS, 1.0,
+                 thermostat=(seed, dt, kT), timestep=t)
+    pairs = oracle.nlist(pos4, Lv, 2, oracle.r_listsq_matrix([1.0] * 4, 0.4), "cell")
+    assert nlist_pairs(r.nl, N) == set(zip(pairs[0].tolist(), pairs[1].tolist()))
+    fo, vo = oracle.dpd_forces(pos4, vel4, np.arange(N), Lv, pairs, 2,
+                               oracle.pack_matrix(4, DPD_PARAMS, 2), np.ones(4), seed, t, dt, kT)
+    assert rel_err(r.force[:, :3], fo[:, :3]) < 1e-12
+    assert rel_err(r.force[:, 3], fo[:, 3]) < 1e-12
+    assert rel_err(r.virial, vo) < 1e-12
+    # random and dissipative pair forces cancel in the sum (Newton's third law)
+    assert np.abs(r.force[:, :3].sum(axis=0)).max() < 1e-9 * np.abs(r.force[:, :3]).max() * N
+
+
+def test_dpd_temperature():
+    """The reference's statistical test (test_pair_dpd.py:13-46) on the GPU."""
+    import azplugins_amd.harness as hoomd
+    from azplugins_amd.harness import testing
+
+    snap = testing.lattice_snapshot_factory(dimensions=3, n=10, a=0.6)
+    sim = testing.simulation_factory(snap, seed=17)
+    all_ = hoomd.filter.All()
+    sim.state.thermalize_particle_momenta(filter=all_, kT=1.5)
+    integrator = hoomd.md.Integrator(dt=0.01)
+    sim.operations.integrator = integrator
+    cell = hoomd.md.nlist.Cell(buffer=0.4)
+    dpd = hoomd.azplugins.pair.DPDGeneralWeight(nlist=cell, kT=1.5, default_r_cut=1.0)
+    dpd.params[("A", "A")] = dict(A=0.0, gamma=4.5, s=0.5)
+    integrator.forces.append(dpd)
+    nve = hoomd.md.methods.ConstantVolume(filter=all_)
+    integrator.methods.append(nve)
+    thermo = hoomd.md.compute.ThermodynamicQuantities(filter=all_)
+    sim.operations.computes.append(thermo)
+    say("[dpd temperature]")
+    sim.run(10)
+    kT = np.zeros(100)
+    for s in range(100):
+        kT[s] = thermo.kinetic_temperature
+        sim.run(1)
+    assert np.mean(kT) == pytest.approx(1.5, 0.1)
+
+
+TPM = {(0, 0): {"M_d": 1.0, "M_r": 0.2, "r_eq": 1.0, "omega": 20.0, "alpha": 0.4, "repulsion": True},
+       (0, 1): {"M_d": 1.5, "M_r": 0.3, "r_eq": 1.1, "omega": 5.0, "alpha": 0.3, "repulsion": False},
+       (1, 1): {"M_d": 0.8, "M_r": 0.25, "r_eq": 0.9, "omega": 10.0, "alpha": 0.5, "repulsion": True}}
+
+
+@pytest.mark.parametrize("mode", ["none", "shift"])
+def test_aniso_fp64_and_mixed(mode):
+    L = 17.0
+    pos4, rng = jittered_lattice(16, L, 0.2, 2, seed=21)
+    q = rng.normal(size=(pos4.shape[0], 4))
+    q /= np.linalg.norm(q, axis=1)[:, None]
+    Lv = np.array([L, L, L])
+    nt = 2
+    say(f"[aniso] mode={mode} N={pos4.shape[0]}")
+    rc = 2.5
+    pairs = oracle.nlist(pos4, Lv, nt, oracle.r_listsq_matrix([rc] * 4, 0.4), "cell")
+    fo, to, vo = oracle.aniso_forces(pos4, q, Lv, pairs, nt, oracle.pack_matrix(5, TPM, nt),
+                                     np.full(4, rc * rc), mode)
+    sysm = GpuSystem(pos4, Lv, quat=q)
+    r = gpu_pair("AnisoPotentialPairTwoPatchMorse", sysm, nt, TPM, rc, mode=mode)
+    assert nlist_pairs(r.nl, sysm.N) == set(zip(pairs[0].tolist(), pairs[1].tolist()))
+    errs = [rel_err(r.force[:, :3], fo[:, :3]), rel_err(r.force[:, 3], fo[:, 3]),
+            rel_err(r.torque[:, :3], to[:, :3]), rel_err(r.virial, vo)]
+    say("  fp64 rel err force/energy/torque/virial", ["%.2e" % e for e in errs])
+    assert max(errs) < 1e-12
+    rm = gpu_pair("AnisoPotentialPairTwoPatchMorse", sysm, nt, TPM, rc, mode=mode, precision="mixed")
+    errs = [rel_err(rm.force[:, :3], fo[:, :3]), rel_err(rm.force[:, 3], fo[:, 3]),
+            rel_err(rm.torque[:, :3], to[:, :3]), rel_err(rm.virial, vo)]
+    say("  mixed rel err force/energy/torque/virial", ["%.2e" % e for e in errs])
+    assert max(errs) < 1e-5
+
+
+def test_launcher_entry_point_matches_object_path():
+    """azp_gpu_compute_pair_forces with raw device pointers (the kernel ABI)."""
+    from gpu_util import raw_args
+
+    L = 15.0
+    pos4, _ = jittered_lattice(14, L, 0.2, 1, seed=31)
+    Lv = np.array([L, L, L])
+    params = {(0, 0): {"epsilon": 1.0, "sigma": 1.0, "attraction_scale_factor": 0.5}}
+    sysm = GpuSystem(pos4, Lv)
+    ref = gpu_pair("PotentialPairPerturbedLennardJones", sysm, 1, params, 3.0)
+    pod = (ctypes.c_char * 32)()
+    _azp.call("azp_param_pack", 2, _azp.dbl_array([1.0, 1.0, 0.5]), pod)
+    d_params = torch.tensor(np.frombuffer(bytes(pod), dtype=np.uint8).copy(), device="cuda")
+    rcsq = torch.tensor([9.0], dtype=torch.float64, device="cuda")
+    N = sysm.N
+    pitch = (N + 63) // 64 * 64
+    for block in (64, 128, 256, 512):
+        f = torch.zeros((N, 4), dtype=torch.float64, device="cuda")
+        v = torch.zeros((6, pitch), dtype=torch.float64, device="cuda")
+        a = raw_args(sysm, ref.nl, rcsq, None, 1, f, v, pitch, block=block)
+        say(f"[launcher] block={block}")
+        _azp.call("azp_gpu_compute_pair_forces", ctypes.byref(a), 2, _azp.dptr(d_params))
+        torch.cuda.synchronize()
+        assert np.array_equal(f.cpu().numpy(), ref.force)
+        assert np.array_equal(v[:, :N].cpu().numpy(), ref.virial)
