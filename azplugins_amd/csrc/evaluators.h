@@ -10,6 +10,8 @@
 // for R = float the same expressions run on the FP32 hardware units.
 #pragma once
 
+#include <type_traits>
+
 #include "device_common.h"
 
 namespace azp
@@ -70,8 +72,11 @@ struct EvalHertz
 struct EvalExpandedYukawa
     {
     using param_type = azp_param_expanded_yukawa;
+    // r - delta cancels when r ~ delta; the FP32 form needs r^2 in FP64
+    static constexpr bool kNeedsRsq64 = true;
     template<typename R> struct Prep
         {
+        double delta_sq;
         R epsilon;
         R kappa;
         R delta;
@@ -81,6 +86,7 @@ struct EvalExpandedYukawa
     __device__ static Prep<R> prepare(const param_type& p, double rcutsq, bool shift)
         {
         Prep<R> q;
+        q.delta_sq = p.delta * p.delta;
         q.epsilon = (R)p.epsilon;
         q.kappa = (R)p.kappa;
         q.delta = (R)p.delta;
@@ -107,6 +113,26 @@ struct EvalExpandedYukawa
         fdivr = f_div(e * (q.kappa + r_delta_inv), r);
         e -= q.eshift;
         }
+    // FP32 form with r - delta = (r^2 - delta^2) / (r + delta), r^2 - delta^2 in FP64
+    __device__ static void compute_precise(const Prep<float>& q, float rsq, double rsq64, float& fdivr, float& e)
+        {
+        const float r = f_sqrt(rsq);
+        const float r_delta = (float)(rsq64 - q.delta_sq) * f_rcp(r + q.delta);
+        const float r_delta_inv = f_rcp(r_delta);
+        e = q.epsilon * f_exp(-q.kappa * r_delta) * r_delta_inv;
+        fdivr = e * (q.kappa + r_delta_inv) * f_rcp(r);
+        e -= q.eshift;
+        }
+    };
+
+// Evaluators that do not need FP64 r^2 in their FP32 form.
+template<class Eval, typename = void> struct NeedsRsq64
+    {
+    static constexpr bool value = false;
+    };
+template<class Eval> struct NeedsRsq64<Eval, std::void_t<decltype(Eval::kNeedsRsq64)>>
+    {
+    static constexpr bool value = Eval::kNeedsRsq64;
     };
 
 // ---------------------------------------------------------------------------

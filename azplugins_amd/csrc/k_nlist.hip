@@ -165,11 +165,12 @@ __global__ void __launch_bounds__(256) k_walk(const WalkArgs a)
         return;
     const double4 pi = a.pos[i];
     const uint32_t ti = checked_type(pi.w, a.ntypes);
-    if (ti == a.ntypes)
+    const bool finite = isfinite(pi.x) && isfinite(pi.y) && isfinite(pi.z);
+    if (ti == a.ntypes || !finite)
         {
-        // no neighbors for a particle whose type has no parameters
+        // no neighbors for a particle without a valid type or position; reported to the host
         a.n_neigh[i] = 0;
-        atomicOr(a.overflow + 2, 1u);
+        atomicOr(a.overflow + 2, finite ? 1u : 2u);
         return;
         }
     const uint32_t c = a.cell_of[i];
@@ -249,6 +250,11 @@ __global__ void k_displacement(const double4* __restrict__ pos,
         return;
     const double4 p = pos[i];
     const double4 q = last[i];
+    if (!(isfinite(p.x) && isfinite(p.y) && isfinite(p.z)))
+        {
+        atomicOr(flag, 2u);
+        return;
+        }
     const double dx = min_image_1d(p.x - q.x, box.L[0], box.Linv[0]);
     const double dy = min_image_1d(p.y - q.y, box.L[1], box.Linv[1]);
     const double dz = min_image_1d(p.z - q.z, box.L[2], box.Linv[2]);
@@ -389,7 +395,9 @@ static int nlist_build(azp_nlist* nl, const azp_particles* p, hipStream_t s)
         AZP_LAUNCH_CHECK();
         AZP_HIP(hipMemcpyAsync(nl->h_flags, nl->flags.ptr, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         AZP_HIP(hipStreamSynchronize(s));
-        if (nl->h_flags[2])
+        if (nl->h_flags[2] & 2u)
+            return set_error(AZP_ERR_INVALID, "a particle position is not finite (NaN or inf)");
+        if (nl->h_flags[2] & 1u)
             return set_error(AZP_ERR_INVALID,
                              "a particle type index is outside [0, %u) (pos.w must hold the type)",
                              nl->ntypes);
@@ -504,6 +512,8 @@ extern "C" int azp_nlist_update(azp_nlist* nl, const azp_particles* p, int force
         AZP_LAUNCH_CHECK();
         AZP_HIP(hipMemcpyAsync(nl->h_flags + 1, nl->flags.ptr + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         AZP_HIP(hipStreamSynchronize(s));
+        if (nl->h_flags[1] & 2u)
+            return set_error(AZP_ERR_INVALID, "a particle position is not finite (NaN or inf)");
         need = nl->h_flags[1] != 0;
         }
     nl->params_dirty = false;

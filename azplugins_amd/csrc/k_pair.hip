@@ -33,7 +33,8 @@ struct PairKernelArgs
     const int4* pos_fixed;
     uint32_t N;
     DevBox box;
-    float fx_scale[3]; // L / 2^32 per axis
+    float fx_scale[3];    // L / 2^32 per axis
+    double fx_scale64[3]; // the same in FP64
     const uint32_t* n_neigh;
     const uint32_t* nlist;
     uint32_t nlist_pitch;
@@ -145,11 +146,95 @@ __device__ __forceinline__ double exact_rsq(const PairKernelArgs& a, uint32_t i,
     return rsq3_nofma(dx, dy, dz);
     }
 
+// Neighbor entries processed per batch: all list indices of a batch are
+// loaded first, then all gathers are issued, then the pair terms are
+// evaluated, so each lane keeps UNROLL gathers in flight instead of one.
+template<bool MIXED> struct Unroll
+    {
+    static constexpr uint32_t value = MIXED ? 8 : 4;
+    };
+
+// Branchless evaluation of one list entry: lanes whose entry is padding or
+// outside the cutoff evaluate at a safe distance and contribute zero.
+template<class Eval, bool VIRIAL>
+__device__ __forceinline__ void term_fp64(const PairKernelArgs& a,
+                                          const typename Eval::template Prep<double>& q,
+                                          const TypePairExtra<double>& x,
+                                          bool valid,
+                                          double dx,
+                                          double dy,
+                                          double dz,
+                                          bool xplor,
+                                          Acc& acc)
+    {
+    const double rsq = dx * dx + dy * dy + dz * dz;
+    const bool in = valid && rsq < x.rcsq64 && Eval::active(q);
+    double fdivr, e;
+    Eval::compute(q, in ? rsq : x.rcsq64, fdivr, e);
+    if (xplor)
+        apply_xplor<double>(rsq, x.rcsq64, x.ronsq, fdivr, e);
+    fdivr = in ? fdivr : 0.0;
+    e = in ? e : 0.0;
+    accumulate<VIRIAL>(acc, fdivr, dx, dy, dz, e);
+    }
+
+template<class Eval, bool VIRIAL>
+__device__ __forceinline__ void term_mixed(const PairKernelArgs& a,
+                                           const typename Eval::template Prep<float>& q,
+                                           const TypePairExtra<float>& x,
+                                           bool valid,
+                                           uint32_t i,
+                                           uint32_t j,
+                                           int32_t ix,
+                                           int32_t iy,
+                                           int32_t iz,
+                                           bool xplor,
+                                           Acc& acc)
+    {
+    const float dx = (float)ix * a.fx_scale[0];
+    const float dy = (float)iy * a.fx_scale[1];
+    const float dz = (float)iz * a.fx_scale[2];
+    float rsq = dx * dx + dy * dy + dz * dz;
+    bool in = valid && rsq < x.rcsq_lo;
+    double r64 = -1.0;
+    if (valid && rsq >= x.rcsq_lo && rsq < x.rcsq_hi)
+        {
+        // FP32 cannot decide this one: apply the FP64 cutoff test
+        r64 = exact_rsq(a, i, j);
+        in = r64 < x.rcsq64;
+        rsq = (float)r64;
+        }
+    in = in && Eval::active(q);
+    const float rs = in ? rsq : x.rcsq;
+    float fdivr, e;
+    if constexpr (NeedsRsq64<Eval>::value)
+        {
+        if (r64 < 0.0)
+            {
+            const double ex = (double)ix * a.fx_scale64[0];
+            const double ey = (double)iy * a.fx_scale64[1];
+            const double ez = (double)iz * a.fx_scale64[2];
+            r64 = ex * ex + ey * ey + ez * ez;
+            }
+        Eval::compute_precise(q, rs, in ? r64 : x.rcsq64, fdivr, e);
+        }
+    else
+        {
+        Eval::compute(q, rs, fdivr, e);
+        }
+    if (xplor)
+        apply_xplor<float>(rs, x.rcsq, x.ronsq, fdivr, e);
+    fdivr = in ? fdivr : 0.0f;
+    e = in ? e : 0.0f;
+    accumulate<VIRIAL>(acc, (double)fdivr, (double)dx, (double)dy, (double)dz, (double)e);
+    }
+
 template<class Eval, bool MIXED, bool VIRIAL, bool SINGLE>
 __global__ void __launch_bounds__(512) k_pair(const PairKernelArgs a)
     {
     using R = std::conditional_t<MIXED, float, double>;
     using Prep = typename Eval::template Prep<R>;
+    constexpr uint32_t U = Unroll<MIXED>::value;
     extern __shared__ double smem[];
     const uint32_t nt = a.ntypes;
     const uint32_t nt2 = nt * nt;
@@ -176,32 +261,32 @@ __global__ void __launch_bounds__(512) k_pair(const PairKernelArgs a)
     Acc acc;
     const uint32_t nn = a.n_neigh[i];
     const bool xplor = a.shift_mode == AZP_SHIFT_XPLOR;
+    const uint32_t* col = a.nlist + i;
+    const size_t stride = a.nlist_pitch;
 
     if constexpr (!MIXED)
         {
         const double4 pi = a.pos[i];
         const uint32_t ti = (uint32_t)pi.w;
-        for (uint32_t k = 0; k < nn; ++k)
+        for (uint32_t k = 0; k < nn; k += U)
             {
-            const uint32_t j = a.nlist[(size_t)k * a.nlist_pitch + i];
-            const double4 pj = a.pos[j];
-            const double dx = min_image_1d(pi.x - pj.x, a.box.L[0], a.box.Linv[0]);
-            const double dy = min_image_1d(pi.y - pj.y, a.box.L[1], a.box.Linv[1]);
-            const double dz = min_image_1d(pi.z - pj.z, a.box.L[2], a.box.Linv[2]);
-            const double rsq = dx * dx + dy * dy + dz * dz;
-            const uint32_t tp = SINGLE ? 0u : ti * nt + (uint32_t)pj.w;
-            const double rcsq = SINGLE ? x0.rcsq64 : s_x[tp].rcsq64;
-            if (rsq < rcsq)
+            uint32_t jj[U];
+#pragma unroll
+            for (uint32_t u = 0; u < U; ++u)
+                jj[u] = (k + u < nn) ? col[(size_t)(k + u) * stride] : i;
+            double4 pj[U];
+#pragma unroll
+            for (uint32_t u = 0; u < U; ++u)
+                pj[u] = a.pos[jj[u]];
+#pragma unroll
+            for (uint32_t u = 0; u < U; ++u)
                 {
-                const Prep& q = SINGLE ? q0 : s_prep[tp];
-                if (Eval::active(q))
-                    {
-                    double fdivr, e;
-                    Eval::compute(q, rsq, fdivr, e);
-                    if (xplor)
-                        apply_xplor<double>(rsq, rcsq, SINGLE ? x0.ronsq : s_x[tp].ronsq, fdivr, e);
-                    accumulate<VIRIAL>(acc, fdivr, dx, dy, dz, e);
-                    }
+                const double dx = min_image_1d(pi.x - pj[u].x, a.box.L[0], a.box.Linv[0]);
+                const double dy = min_image_1d(pi.y - pj[u].y, a.box.L[1], a.box.Linv[1]);
+                const double dz = min_image_1d(pi.z - pj[u].z, a.box.L[2], a.box.Linv[2]);
+                const uint32_t tp = SINGLE ? 0u : ti * nt + (uint32_t)pj[u].w;
+                term_fp64<Eval, VIRIAL>(a, SINGLE ? q0 : s_prep[tp], SINGLE ? x0 : s_x[tp], k + u < nn,
+                                        dx, dy, dz, xplor, acc);
                 }
             }
         }
@@ -209,35 +294,23 @@ __global__ void __launch_bounds__(512) k_pair(const PairKernelArgs a)
         {
         const int4 pi = a.pos_fixed[i];
         const uint32_t ti = (uint32_t)pi.w;
-        const float sx = a.fx_scale[0], sy = a.fx_scale[1], sz = a.fx_scale[2];
-        for (uint32_t k = 0; k < nn; ++k)
+        for (uint32_t k = 0; k < nn; k += U)
             {
-            const uint32_t j = a.nlist[(size_t)k * a.nlist_pitch + i];
-            const int4 pj = a.pos_fixed[j];
-            const float dx = (float)fixed_delta(pi.x, pj.x) * sx;
-            const float dy = (float)fixed_delta(pi.y, pj.y) * sy;
-            const float dz = (float)fixed_delta(pi.z, pj.z) * sz;
-            float rsq = dx * dx + dy * dy + dz * dz;
-            const uint32_t tp = SINGLE ? 0u : ti * nt + (uint32_t)pj.w;
-            const TypePairExtra<R>& x = SINGLE ? x0 : s_x[tp];
-            if (rsq < x.rcsq_hi)
+            uint32_t jj[U];
+#pragma unroll
+            for (uint32_t u = 0; u < U; ++u)
+                jj[u] = (k + u < nn) ? col[(size_t)(k + u) * stride] : i;
+            int4 pj[U];
+#pragma unroll
+            for (uint32_t u = 0; u < U; ++u)
+                pj[u] = a.pos_fixed[jj[u]];
+#pragma unroll
+            for (uint32_t u = 0; u < U; ++u)
                 {
-                bool in = rsq < x.rcsq_lo;
-                if (!in)
-                    {
-                    const double r64 = exact_rsq(a, i, j);
-                    in = r64 < x.rcsq64;
-                    rsq = (float)r64;
-                    }
-                const Prep& q = SINGLE ? q0 : s_prep[tp];
-                if (in && Eval::active(q))
-                    {
-                    float fdivr, e;
-                    Eval::compute(q, rsq, fdivr, e);
-                    if (xplor)
-                        apply_xplor<float>(rsq, x.rcsq, x.ronsq, fdivr, e);
-                    accumulate<VIRIAL>(acc, (double)fdivr, (double)dx, (double)dy, (double)dz, (double)e);
-                    }
+                const uint32_t tp = SINGLE ? 0u : ti * nt + (uint32_t)pj[u].w;
+                term_mixed<Eval, VIRIAL>(a, SINGLE ? q0 : s_prep[tp], SINGLE ? x0 : s_x[tp], k + u < nn, i,
+                                         jj[u], fixed_delta(pi.x, pj[u].x), fixed_delta(pi.y, pj[u].y),
+                                         fixed_delta(pi.z, pj[u].z), xplor, acc);
                 }
             }
         }
@@ -360,7 +433,10 @@ extern "C" int azp_gpu_compute_pair_forces(const azp_pair_args* args, int evalua
     ka.N = p.N;
     ka.box = make_box(p.box);
     for (int k = 0; k < 3; ++k)
-        ka.fx_scale[k] = (float)(ka.box.L[k] / 4294967296.0);
+        {
+        ka.fx_scale64[k] = ka.box.L[k] / 4294967296.0;
+        ka.fx_scale[k] = (float)ka.fx_scale64[k];
+        }
     ka.n_neigh = args->d_n_neigh;
     ka.nlist = args->d_nlist;
     ka.nlist_pitch = args->nlist_pitch;

@@ -28,6 +28,9 @@ class Method:
         st = self._sim.state
         dt = integ.dt if dt is None else dt
         lang, gamma, gamma_r, kT = self._langevin(timestep)
+        if dt == 0.0:
+            # accelerations of the current forces only (no thermostat terms)
+            lang = 0
         seed = self._sim.seed
         _azp.call("azp_gpu_integrate_vv_two", _azp.dptr(st.vel), _azp.dptr(st.accel),
                   _azp.dptr(integ._net_force), _azp.dptr(st.tag), st.N, dt, lang, gamma, kT, seed,

@@ -1,0 +1,353 @@
+"""Benchmark of the pair-force hot path on one MI355X.
+
+Workload (plj_1m): PerturbedLennardJones, N = 1,048,576 particles on a
+jittered FCC 64^3 lattice at density 0.85 (L = 107.249), epsilon = sigma = 1,
+attraction_scale_factor = 0.5, r_cut = 3.0, r_buff = 0.4, mode none, one
+particle type.  Particles are in Morton cell order.
+
+A step is one force evaluation of the whole system in the mixed form
+(positions -> 32-bit fixed point, then the pair kernel: FP32 pair arithmetic,
+FP64 accumulation of force, energy and virial), with the neighbor list built
+beforehand (it is rebuilt every ~10-20 MD steps, not every evaluation).  The
+line also carries the plain FP64 form, the timesteps/s of full Langevin MD
+steps (kT = 1.2, gamma = 1, dt = 0.005, rebuild checks and rebuilds included),
+the mixed form's force error against the FP64 form, the roofline of the pair
+kernel and the CPU baseline (the FP64 oracle, one thread).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-tps]
+"""
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "pair interactions/s & TPS at 1M particles, mixed vs FP64 (force rel err ≤1e-5)"
+HBM_PEAK_GBS = 8000.0
+# bytes per particle besides the list: pos 16 (fixed) + n_neigh 4 + head 4 + force 32 + virial 48
+MIXED_BYTES_PER_PARTICLE = 104
+FP64_BYTES_PER_PARTICLE = 124
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tps", action="store_true")
+    ap.add_argument("--tps-steps", type=int, default=200)
+    ap.add_argument("--cpu-cells", type=int, default=40, help="FCC cells per side of the CPU sample")
+    ap.add_argument("--sweep-blocks", action="store_true", help="time the pair kernels per block size")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 1 GiB device copy (HBM counter calibration) and exit")
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                          "(multi-GPU sharding is designed but not built)"}))
+        sys.exit(2)
+
+    from azplugins_amd import _azp, _azplugins, configs
+
+    torch = _azp.torch
+    dev = torch.device("cuda:0")
+    if args.calibrate_copy:
+        # a plain copy kernel of known size: reads and writes 2^30 bytes each
+        src = torch.ones(2**27, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("bench: copy calibration, 1073741824 bytes read + written per launch")
+        return
+    c = configs.plj_1m()
+    pos4, L = c["pos4"], c["L"]
+    N = pos4.shape[0]
+    log(f"bench: plj_1m N={N} L={L[0]:.4f}")
+    pos = torch.tensor(pos4, device=dev)
+    pos_fixed = torch.zeros((N, 4), dtype=torch.int32, device=dev)
+    parts = _azp.Particles(N, _azp.Box(*L), pos.data_ptr(), pos_fixed.data_ptr(), None, None, None)
+    box = _azp.Box(*L)
+    stream = _azp.stream_ptr()
+
+    pp = _azplugins.PotentialPairPerturbedLennardJonesGPU(1)
+    pp.setParams(0, 0, [1.0, 1.0, 0.5])
+    pp.setRCut(0, 0, 3.0)
+    nl = _azplugins.NeighborListGPU(0.4)
+    nl.set_r_cut(1, pp.getRCutMatrix())
+    t0 = time.perf_counter()
+    nl.update(parts, force_rebuild=True)
+    torch.cuda.synchronize()
+    t_nlist = time.perf_counter() - t0
+    d_nn, d_list, pitch, mx = nl.get()
+    nn_host, _ = nl.download(N)
+    nlist_entries = int(nn_host.sum())
+
+    # device params / cutoffs for the launcher ABI
+    pod = (ctypes.c_char * 32)()
+    _azp.call("azp_param_pack", 2, _azp.dbl_array([1.0, 1.0, 0.5]), pod)
+    d_params = torch.tensor(np.frombuffer(bytes(pod), dtype=np.uint8).copy(), device=dev)
+    rcsq = torch.tensor([9.0], dtype=torch.float64, device=dev)
+    vpitch = (N + 63) // 64 * 64
+    force = torch.zeros((N, 4), dtype=torch.float64, device=dev)
+    virial = torch.zeros((6, vpitch), dtype=torch.float64, device=dev)
+
+    def make_args(precision):
+        a = _azp.PairArgs()
+        a.particles = parts
+        a.out.d_force = force.data_ptr()
+        a.out.d_virial = virial.data_ptr()
+        a.out.virial_pitch = vpitch
+        a.out.d_torque = None
+        a.d_n_neigh = d_nn
+        a.d_nlist = d_list
+        a.nlist_pitch = pitch
+        a.max_neighbors = mx
+        a.d_rcutsq = rcsq.data_ptr()
+        a.d_ronsq = None
+        a.ntypes = 1
+        a.shift_mode = 0
+        a.compute_virial = 1
+        a.precision = precision
+        a.block_size = 0
+        a.stream = stream
+        return a
+
+    args_mixed = make_args(1)
+    args_fp64 = make_args(0)
+
+    def step_mixed():
+        _azp.call("azp_gpu_pack_fixed", _azp.dptr(pos), N, ctypes.byref(box), _azp.dptr(pos_fixed), stream)
+        _azp.call("azp_gpu_compute_pair_forces", ctypes.byref(args_mixed), 2, _azp.dptr(d_params))
+
+    def step_fp64():
+        _azp.call("azp_gpu_compute_pair_forces", ctypes.byref(args_fp64), 2, _azp.dptr(d_params))
+
+    def time_steps(fn, steps, warmup, kernel_events=False):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps
+
+    def time_kernel(precision, reps):
+        """Average duration of the pair kernel alone, HIP events on its stream."""
+        a = args_mixed if precision == 1 else args_fp64
+        starts = [torch.cuda.Event(enable_timing=True) for _ in range(reps)]
+        ends = [torch.cuda.Event(enable_timing=True) for _ in range(reps)]
+        for k in range(reps):
+            starts[k].record()
+            _azp.call("azp_gpu_compute_pair_forces", ctypes.byref(a), 2, _azp.dptr(d_params))
+            ends[k].record()
+        torch.cuda.synchronize()
+        return float(np.mean([s.elapsed_time(e) for s, e in zip(starts, ends)])) / 1e3
+
+    # reference results (FP64 form) for the error of the mixed form
+    step_fp64()
+    torch.cuda.synchronize()
+    f64 = force.clone()
+    v64 = virial.clone()
+    step_mixed()
+    torch.cuda.synchronize()
+    df = (force[:, :3] - f64[:, :3]).abs().max().item() / f64[:, :3].abs().max().item()
+    de = (force[:, 3] - f64[:, 3]).abs().max().item() / f64[:, 3].abs().max().item()
+    dv = (virial - v64).abs().max().item() / v64.abs().max().item()
+    l2 = ((force[:, :3] - f64[:, :3]).norm() / f64[:, :3].norm()).item()
+    log(f"bench: mixed vs fp64 force rel err {df:.2e} (l2 {l2:.2e}) energy {de:.2e} virial {dv:.2e}")
+
+    # count of unordered pairs inside r_cut (the unit of work)
+    p64 = pos
+    pairs_in_cut = count_pairs_in_cut(torch, p64, nn_host, nl, N, L, 9.0)
+    log(f"bench: nlist entries {nlist_entries} ({nlist_entries / N:.1f}/particle), "
+        f"pairs in cut {pairs_in_cut} ({2 * pairs_in_cut / N:.1f}/particle), nlist build {t_nlist * 1e3:.1f} ms")
+
+    if args.sweep_blocks:
+        for prec in (1, 0):
+            a = args_mixed if prec == 1 else args_fp64
+            for b in (64, 128, 256, 512):
+                a.block_size = b
+                time_kernel(prec, 3)
+                log(f"bench: sweep precision={'mixed' if prec else 'fp64'} block={b} "
+                    f"kernel {time_kernel(prec, 10) * 1e6:.1f} us")
+            a.block_size = 0
+
+    t_mixed = time_steps(step_mixed, args.steps, args.warmup)
+    t_fp64 = time_steps(step_fp64, args.steps, args.warmup)
+    k_mixed = time_kernel(1, max(10, args.steps))
+    k_fp64 = time_kernel(0, max(10, args.steps))
+    log(f"bench: step mixed {t_mixed * 1e6:.1f} us, fp64 {t_fp64 * 1e6:.1f} us; "
+        f"kernel mixed {k_mixed * 1e6:.1f} us, fp64 {k_fp64 * 1e6:.1f} us")
+
+    bytes_mixed = N * MIXED_BYTES_PER_PARTICLE + 4 * nlist_entries
+    bytes_fp64 = N * FP64_BYTES_PER_PARTICLE + 4 * nlist_entries
+    achieved = bytes_mixed / k_mixed / 1e9
+    traffic = load_traffic()
+    roofline = {
+        "bound": "hbm",
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": traffic,
+        "kernel": "k_pair<EvalPerturbedLJ, MIXED, VIRIAL, SINGLE>",
+        "kernel_us": round(k_mixed * 1e6, 2),
+        "algorithmic_bytes": bytes_mixed,
+    }
+
+    out = {
+        "metric": METRIC,
+        "value": pairs_in_cut / t_mixed,
+        "unit": "pair interactions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_mixed * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 pair arithmetic, f64 accumulation",
+        "data": "synthetic",
+        "config": {
+            "workload": "plj_1m",
+            "potential": "PerturbedLennardJones",
+            "N": N,
+            "density": 0.85,
+            "r_cut": 3.0,
+            "r_buff": 0.4,
+            "pairs_in_cut": pairs_in_cut,
+            "nlist_entries": nlist_entries,
+            "precision": "mixed",
+        },
+        "roofline": roofline,
+        "fp64": {
+            "value": pairs_in_cut / t_fp64,
+            "ms_per_step": t_fp64 * 1e3,
+            "kernel_us": round(k_fp64 * 1e6, 2),
+            "achieved_GBs": round(bytes_fp64 / k_fp64 / 1e9, 1),
+        },
+        "mixed_over_fp64": t_fp64 / t_mixed,
+        "max_rel_force_err_vs_fp64": df,
+        "l2_rel_force_err_vs_fp64": l2,
+        "nlist_build_ms": t_nlist * 1e3,
+    }
+    if not args.no_tps:
+        out["tps"] = run_tps(c, args.tps_steps)
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_cells)
+    print(json.dumps(out))
+
+
+def count_pairs_in_cut(torch, pos, nn_host, nl, N, L, rcutsq):
+    """Unordered pairs with r < r_cut, counted on the GPU from the full list."""
+    _, lst = nl.download(N)
+    dev = pos.device
+    total = 0
+    chunk = 16
+    nn_t = torch.tensor(nn_host.astype(np.int64), device=dev)
+    Lt = torch.tensor(L, device=dev)
+    for k0 in range(0, lst.shape[0], chunk):
+        rows = torch.tensor(lst[k0:k0 + chunk, :N].astype(np.int64), device=dev)
+        valid = (torch.arange(k0, k0 + rows.shape[0], device=dev)[:, None] < nn_t[None, :])
+        pj = pos[rows.clamp(max=N - 1), :3]
+        d = pos[None, :, :3] - pj
+        d = d - Lt * torch.round(d / Lt)
+        rsq = (d * d).sum(-1)
+        total += int(((rsq < rcutsq) & valid).sum().item())
+    return total // 2
+
+
+def run_tps(c, steps):
+    """Timesteps/s of full Langevin MD steps through the drop-in API."""
+    import azplugins_amd.harness as hoomd
+    from azplugins_amd import _azp
+
+    torch = _azp.torch
+    snap = hoomd.Snapshot()
+    N = c["pos4"].shape[0]
+    snap.configuration.box = [*c["L"], 0, 0, 0]
+    snap.particles.N = N
+    snap.particles.position[:] = c["pos4"][:, :3]
+    sim = hoomd.Simulation(seed=2)
+    sim.create_state_from_snapshot(snap)
+    sim.state.thermalize_particle_momenta(hoomd.filter.All(), kT=1.2)
+    integ = hoomd.md.Integrator(dt=0.005)
+    nl = hoomd.md.nlist.Cell(buffer=0.4)
+    plj = hoomd.azplugins.pair.PerturbedLennardJones(nlist=nl, default_r_cut=3.0)
+    plj.params[("A", "A")] = dict(epsilon=1.0, sigma=1.0, attraction_scale_factor=0.5)
+    plj.precision = "mixed"
+    integ.forces = [plj]
+    integ.methods = [hoomd.md.methods.Langevin(hoomd.filter.All(), kT=1.2, default_gamma=1.0)]
+    sim.operations.integrator = integ
+    sim.run(20)
+    torch.cuda.synchronize()
+    b0 = nl.num_builds
+    t0 = time.perf_counter()
+    sim.run(steps)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    res = {"value": steps / dt, "unit": "timesteps/s", "steps": steps,
+           "nlist_builds": nl.num_builds - b0, "precision": "mixed", "method": "Langevin kT=1.2 gamma=1 dt=0.005"}
+    log(f"bench: TPS {steps / dt:.1f} ({nl.num_builds - b0} rebuilds in {steps} steps)")
+    return res
+
+
+def cpu_baseline(cells):
+    """FP64 oracle (half list, Newton's third law, one thread) on a bounded sample."""
+    import oracle
+    from azplugins_amd import configs
+
+    if not os.path.exists(oracle._LIB_PATH):
+        return None
+    c = configs.plj_1m(n=cells)
+    pos4, L = c["pos4"], c["L"]
+    N = pos4.shape[0]
+    pairs = oracle.nlist(pos4, L, 1, oracle.r_listsq_matrix([3.0], 0.4), "cell")
+    packed = oracle.pack_matrix(oracle.PERTURBED_LJ, c["params"], 1)
+    n_cut = oracle.count_pairs_in_cut(pos4, L, pairs, 1, np.array([9.0]))
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        oracle.pair_forces(oracle.PERTURBED_LJ, pos4, L, pairs, 1, packed, np.array([9.0]))
+        times.append(time.perf_counter() - t0)
+    t = float(np.median(times))
+    log(f"bench: CPU oracle {n_cut / t:.3e} pair interactions/s on N={N} ({t:.2f} s per evaluation)")
+    return {
+        "value": n_cut / t,
+        "unit": "pair interactions/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"PerturbedLJ FCC {cells}^3 (N={N}) at density 0.85, r_cut 3.0, 3 force "
+                  f"evaluations (median {t:.2f} s), FP64 half list with third law; host "
+                  f"{os.cpu_count()} logical CPUs",
+    }
+
+
+def load_traffic():
+    """HBM bytes per launch of the mixed pair kernel from rocprofv3 counters, if measured."""
+    p = os.path.join(ROOT, "profiles", "pmc_traffic_plj_1m.json")
+    if not os.path.exists(p):
+        return None
+    with open(p) as fh:
+        return json.load(fh).get("traffic_bytes_per_launch")
+
+
+if __name__ == "__main__":
+    main()

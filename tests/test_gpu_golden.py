@@ -64,9 +64,19 @@ def test_energy_and_force(potential_test, precision):
     sim.run(0)
     assert potential.params[("A", "A")] == potential_test["params"]
     e = potential_test["energy"]
-    numpy.testing.assert_array_almost_equal(potential.energies, [0.5 * e, 0.5 * e], decimal=4)
     f = potential_test["force"]
-    numpy.testing.assert_array_almost_equal(potential.forces, [[-f, 0, 0], [f, 0, 0]], decimal=4)
+    _check(potential.energies, [0.5 * e, 0.5 * e], precision)
+    _check(potential.forces, [[-f, 0, 0], [f, 0, 0]], precision)
+
+
+def _check(actual, desired, precision):
+    """The reference's decimal=4 for the FP64 form; for the mixed form the same
+    absolute bar or 2e-6 relative, whichever is looser (decimal=4 on |F| ~ 1200
+    asks for 1e-7 relative, below FP32 resolution)."""
+    if precision == "fp64":
+        numpy.testing.assert_array_almost_equal(actual, desired, decimal=4)
+    else:
+        numpy.testing.assert_allclose(actual, desired, rtol=2e-6, atol=1.5e-4)
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])

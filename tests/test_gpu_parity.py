@@ -167,3 +167,29 @@ def test_empty_and_isolated():
                  {(0, 0): ISO_PARAMS[2][(0, 0)]}, 3.0)
     assert np.all(r.force == 0.0)
     assert nlist_pairs(r.nl, 3) == set()
+
+
+def test_invalid_inputs_are_errors():
+    """Non-finite positions and type indices without parameters are reported
+    as errors by the neighbor-list build and the rebuild check (no silent
+    self-interaction, no out-of-range table reads)."""
+    from azplugins_amd import _azp
+
+    L = 10.0
+    pos4, _ = jittered_lattice(6, L, 0.1, 1, seed=1)
+    Lv = np.array([L, L, L])
+    params = {(0, 0): {"epsilon": 1.0}}
+    bad = pos4.copy()
+    bad[5, 1] = np.nan
+    with pytest.raises(_azp.AzpError, match="not finite"):
+        gpu_pair("PotentialPairHertz", GpuSystem(bad, Lv), 1, params, 1.5)
+    bad = pos4.copy()
+    bad[7, 3] = 3.0
+    with pytest.raises(_azp.AzpError, match="type index"):
+        gpu_pair("PotentialPairHertz", GpuSystem(bad, Lv), 1, params, 1.5)
+    # NaN appearing after the build is caught by the rebuild check
+    sysm = GpuSystem(pos4, Lv)
+    r = gpu_pair("PotentialPairHertz", sysm, 1, params, 1.5)
+    sysm.pos[3, 0] = float("nan")
+    with pytest.raises(_azp.AzpError, match="not finite"):
+        r.nl.update(sysm.particles())
