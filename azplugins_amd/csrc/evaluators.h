@@ -202,6 +202,30 @@ struct EvalPerturbedLJ
             }
         e -= q.eshift;
         }
+    // The same with the cutoff mask folded in (FP32 form): both branches become
+    // f *= s, e = e s + c with (s, c) = (1, wca_shift - eshift) inside the WCA
+    // radius, (asf, -eshift) outside, and (0, 0) for masked entries.
+    static constexpr bool kMasked = true;
+    __device__ static void compute_masked(const Prep<float>& q, float rsq, bool in, float& fdivr, float& e)
+        {
+        const float r2inv = f_rcp(rsq);
+        const float r6inv = r2inv * r2inv * r2inv;
+        const bool wca = rsq < q.rwcasq;
+        const float s = in ? (wca ? 1.0f : q.asf) : 0.0f;
+        const float c = in ? (wca ? q.wca_shift - q.eshift : -q.eshift) : 0.0f;
+        fdivr = (r2inv * r6inv * (q.lj1x12 * r6inv - q.lj2x6)) * s;
+        e = (r6inv * (q.lj1 * r6inv - q.lj2)) * s + c;
+        }
+    };
+
+// Evaluators with a fused, masked FP32 form.
+template<class Eval, typename = void> struct HasMasked
+    {
+    static constexpr bool value = false;
+    };
+template<class Eval> struct HasMasked<Eval, std::void_t<decltype(Eval::kMasked)>>
+    {
+    static constexpr bool value = Eval::kMasked;
     };
 
 // ---------------------------------------------------------------------------

@@ -158,20 +158,17 @@ __device__ __forceinline__ uint32_t checked_type(double w, uint32_t ntypes)
     return (w >= 0.0 && w < (double)ntypes) ? (uint32_t)w : ntypes;
     }
 
-__global__ void __launch_bounds__(256) k_walk(const WalkArgs a)
+// Lists particle i's neighbors and returns how many it has.
+__device__ uint32_t walk_one(const WalkArgs& a, uint32_t i)
     {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.N)
-        return;
     const double4 pi = a.pos[i];
     const uint32_t ti = checked_type(pi.w, a.ntypes);
     const bool finite = isfinite(pi.x) && isfinite(pi.y) && isfinite(pi.z);
     if (ti == a.ntypes || !finite)
         {
         // no neighbors for a particle without a valid type or position; reported to the host
-        a.n_neigh[i] = 0;
         atomicOr(a.overflow + 2, finite ? 1u : 2u);
-        return;
+        return 0;
         }
     const uint32_t c = a.cell_of[i];
     const int cx = (int)(c % (uint32_t)a.g.n[0]);
@@ -232,9 +229,31 @@ __global__ void __launch_bounds__(256) k_walk(const WalkArgs a)
                         }
                     }
                 }
-    a.n_neigh[i] = count;
-    if (count > a.max_neighbors)
-        atomicMax(a.overflow, count);
+    return count;
+    }
+
+// One lane per particle.  After listing, the 64 lanes of a wavefront pad their
+// columns with their own index up to the wave's longest list rounded up to a
+// multiple of 8, so consumers can run a wave-uniform loop over batches of 8
+// entries with no per-entry bounds test (padding entries have j == i).
+__global__ void __launch_bounds__(256) k_walk(const WalkArgs a)
+    {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = i < a.N;
+    const uint32_t count = active ? walk_one(a, i) : 0u;
+    if (active)
+        {
+        a.n_neigh[i] = count;
+        if (count > a.max_neighbors)
+            atomicMax(a.overflow, count);
+        }
+    uint32_t wmax = count;
+    for (int off = 32; off > 0; off >>= 1)
+        wmax = max(wmax, (uint32_t)__shfl_xor((int)wmax, off));
+    const uint32_t padded = (wmax + 7u) & ~7u;
+    if (active && wmax <= a.max_neighbors)
+        for (uint32_t k = count; k < padded; ++k)
+            a.nlist[(size_t)k * a.pitch + i] = i;
     }
 
 // Sets *flag when some particle moved farther than sqrt(thresh_sq).

@@ -35,6 +35,7 @@ struct PairKernelArgs
     DevBox box;
     float fx_scale[3];    // L / 2^32 per axis
     double fx_scale64[3]; // the same in FP64
+    float fx_scale2[3];   // (L / 2^32)^2 per axis
     const uint32_t* n_neigh;
     const uint32_t* nlist;
     uint32_t nlist_pitch;
@@ -146,58 +147,87 @@ __device__ __forceinline__ double exact_rsq(const PairKernelArgs& a, uint32_t i,
     return rsq3_nofma(dx, dy, dz);
     }
 
-// Neighbor entries processed per batch: all list indices of a batch are
-// loaded first, then all gathers are issued, then the pair terms are
-// evaluated, so each lane keeps UNROLL gathers in flight instead of one.
+// Neighbor entries processed per batch: the batch's list words are loaded
+// first, then all its gathers are issued, then the pair terms are evaluated,
+// so each lane keeps U gathers in flight.  The neighbor list pads every
+// wavefront's columns with the particle's own index to a multiple of 8, so the
+// batch loop is wave-uniform and padding entries are masked by j == i.
 template<bool MIXED> struct Unroll
     {
     static constexpr uint32_t value = MIXED ? 8 : 4;
     };
 
-// Branchless evaluation of one list entry: lanes whose entry is padding or
-// outside the cutoff evaluate at a safe distance and contribute zero.
-template<class Eval, bool VIRIAL>
-__device__ __forceinline__ void term_fp64(const PairKernelArgs& a,
-                                          const typename Eval::template Prep<double>& q,
+__device__ __forceinline__ uint32_t wave_max(uint32_t v)
+    {
+    for (int off = 32; off > 0; off >>= 1)
+        v = max(v, (uint32_t)__shfl_xor((int)v, off));
+    return v;
+    }
+
+// Accumulators of the mixed form are kept in integer-grid units of the
+// fixed-point positions (dx = n * L / 2^32): sum f n and sum f n_a n_b are
+// formed from exact integer-to-FP64 conversions and scaled once at the end.
+template<bool VIRIAL>
+__device__ __forceinline__ void store_scaled(const PairKernelArgs& a, uint32_t i, const Acc& acc)
+    {
+    const double sx = a.fx_scale64[0], sy = a.fx_scale64[1], sz = a.fx_scale64[2];
+    a.force[i] = make_double4(acc.fx * sx, acc.fy * sy, acc.fz * sz, 0.5 * acc.e);
+    if (VIRIAL)
+        {
+        const size_t p = a.vpitch;
+        a.virial[0 * p + i] = 0.5 * acc.v0 * (sx * sx);
+        a.virial[1 * p + i] = 0.5 * acc.v1 * (sx * sy);
+        a.virial[2 * p + i] = 0.5 * acc.v2 * (sx * sz);
+        a.virial[3 * p + i] = 0.5 * acc.v3 * (sy * sy);
+        a.virial[4 * p + i] = 0.5 * acc.v4 * (sy * sz);
+        a.virial[5 * p + i] = 0.5 * acc.v5 * (sz * sz);
+        }
+    }
+
+// Branchless evaluation of one list entry (FP64 form): padding entries and
+// pairs beyond the cutoff are evaluated at a safe distance and contribute zero.
+template<class Eval, bool VIRIAL, bool XPLOR>
+__device__ __forceinline__ void term_fp64(const typename Eval::template Prep<double>& q,
                                           const TypePairExtra<double>& x,
                                           bool valid,
                                           double dx,
                                           double dy,
                                           double dz,
-                                          bool xplor,
                                           Acc& acc)
     {
     const double rsq = dx * dx + dy * dy + dz * dz;
     const bool in = valid && rsq < x.rcsq64 && Eval::active(q);
+    const double rs = in ? rsq : x.rcsq64;
     double fdivr, e;
-    Eval::compute(q, in ? rsq : x.rcsq64, fdivr, e);
-    if (xplor)
-        apply_xplor<double>(rsq, x.rcsq64, x.ronsq, fdivr, e);
+    Eval::compute(q, rs, fdivr, e);
+    if constexpr (XPLOR)
+        apply_xplor<double>(rs, x.rcsq64, x.ronsq, fdivr, e);
     fdivr = in ? fdivr : 0.0;
     e = in ? e : 0.0;
     accumulate<VIRIAL>(acc, fdivr, dx, dy, dz, e);
     }
 
-template<class Eval, bool VIRIAL>
+template<class Eval, bool VIRIAL, bool XPLOR, bool CUBIC>
 __device__ __forceinline__ void term_mixed(const PairKernelArgs& a,
                                            const typename Eval::template Prep<float>& q,
                                            const TypePairExtra<float>& x,
-                                           bool valid,
                                            uint32_t i,
                                            uint32_t j,
                                            int32_t ix,
                                            int32_t iy,
                                            int32_t iz,
-                                           bool xplor,
                                            Acc& acc)
     {
-    const float dx = (float)ix * a.fx_scale[0];
-    const float dy = (float)iy * a.fx_scale[1];
-    const float dz = (float)iz * a.fx_scale[2];
-    float rsq = dx * dx + dy * dy + dz * dz;
+    const float fx = (float)ix, fy = (float)iy, fz = (float)iz;
+    float rsq;
+    if constexpr (CUBIC)
+        rsq = (fx * fx + fy * fy + fz * fz) * a.fx_scale2[0];
+    else
+        rsq = fx * fx * a.fx_scale2[0] + fy * fy * a.fx_scale2[1] + fz * fz * a.fx_scale2[2];
+    const bool valid = j != i;
     bool in = valid && rsq < x.rcsq_lo;
     double r64 = -1.0;
-    if (valid && rsq >= x.rcsq_lo && rsq < x.rcsq_hi)
+    if (valid && !in && rsq < x.rcsq_hi)
         {
         // FP32 cannot decide this one: apply the FP64 cutoff test
         r64 = exact_rsq(a, i, j);
@@ -207,6 +237,12 @@ __device__ __forceinline__ void term_mixed(const PairKernelArgs& a,
     in = in && Eval::active(q);
     const float rs = in ? rsq : x.rcsq;
     float fdivr, e;
+    if constexpr (HasMasked<Eval>::value && !XPLOR)
+        {
+        Eval::compute_masked(q, rs, in, fdivr, e);
+        accumulate<VIRIAL>(acc, (double)fdivr, (double)ix, (double)iy, (double)iz, (double)e);
+        return;
+        }
     if constexpr (NeedsRsq64<Eval>::value)
         {
         if (r64 < 0.0)
@@ -222,14 +258,14 @@ __device__ __forceinline__ void term_mixed(const PairKernelArgs& a,
         {
         Eval::compute(q, rs, fdivr, e);
         }
-    if (xplor)
+    if constexpr (XPLOR)
         apply_xplor<float>(rs, x.rcsq, x.ronsq, fdivr, e);
     fdivr = in ? fdivr : 0.0f;
     e = in ? e : 0.0f;
-    accumulate<VIRIAL>(acc, (double)fdivr, (double)dx, (double)dy, (double)dz, (double)e);
+    accumulate<VIRIAL>(acc, (double)fdivr, (double)ix, (double)iy, (double)iz, (double)e);
     }
 
-template<class Eval, bool MIXED, bool VIRIAL, bool SINGLE>
+template<class Eval, bool MIXED, bool VIRIAL, bool SINGLE, bool XPLOR, bool CUBIC>
 __global__ void __launch_bounds__(512) k_pair(const PairKernelArgs a)
     {
     using R = std::conditional_t<MIXED, float, double>;
@@ -255,25 +291,26 @@ __global__ void __launch_bounds__(512) k_pair(const PairKernelArgs a)
         }
 
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.N)
+    // wavefronts entirely past N leave together; lanes past N in the last
+    // wavefront shadow particle N-1 (same column, results not stored)
+    if (blockIdx.x * blockDim.x + (threadIdx.x & ~63u) >= a.N)
         return;
-
-    Acc acc;
-    const uint32_t nn = a.n_neigh[i];
-    const bool xplor = a.shift_mode == AZP_SHIFT_XPLOR;
-    const uint32_t* col = a.nlist + i;
+    const uint32_t ie = min(i, a.N - 1);
+    const uint32_t nn = a.n_neigh[ie];
+    const uint32_t P = (wave_max(nn) + U - 1) & ~(U - 1);
     const size_t stride = a.nlist_pitch;
 
+    Acc acc;
     if constexpr (!MIXED)
         {
-        const double4 pi = a.pos[i];
+        const double4 pi = a.pos[ie];
         const uint32_t ti = (uint32_t)pi.w;
-        for (uint32_t k = 0; k < nn; k += U)
+        for (uint32_t k = 0; k < P; k += U)
             {
             uint32_t jj[U];
 #pragma unroll
             for (uint32_t u = 0; u < U; ++u)
-                jj[u] = (k + u < nn) ? col[(size_t)(k + u) * stride] : i;
+                jj[u] = (a.nlist + (size_t)(k + u) * stride)[ie];
             double4 pj[U];
 #pragma unroll
             for (uint32_t u = 0; u < U; ++u)
@@ -285,36 +322,47 @@ __global__ void __launch_bounds__(512) k_pair(const PairKernelArgs a)
                 const double dy = min_image_1d(pi.y - pj[u].y, a.box.L[1], a.box.Linv[1]);
                 const double dz = min_image_1d(pi.z - pj[u].z, a.box.L[2], a.box.Linv[2]);
                 const uint32_t tp = SINGLE ? 0u : ti * nt + (uint32_t)pj[u].w;
-                term_fp64<Eval, VIRIAL>(a, SINGLE ? q0 : s_prep[tp], SINGLE ? x0 : s_x[tp], k + u < nn,
-                                        dx, dy, dz, xplor, acc);
+                term_fp64<Eval, VIRIAL, XPLOR>(SINGLE ? q0 : s_prep[tp], SINGLE ? x0 : s_x[tp],
+                                               jj[u] != ie, dx, dy, dz, acc);
                 }
             }
+        if (i < a.N)
+            store<VIRIAL>(a, i, acc);
         }
     else
         {
-        const int4 pi = a.pos_fixed[i];
+        const int4 pi = a.pos_fixed[ie];
         const uint32_t ti = (uint32_t)pi.w;
-        for (uint32_t k = 0; k < nn; k += U)
+        // 32-bit byte offsets from wave-uniform bases (global loads with an
+        // SGPR base and a VGPR offset: no per-entry 64-bit address math)
+        const uint32_t col_off = ie * 4u;
+        const char* pf_base = reinterpret_cast<const char*>(a.pos_fixed);
+        for (uint32_t k = 0; k < P; k += U)
             {
             uint32_t jj[U];
 #pragma unroll
             for (uint32_t u = 0; u < U; ++u)
-                jj[u] = (k + u < nn) ? col[(size_t)(k + u) * stride] : i;
+                {
+                const char* row = reinterpret_cast<const char*>(a.nlist + (size_t)(k + u) * stride);
+                jj[u] = *reinterpret_cast<const uint32_t*>(row + col_off);
+                }
             int4 pj[U];
 #pragma unroll
             for (uint32_t u = 0; u < U; ++u)
-                pj[u] = a.pos_fixed[jj[u]];
+                pj[u] = *reinterpret_cast<const int4*>(pf_base + (jj[u] << 4));
 #pragma unroll
             for (uint32_t u = 0; u < U; ++u)
                 {
                 const uint32_t tp = SINGLE ? 0u : ti * nt + (uint32_t)pj[u].w;
-                term_mixed<Eval, VIRIAL>(a, SINGLE ? q0 : s_prep[tp], SINGLE ? x0 : s_x[tp], k + u < nn, i,
-                                         jj[u], fixed_delta(pi.x, pj[u].x), fixed_delta(pi.y, pj[u].y),
-                                         fixed_delta(pi.z, pj[u].z), xplor, acc);
+                term_mixed<Eval, VIRIAL, XPLOR, CUBIC>(a, SINGLE ? q0 : s_prep[tp], SINGLE ? x0 : s_x[tp], ie,
+                                                       jj[u], fixed_delta(pi.x, pj[u].x),
+                                                       fixed_delta(pi.y, pj[u].y), fixed_delta(pi.z, pj[u].z),
+                                                       acc);
                 }
             }
+        if (i < a.N)
+            store_scaled<VIRIAL>(a, i, acc);
         }
-    store<VIRIAL>(a, i, acc);
     }
 
 __global__ void k_pack_fixed(const double4* __restrict__ pos, uint32_t N, DevBox box, int4* __restrict__ out)
@@ -329,7 +377,7 @@ __global__ void k_pack_fixed(const double4* __restrict__ pos, uint32_t N, DevBox
                        (int)p.w);
     }
 
-template<class Eval, bool MIXED, bool VIRIAL, bool SINGLE>
+template<class Eval, bool MIXED, bool VIRIAL, bool SINGLE, bool XPLOR, bool CUBIC>
 static int launch_t(const PairKernelArgs& ka, unsigned block, hipStream_t stream)
     {
     using R = std::conditional_t<MIXED, float, double>;
@@ -339,9 +387,30 @@ static int launch_t(const PairKernelArgs& ka, unsigned block, hipStream_t stream
     if (shmem > 64 * 1024)
         return set_error(AZP_ERR_UNSUPPORTED, "too many particle types (%u) for LDS staging", ka.ntypes);
     const unsigned grid = grid_for(ka.N, block);
-    hipLaunchKernelGGL((k_pair<Eval, MIXED, VIRIAL, SINGLE>), dim3(grid), dim3(block), shmem, stream, ka);
+    hipLaunchKernelGGL((k_pair<Eval, MIXED, VIRIAL, SINGLE, XPLOR, CUBIC>), dim3(grid), dim3(block), shmem,
+                       stream, ka);
     AZP_LAUNCH_CHECK();
     return AZP_OK;
+    }
+
+template<class Eval, bool MIXED, bool VIRIAL, bool SINGLE>
+static int launch_mode(const PairKernelArgs& ka, unsigned block, hipStream_t s)
+    {
+    const bool xplor = ka.shift_mode == AZP_SHIFT_XPLOR;
+    if constexpr (MIXED)
+        {
+        const bool cubic = ka.fx_scale2[0] == ka.fx_scale2[1] && ka.fx_scale2[0] == ka.fx_scale2[2];
+        if (xplor)
+            return cubic ? launch_t<Eval, true, VIRIAL, SINGLE, true, true>(ka, block, s)
+                         : launch_t<Eval, true, VIRIAL, SINGLE, true, false>(ka, block, s);
+        return cubic ? launch_t<Eval, true, VIRIAL, SINGLE, false, true>(ka, block, s)
+                     : launch_t<Eval, true, VIRIAL, SINGLE, false, false>(ka, block, s);
+        }
+    else
+        {
+        return xplor ? launch_t<Eval, false, VIRIAL, SINGLE, true, false>(ka, block, s)
+                     : launch_t<Eval, false, VIRIAL, SINGLE, false, false>(ka, block, s);
+        }
     }
 
 template<class Eval, bool MIXED>
@@ -349,10 +418,10 @@ static int launch_eval(const PairKernelArgs& ka, bool virial, unsigned block, hi
     {
     const bool single = ka.ntypes == 1;
     if (virial)
-        return single ? launch_t<Eval, MIXED, true, true>(ka, block, s)
-                      : launch_t<Eval, MIXED, true, false>(ka, block, s);
-    return single ? launch_t<Eval, MIXED, false, true>(ka, block, s)
-                  : launch_t<Eval, MIXED, false, false>(ka, block, s);
+        return single ? launch_mode<Eval, MIXED, true, true>(ka, block, s)
+                      : launch_mode<Eval, MIXED, true, false>(ka, block, s);
+    return single ? launch_mode<Eval, MIXED, false, true>(ka, block, s)
+                  : launch_mode<Eval, MIXED, false, false>(ka, block, s);
     }
 
 int pack_fixed(const double* d_pos, uint32_t N, const azp_box& box, int32_t* d_out, hipStream_t stream)
@@ -436,6 +505,7 @@ extern "C" int azp_gpu_compute_pair_forces(const azp_pair_args* args, int evalua
         {
         ka.fx_scale64[k] = ka.box.L[k] / 4294967296.0;
         ka.fx_scale[k] = (float)ka.fx_scale64[k];
+        ka.fx_scale2[k] = (float)(ka.fx_scale64[k] * ka.fx_scale64[k]);
         }
     ka.n_neigh = args->d_n_neigh;
     ka.nlist = args->d_nlist;
