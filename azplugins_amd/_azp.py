@@ -116,6 +116,8 @@ SIGNATURES = [
     ("azp_gpu_compute_pair_aniso_forces", _i, [ctypes.POINTER(PairArgs), _i, _vp]),
     ("azp_gpu_pack_fixed", _i, [_vp, _u32, ctypes.POINTER(Box), _vp, _vp]),
     ("azp_gpu_dpd_alpha", _i, [ctypes.c_uint16, _u64, _u64, _vp, _vp, _vp, _vp]),
+    ("azp_adjoint_workspace_size", ctypes.c_size_t, [_u32, _u32]),
+    ("azp_gpu_pair_adjoint", _i, [ctypes.POINTER(PairArgs), _i, _vp, _vp, _vp, _vp, _vp, ctypes.c_size_t]),
     ("azp_gpu_integrate_vv_one", _i, [_vp, _vp, _vp, _vp, _u32, ctypes.POINTER(Box), ctypes.c_double, _vp]),
     ("azp_gpu_integrate_vv_two", _i, [_vp, _vp, _vp, _vp, _u32, ctypes.c_double, _i, ctypes.c_double,
                                       ctypes.c_double, ctypes.c_uint16, _u64, _vp]),

@@ -218,6 +218,26 @@ int azp_gpu_dpd_alpha(uint16_t seed,
                       double* d_alpha,
                       void* stream);
 
+/* ---- adjoint of the isotropic pair forces (no counterpart in the reference:
+ * the force-matching extension of the path).  For ExpandedYukawa
+ * (theta = epsilon, kappa, delta) and PerturbedLennardJones (theta = epsilon,
+ * sigma, attraction_scale_factor), modes none/shift, 1..4 types.
+ * d_grad:        double4[N] (dL/dF_x, dL/dF_y, dL/dF_z, dL/dE_i)
+ * d_grad_pos:    double4[N] dL/dr (x, y, z, 0)
+ * d_grad_params: double[ntypes^2 * 3], dL/dtheta per ordered type pair (t_i, t_j)
+ *                (the gradient of a symmetric parameter is the (a,b) plus (b,a) entries)
+ * d_work:        azp_adjoint_workspace_size(N, ntypes) doubles.
+ * args->precision selects FP64 or FP32 pair arithmetic (FP64 accumulation). */
+size_t azp_adjoint_workspace_size(uint32_t N, uint32_t ntypes);
+int azp_gpu_pair_adjoint(const azp_pair_args* args,
+                         int evaluator,
+                         const void* d_params,
+                         const double* d_grad,
+                         double* d_grad_pos,
+                         double* d_grad_params,
+                         double* d_work,
+                         size_t work_len);
+
 /* ---- integration around the force path (replace the upstream
  * TwoStepConstantVolume / TwoStepLangevin kernels reached from the reference
  * tests, test_pair.py:325-326, test_pair_dpd.py:189-200) ----

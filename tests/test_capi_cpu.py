@@ -19,7 +19,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def _declared():
     with open(os.path.join(ROOT, "include", "azp.h")) as fh:
         text = fh.read()
-    return sorted(set(re.findall(r"^(?:int|const char\*) (azp_[a-z0-9_]+)\(", text, re.M)))
+    return sorted(set(re.findall(r"^(?:int|size_t|const char\*) (azp_[a-z0-9_]+)\(", text, re.M)))
 
 
 def test_library_exports_every_declared_symbol():

@@ -1,0 +1,169 @@
+"""Adjoint kernel (azp_gpu_pair_adjoint) behind torch.autograd.
+
+Checks, each of which a dropped term or a wrong sign fails:
+* forward-force identity: the VJP of the total energy with respect to the
+  positions is -F (dL/dE_i = 1, dL/dF = 0), to 1e-12,
+* parameter gradients of a force-matching loss (MSE of forces plus an energy
+  term) against FP64 central finite differences (h = 1e-6) of the same loss
+  evaluated by the oracle, to 1e-6 relative (FP64 form),
+* position VJPs against the same finite differences,
+* the headline differentiation workload (ExpandedYukawa, N = 262,144, 70/30,
+  r_cut = 4): gradients against finite differences of the FP64 GPU forward.
+"""
+
+import numpy as np
+import pytest
+
+import oracle
+from azplugins_amd import _azp, configs
+from azplugins_amd.adjoint import PairForceField
+from gpu_util import jittered_lattice, say
+
+torch = _azp.torch
+pytestmark = pytest.mark.gpu
+
+EV = {"ExpandedYukawa": oracle.EXPANDED_YUKAWA, "PerturbedLennardJones": oracle.PERTURBED_LJ}
+KEYS = {"ExpandedYukawa": ("epsilon", "kappa", "delta"),
+        "PerturbedLennardJones": ("epsilon", "sigma", "attraction_scale_factor")}
+THETA = {
+    "ExpandedYukawa": [[1.0, 2.0, 0.2], [0.7, 1.8, 0.35], [1.3, 2.2, 0.1]],
+    "PerturbedLennardJones": [[1.0, 1.0, 0.5], [0.8, 0.92, 0.3], [1.2, 0.95, 0.7]],
+}
+RCUT = {"ExpandedYukawa": 3.0, "PerturbedLennardJones": 2.5}
+
+
+def _system(name, seed=3):
+    L = 13.0 if name == "ExpandedYukawa" else 12.2
+    pos4, rng = jittered_lattice(12, L, 0.12, 2, seed=seed)
+    return pos4, np.array([L, L, L]), rng
+
+
+def _oracle_loss(name, pos4, Lv, theta, target, wE, mode):
+    ev = EV[name]
+    params = {k: dict(zip(KEYS[name], row)) for k, row in zip([(0, 0), (0, 1), (1, 1)], theta)}
+    rc = RCUT[name]
+    pairs = oracle.nlist(pos4, Lv, 2, oracle.r_listsq_matrix([rc] * 4, 0.4), "cell")
+    f, _ = oracle.pair_forces(ev, pos4, Lv, pairs, 2, oracle.pack_matrix(ev, params, 2), np.full(4, rc * rc),
+                              mode=mode)
+    return np.mean((f[:, :3] - target) ** 2) + wE * f[:, 3].sum()
+
+
+def _gpu_grads(name, pos4, Lv, theta, target, wE, mode, precision):
+    ff = PairForceField(name, 2, Lv, RCUT[name], mode=mode, precision=precision)
+    pos = torch.tensor(pos4[:, :3], device="cuda", requires_grad=True)
+    th = torch.tensor(theta, dtype=torch.float64, device="cuda", requires_grad=True)
+    typeid = torch.tensor(pos4[:, 3], device="cuda")
+    F, E = ff(pos, typeid, th)
+    tgt = torch.tensor(target, device="cuda")
+    loss = ((F - tgt) ** 2).mean() + wE * E.sum()
+    loss.backward()
+    return loss.item(), pos.grad.cpu().numpy(), th.grad.cpu().numpy(), F.detach().cpu().numpy()
+
+
+def test_energy_vjp_is_minus_force():
+    name = "PerturbedLennardJones"
+    pos4, Lv, _ = _system(name)
+    for precision, tol in (("fp64", 1e-12), ("mixed", 2e-6)):
+        ff = PairForceField(name, 2, Lv, RCUT[name], precision=precision)
+        pos = torch.tensor(pos4[:, :3], device="cuda", requires_grad=True)
+        th = torch.tensor(THETA[name], dtype=torch.float64, device="cuda")
+        F, E = ff(pos, torch.tensor(pos4[:, 3], device="cuda"), th)
+        E.sum().backward()
+        g = pos.grad.cpu().numpy()
+        Fn = F.detach().cpu().numpy()
+        err = np.abs(g + Fn).max() / np.abs(Fn).max()
+        say(f"[adjoint] energy identity {precision}: {err:.2e}")
+        assert err < tol
+
+
+@pytest.mark.parametrize("mode", ["none", "shift"])
+@pytest.mark.parametrize("name", ["ExpandedYukawa", "PerturbedLennardJones"])
+def test_param_and_position_grads_vs_fd(name, mode):
+    pos4, Lv, rng = _system(name)
+    theta = np.array(THETA[name])
+    # target forces from perturbed parameters (force matching)
+    ptheta = theta * (1.0 + 0.05 * rng.standard_normal(theta.shape))
+    ev = EV[name]
+    params_t = {k: dict(zip(KEYS[name], row)) for k, row in zip([(0, 0), (0, 1), (1, 1)], ptheta)}
+    rc = RCUT[name]
+    pairs = oracle.nlist(pos4, Lv, 2, oracle.r_listsq_matrix([rc] * 4, 0.4), "cell")
+    ft, _ = oracle.pair_forces(ev, pos4, Lv, pairs, 2, oracle.pack_matrix(ev, params_t, 2), np.full(4, rc * rc),
+                               mode=mode)
+    target = ft[:, :3]
+    wE = 1e-3
+    say(f"[adjoint fd] {name} {mode}")
+    L0, gpos, gth, _ = _gpu_grads(name, pos4, Lv, theta, target, wE, mode, "fp64")
+    assert L0 == pytest.approx(_oracle_loss(name, pos4, Lv, theta, target, wE, mode), rel=1e-10)
+    h = 1e-6
+    fd = np.zeros_like(theta)
+    for a in range(theta.shape[0]):
+        for b in range(theta.shape[1]):
+            tp = theta.copy()
+            tp[a, b] += h
+            tm = theta.copy()
+            tm[a, b] -= h
+            fd[a, b] = (_oracle_loss(name, pos4, Lv, tp, target, wE, mode)
+                        - _oracle_loss(name, pos4, Lv, tm, target, wE, mode)) / (2 * h)
+    err = np.abs(gth - fd).max() / np.abs(fd).max()
+    say(f"  param grad rel err (fp64) {err:.2e}")
+    assert err < 1e-6
+    # position VJP on a few coordinates
+    fdp = []
+    for (i, c) in [(0, 0), (17, 1), (401, 2), (777, 0)]:
+        pp = pos4.copy()
+        pp[i, c] += h
+        lp = _oracle_loss(name, pp, Lv, theta, target, wE, mode)
+        pp[i, c] -= 2 * h
+        lm = _oracle_loss(name, pp, Lv, theta, target, wE, mode)
+        fdp.append(((lp - lm) / (2 * h), gpos[i, c]))
+    fdp = np.array(fdp)
+    perr = np.abs(fdp[:, 0] - fdp[:, 1]).max() / np.abs(gpos).max()
+    say(f"  position grad rel err (fp64) {perr:.2e}")
+    assert perr < 1e-6
+    # mixed form: same gradients within FP32 pair arithmetic
+    _, gpos_m, gth_m, _ = _gpu_grads(name, pos4, Lv, theta, target, wE, mode, "mixed")
+    merr = np.abs(gth_m - fd).max() / np.abs(fd).max()
+    say(f"  param grad rel err (mixed) {merr:.2e}")
+    assert merr < 1e-3
+
+
+def test_yukawa_262k_gradients_vs_gpu_fd():
+    """Differentiation workload at full size: 64^3 simple cubic, density 0.5, 70/30, r_cut 4."""
+    c = configs.yukawa_262k()
+    pos4, Lv = c["pos4"], c["L"]
+    keys = [(0, 0), (0, 1), (1, 1)]
+    theta = np.array([[c["params"][k]["epsilon"], c["params"][k]["kappa"], c["params"][k]["delta"]] for k in keys])
+    rng = np.random.default_rng(44)
+    ptheta = theta * (1.0 + 0.05 * rng.standard_normal(theta.shape))
+    ff = PairForceField("ExpandedYukawa", 2, Lv, 4.0, precision="fp64")
+    typeid = torch.tensor(pos4[:, 3], device="cuda")
+    pos = torch.tensor(pos4[:, :3], device="cuda")
+    with torch.no_grad():
+        tgt, _ = ff(pos, typeid, torch.tensor(ptheta, device="cuda"))
+
+    def loss_of(th, p=pos):
+        with torch.no_grad():
+            F, E = ff(p, typeid, torch.tensor(th, device="cuda"))
+            return ((F - tgt) ** 2).mean().item()
+
+    say("[adjoint 262k] backward")
+    for precision in ("fp64", "mixed"):
+        ffp = PairForceField("ExpandedYukawa", 2, Lv, 4.0, precision=precision)
+        th = torch.tensor(theta, device="cuda", requires_grad=True)
+        pr = pos.clone().requires_grad_(True)
+        F, _ = ffp(pr, typeid, th)
+        ((F - tgt) ** 2).mean().backward()
+        g = th.grad.cpu().numpy()
+        if precision == "fp64":
+            h = 1e-6
+            fd = np.zeros_like(theta)
+            for a in range(3):
+                for b in range(3):
+                    tp = theta.copy()
+                    tp[a, b] += h
+                    tm = theta.copy()
+                    tm[a, b] -= h
+                    fd[a, b] = (loss_of(tp) - loss_of(tm)) / (2 * h)
+        err = np.abs(g - fd).max() / np.abs(fd).max()
+        say(f"  {precision}: param grad rel err vs FD {err:.2e}")
+        assert err < (1e-6 if precision == "fp64" else 1e-3)
