@@ -30,6 +30,27 @@ __device__ __forceinline__ double rsq3_nofma(double x, double y, double z)
     }
 
 // ---------------------------------------------------------------------------
+// XCD-aware workgroup order.  gfx950 deals workgroups round-robin to its 8
+// XCDs (workgroup b runs on XCD b % 8), and each XCD has its own L2.  This
+// maps hardware workgroup b to a logical workgroup so that every XCD owns one
+// contiguous eighth of the logical range: with particles in space-filling-curve
+// order, the neighbor gathers of an XCD stay within a compact region that fits
+// its L2.  The map is a bijection of [0, gridDim.x).
+// ---------------------------------------------------------------------------
+constexpr uint32_t kNumXcd = 8;
+
+__device__ __forceinline__ uint32_t xcd_logical_block()
+    {
+    const uint32_t nb = gridDim.x;
+    const uint32_t b = blockIdx.x;
+    const uint32_t x = b % kNumXcd;
+    const uint32_t idx = b / kNumXcd;
+    const uint32_t q = nb / kNumXcd;
+    const uint32_t r = nb % kNumXcd;
+    return x * q + min(x, r) + idx;
+    }
+
+// ---------------------------------------------------------------------------
 // Precision-generic math.  FP64 keeps the IEEE operations of the reference
 // expressions; FP32 uses the hardware rcp/sqrt/exp/log units (about 1 ulp).
 // ---------------------------------------------------------------------------

@@ -252,7 +252,7 @@ __global__ void __launch_bounds__(256) k_adjoint(const AdjArgs a)
         }
     __syncthreads();
 
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = xcd_logical_block() * blockDim.x + threadIdx.x;
     double acc[NT * kAdjNpar];
 #pragma unroll
     for (int k = 0; k < NT * kAdjNpar; ++k)

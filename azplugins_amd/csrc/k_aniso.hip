@@ -142,7 +142,7 @@ __global__ void __launch_bounds__(512) k_aniso(const AnisoKernelArgs a)
             s_q[t] = tpm_prepare<R>(a.params[t], a.rcutsq[t], a.shift_mode == AZP_SHIFT_SHIFT);
         __syncthreads();
         }
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = xcd_logical_block() * blockDim.x + threadIdx.x;
     if (i >= a.N)
         return;
 

@@ -90,7 +90,7 @@ __global__ void __launch_bounds__(512) k_dpd(const DPDKernelArgs a)
         s_p[t].rcutsq = a.rcutsq[t];
         }
     __syncthreads();
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = xcd_logical_block() * blockDim.x + threadIdx.x;
     if (i >= a.N)
         return;
     const double4 pi = a.pos[i];

@@ -238,7 +238,7 @@ __device__ uint32_t walk_one(const WalkArgs& a, uint32_t i)
 // entries with no per-entry bounds test (padding entries have j == i).
 __global__ void __launch_bounds__(256) k_walk(const WalkArgs a)
     {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = xcd_logical_block() * blockDim.x + threadIdx.x;
     const bool active = i < a.N;
     const uint32_t count = active ? walk_one(a, i) : 0u;
     if (active)

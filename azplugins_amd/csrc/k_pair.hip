@@ -290,10 +290,11 @@ __global__ void __launch_bounds__(512) k_pair(const PairKernelArgs a)
         __syncthreads();
         }
 
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t lb = xcd_logical_block();
+    const uint32_t i = lb * blockDim.x + threadIdx.x;
     // wavefronts entirely past N leave together; lanes past N in the last
     // wavefront shadow particle N-1 (same column, results not stored)
-    if (blockIdx.x * blockDim.x + (threadIdx.x & ~63u) >= a.N)
+    if (lb * blockDim.x + (threadIdx.x & ~63u) >= a.N)
         return;
     const uint32_t ie = min(i, a.N - 1);
     const uint32_t nn = a.n_neigh[ie];
