@@ -97,6 +97,44 @@ class PairArgs(ctypes.Structure):
     ]
 
 
+class BatchArrays(ctypes.Structure):
+    _fields_ = [
+        ("d_pos", ctypes.c_void_p),
+        ("d_pos_fixed", ctypes.c_void_p),
+        ("d_orientation", ctypes.c_void_p),
+        ("d_vel", ctypes.c_void_p),
+        ("d_angmom", ctypes.c_void_p),
+        ("d_accel", ctypes.c_void_p),
+        ("d_force", ctypes.c_void_p),
+        ("d_torque", ctypes.c_void_p),
+        ("d_virial", ctypes.c_void_p),
+        ("virial_pitch", ctypes.c_uint32),
+    ]
+
+
+class BatchTuning(ctypes.Structure):
+    _fields_ = [
+        ("block_size", ctypes.c_uint32),
+        ("grab", ctypes.c_uint32),
+        ("workgroups_per_cu", ctypes.c_uint32),
+        ("walk_block", ctypes.c_uint32),
+        ("force_us", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class BatchStats(ctypes.Structure):
+    _fields_ = [
+        ("timestep", ctypes.c_uint64),
+        ("n_builds", ctypes.c_uint64),
+        ("graph_launches", ctypes.c_uint64),
+        ("max_neighbors", ctypes.c_uint32),
+        ("n_cells", ctypes.c_uint32),
+    ]
+
+
 # (name, restype, argtypes) of every entry point in include/azp.h
 _vp = ctypes.c_void_p
 _pd = ctypes.POINTER(ctypes.c_double)
@@ -145,6 +183,25 @@ SIGNATURES = [
     ("azp_pair_set_thermostat", _i, [_vp, ctypes.c_uint16, ctypes.c_double, ctypes.c_double]),
     ("azp_pair_get_r_cut", _i, [_vp, _pd]),
     ("azp_pair_compute", _i, [_vp, _vp, ctypes.POINTER(Particles), ctypes.POINTER(Outputs), _u64, _i, _vp]),
+    ("azp_batch_create", _i, [ctypes.POINTER(_vp), _u32, ctypes.POINTER(_u32), ctypes.POINTER(Box), _u32, _i]),
+    ("azp_batch_destroy", _i, [_vp]),
+    ("azp_batch_size", _i, [_vp, ctypes.POINTER(_u32), ctypes.POINTER(_u32)]),
+    ("azp_batch_set_params", _i, [_vp, _u32, _u32, _pd]),
+    ("azp_batch_set_r_cut", _i, [_vp, _u32, _u32, ctypes.c_double]),
+    ("azp_batch_set_r_buff", _i, [_vp, ctypes.c_double]),
+    ("azp_batch_set_mode", _i, [_vp, _i, _i, _i]),
+    ("azp_batch_set_langevin", _i, [_vp, _pd, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                    ctypes.c_double, ctypes.c_uint16]),
+    ("azp_batch_bind", _i, [_vp, ctypes.POINTER(BatchArrays)]),
+    ("azp_batch_set_timestep", _i, [_vp, _u64, _vp]),
+    ("azp_batch_compute", _i, [_vp, _i, _vp]),
+    ("azp_batch_run", _i, [_vp, _u32, _i, _vp]),
+    ("azp_batch_thermo", _i, [_vp, _vp, _vp]),
+    ("azp_batch_autotune", _i, [_vp, ctypes.POINTER(BatchTuning), _vp]),
+    ("azp_batch_get_tuning", _i, [_vp, ctypes.POINTER(BatchTuning)]),
+    ("azp_batch_set_tuning", _i, [_vp, ctypes.POINTER(BatchTuning)]),
+    ("azp_batch_stats", _i, [_vp, ctypes.POINTER(BatchStats)]),
+    ("azp_batch_nlist_download", _i, [_vp, _vp, _vp, _u64, ctypes.POINTER(_u32), ctypes.POINTER(_u32)]),
 ]
 
 _lib = None

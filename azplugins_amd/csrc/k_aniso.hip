@@ -8,7 +8,7 @@
 // (i, j) exactly in exact arithmetic (both are dU/dgamma_j (r_hat x n_j)).
 #include <type_traits>
 
-#include "device_common.h"
+#include "tpm.h"
 
 namespace azp
 {
@@ -34,96 +34,6 @@ struct AnisoKernelArgs
     uint32_t vpitch;
     };
 
-template<typename R> struct TPMPrep
-    {
-    R M_d, M_rinv, r_eq, omega, alpha;
-    R rcsq, rcsq_lo, rcsq_hi;
-    R ushift; // U_Morse(rc), used when shifting
-    double rcsq64;
-    int repulsion;
-    int shift;
-    };
-
-template<typename R>
-__device__ TPMPrep<R> tpm_prepare(const azp_param_two_patch_morse& p, double rcsq, bool shift)
-    {
-    TPMPrep<R> q;
-    q.M_d = (R)p.M_d;
-    q.M_rinv = (R)p.M_rinv;
-    q.r_eq = (R)p.r_eq;
-    q.omega = (R)p.omega;
-    q.alpha = (R)p.alpha;
-    q.rcsq = (R)rcsq;
-    q.rcsq_lo = (R)(rcsq * (1.0 - 1e-5));
-    q.rcsq_hi = (R)(rcsq * (1.0 + 1e-5));
-    q.rcsq64 = rcsq;
-    q.repulsion = p.repulsion ? 1 : 0;
-    q.shift = shift ? 1 : 0;
-    const double rcut = sqrt(rcsq);
-    const double me = exp(-(rcut - p.r_eq) * p.M_rinv);
-    const double ome = 1.0 - me;
-    q.ushift = (R)(p.M_d * (ome * ome - 1.0));
-    return q;
-    }
-
-// n = rotate(q, (1,0,0)) = (s^2 - |v|^2) e_x + 2 s (v x e_x) + 2 (v . e_x) v
-template<typename R> __device__ __forceinline__ void director(R s, R vx, R vy, R vz, R n[3])
-    {
-    const R a = s * s - (vx * vx + vy * vy + vz * vz);
-    n[0] = a + R(2) * vx * vx;
-    n[1] = R(2) * s * vz + R(2) * vx * vy;
-    n[2] = -R(2) * s * vy + R(2) * vx * vz;
-    }
-
-template<typename R>
-__device__ __forceinline__ void tpm_eval(const TPMPrep<R>& q,
-                                         const R dr[3],
-                                         R rsq,
-                                         const R n_i[3],
-                                         const R n_j[3],
-                                         R f[3],
-                                         R t_i[3],
-                                         R& e)
-    {
-    const R rinv = f_rsqrt(rsq);
-    const R r = R(1) / rinv;
-    const R u[3] = {dr[0] * rinv, dr[1] * rinv, dr[2] * rinv};
-    R UMorse = R(-1) * q.M_d;
-    R dUMorse_dr = R(0);
-    if (r > q.r_eq || q.repulsion)
-        {
-        const R me = f_exp(-(r - q.r_eq) * q.M_rinv);
-        const R ome = R(1) - me;
-        UMorse = q.M_d * (ome * ome - R(1));
-        dUMorse_dr = R(2) * q.M_d * q.M_rinv * me * ome;
-        }
-    const R gi = u[0] * n_i[0] + u[1] * n_i[1] + u[2] * n_i[2];
-    const R gie = f_exp(-q.omega * (gi * gi - q.alpha));
-    const R Oi = R(1) / (R(1) + gie);
-    const R gj = u[0] * n_j[0] + u[1] * n_j[1] + u[2] * n_j[2];
-    const R gje = f_exp(-q.omega * (gj * gj - q.alpha));
-    const R Oj = R(1) / (R(1) + gje);
-    e = UMorse * Oi * Oj;
-    const R dU_dr = dUMorse_dr * Oi * Oj;
-    const R dOi = R(2) * q.omega * gi * gie * Oi * Oi;
-    const R dOj = R(2) * q.omega * gj * gje * Oj * Oj;
-    const R dU_dgi = dOi * UMorse * Oj;
-    const R dU_dgj = dOj * UMorse * Oi;
-    // c = u x n ; n_perp = (-u) x c
-    const R ci[3] = {u[1] * n_i[2] - u[2] * n_i[1], u[2] * n_i[0] - u[0] * n_i[2], u[0] * n_i[1] - u[1] * n_i[0]};
-    const R cj[3] = {u[1] * n_j[2] - u[2] * n_j[1], u[2] * n_j[0] - u[0] * n_j[2], u[0] * n_j[1] - u[1] * n_j[0]};
-    const R pi[3] = {-u[1] * ci[2] + u[2] * ci[1], -u[2] * ci[0] + u[0] * ci[2], -u[0] * ci[1] + u[1] * ci[0]};
-    const R pj[3] = {-u[1] * cj[2] + u[2] * cj[1], -u[2] * cj[0] + u[0] * cj[2], -u[0] * cj[1] + u[1] * cj[0]};
-#pragma unroll
-    for (int k = 0; k < 3; ++k)
-        {
-        f[k] = -dU_dr * u[k] - rinv * (dU_dgi * pi[k] + dU_dgj * pj[k]);
-        t_i[k] = dU_dgi * ci[k];
-        }
-    if (q.shift)
-        e -= q.ushift * Oi * Oj;
-    }
-
 template<bool MIXED, bool VIRIAL, bool SINGLE>
 __global__ void __launch_bounds__(512) k_aniso(const AnisoKernelArgs a)
     {
@@ -146,94 +56,20 @@ __global__ void __launch_bounds__(512) k_aniso(const AnisoKernelArgs a)
     if (i >= a.N)
         return;
 
-    const double4 oi = a.orientation[i];
-    R n_i[3];
-    director<R>((R)oi.x, (R)oi.y, (R)oi.z, (R)oi.w, n_i);
-    double fx = 0, fy = 0, fz = 0, tx = 0, ty = 0, tz = 0, pe = 0;
-    double v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0, v5 = 0;
+    const AnisoSelf<R> me = aniso_self<MIXED>(a.pos, a.pos_fixed, a.orientation, i);
+    LaneBox box;
+    for (int k = 0; k < 3; ++k)
+        {
+        box.L[k] = a.box.L[k];
+        box.Linv[k] = a.box.Linv[k];
+        box.fx_scale[k] = a.fx_scale[k];
+        }
+    AnisoAcc acc;
     const uint32_t nn = a.n_neigh[i];
-    const double4 pi64 = a.pos[i];
-    int4 pfi;
-    if constexpr (MIXED)
-        pfi = a.pos_fixed[i];
-    const uint32_t ti = (uint32_t)pi64.w;
     for (uint32_t k = 0; k < nn; ++k)
-        {
-        const uint32_t j = a.nlist[(size_t)k * a.nlist_pitch + i];
-        R dr[3];
-        R rsq;
-        uint32_t tj;
-        bool in;
-        if constexpr (!MIXED)
-            {
-            const double4 pj = a.pos[j];
-            dr[0] = min_image_1d(pi64.x - pj.x, a.box.L[0], a.box.Linv[0]);
-            dr[1] = min_image_1d(pi64.y - pj.y, a.box.L[1], a.box.Linv[1]);
-            dr[2] = min_image_1d(pi64.z - pj.z, a.box.L[2], a.box.Linv[2]);
-            rsq = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
-            tj = (uint32_t)pj.w;
-            const TPMPrep<R>& q = SINGLE ? q0 : s_q[ti * nt + tj];
-            in = !(rsq > q.rcsq); // AnisoPairEvaluatorTwoPatchMorse.h:135
-            }
-        else
-            {
-            const int4 pj = a.pos_fixed[j];
-            dr[0] = (R)fixed_delta(pfi.x, pj.x) * a.fx_scale[0];
-            dr[1] = (R)fixed_delta(pfi.y, pj.y) * a.fx_scale[1];
-            dr[2] = (R)fixed_delta(pfi.z, pj.z) * a.fx_scale[2];
-            rsq = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
-            tj = (uint32_t)pj.w;
-            const TPMPrep<R>& q = SINGLE ? q0 : s_q[ti * nt + tj];
-            in = !(rsq > q.rcsq_hi);
-            if (in && rsq > q.rcsq_lo)
-                {
-                const double4 pj64 = a.pos[j];
-                const double ex = min_image_1d(pi64.x - pj64.x, a.box.L[0], a.box.Linv[0]);
-                const double ey = min_image_1d(pi64.y - pj64.y, a.box.L[1], a.box.Linv[1]);
-                const double ez = min_image_1d(pi64.z - pj64.z, a.box.L[2], a.box.Linv[2]);
-                const double r64 = rsq3_nofma(ex, ey, ez);
-                in = !(r64 > q.rcsq64);
-                }
-            }
-        if (!in)
-            continue;
-        const TPMPrep<R>& q = SINGLE ? q0 : s_q[ti * nt + tj];
-        const double4 oj = a.orientation[j];
-        R n_j[3];
-        director<R>((R)oj.x, (R)oj.y, (R)oj.z, (R)oj.w, n_j);
-        R f[3], t[3], e;
-        tpm_eval<R>(q, dr, rsq, n_i, n_j, f, t, e);
-        const double Fx = f[0], Fy = f[1], Fz = f[2];
-        fx += Fx;
-        fy += Fy;
-        fz += Fz;
-        tx += (double)t[0];
-        ty += (double)t[1];
-        tz += (double)t[2];
-        pe += (double)e;
-        if (VIRIAL)
-            {
-            const double dx = dr[0], dy = dr[1], dz = dr[2];
-            v0 = fma(dx, Fx, v0);
-            v1 = fma(dy, Fx, v1);
-            v2 = fma(dz, Fx, v2);
-            v3 = fma(dy, Fy, v3);
-            v4 = fma(dz, Fy, v4);
-            v5 = fma(dz, Fz, v5);
-            }
-        }
-    a.force[i] = make_double4(fx, fy, fz, 0.5 * pe);
-    a.torque[i] = make_double4(tx, ty, tz, 0.0);
-    if (VIRIAL)
-        {
-        const size_t p = a.vpitch;
-        a.virial[0 * p + i] = 0.5 * v0;
-        a.virial[1 * p + i] = 0.5 * v1;
-        a.virial[2 * p + i] = 0.5 * v2;
-        a.virial[3 * p + i] = 0.5 * v3;
-        a.virial[4 * p + i] = 0.5 * v4;
-        a.virial[5 * p + i] = 0.5 * v5;
-        }
+        tpm_entry<MIXED, VIRIAL, SINGLE>(a.nlist[(size_t)k * a.nlist_pitch + i], a.pos, a.pos_fixed,
+                                         a.orientation, me, box, nt, q0, s_q, acc);
+    aniso_store<VIRIAL>(a.force, a.torque, a.virial, a.vpitch, i, acc);
     }
 
 template<bool MIXED, bool VIRIAL, bool SINGLE>

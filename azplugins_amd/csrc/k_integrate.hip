@@ -180,11 +180,14 @@ __global__ void k_rot_two(double4* __restrict__ angmom,
         a.v[0] = r0.v[0]; a.v[1] = r0.v[1];
         b.v[0] = r0.v[2]; b.v[1] = r0.v[3];
         d.v[0] = r1.v[0]; d.v[1] = r1.v[1];
+        // the rotational thermostat acts once per step (the first half step
+        // uses the pair torque only), so its drag and noise take the full dt:
+        // <L_a^2> = I kT at equilibrium
         const double I = inertia[i];
         const double coeff = sqrt(6.0 * gamma_r * kT / dt);
-        tx += -gamma_r * L.x / I + coeff * uniform_m1_1(a);
-        ty += -gamma_r * L.y / I + coeff * uniform_m1_1(b);
-        tz += -gamma_r * L.z / I + coeff * uniform_m1_1(d);
+        tx += 2.0 * (-gamma_r * L.x / I + coeff * uniform_m1_1(a));
+        ty += 2.0 * (-gamma_r * L.y / I + coeff * uniform_m1_1(b));
+        tz += 2.0 * (-gamma_r * L.z / I + coeff * uniform_m1_1(d));
         }
     L.x += 0.5 * dt * tx;
     L.y += 0.5 * dt * ty;

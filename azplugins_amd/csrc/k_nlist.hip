@@ -22,27 +22,10 @@
 #include <cmath>
 #include <vector>
 
-#include "device_common.h"
+#include "cells.h"
 
 namespace azp
 {
-
-struct CellGrid
-    {
-    int n[3];
-    double w[3];
-    };
-
-__device__ __forceinline__ int cell_coord(double x, double L, double Linv, int n)
-    {
-    // fractional coordinate in [0, 1) for x in [-L/2, L/2)
-    double f = (x + 0.5 * L) * Linv;
-    int c = (int)floor(f * (double)n);
-    c %= n;
-    if (c < 0)
-        c += n;
-    return c;
-    }
 
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n)
     {
@@ -152,12 +135,6 @@ struct WalkArgs
     uint32_t* overflow; // [0] largest count seen when > max_neighbors, [2] bad type seen
     };
 
-// Type index of a particle, or ntypes when pos.w is not a valid type index.
-__device__ __forceinline__ uint32_t checked_type(double w, uint32_t ntypes)
-    {
-    return (w >= 0.0 && w < (double)ntypes) ? (uint32_t)w : ntypes;
-    }
-
 // Lists particle i's neighbors and returns how many it has.
 __device__ uint32_t walk_one(const WalkArgs& a, uint32_t i)
     {
@@ -174,30 +151,11 @@ __device__ uint32_t walk_one(const WalkArgs& a, uint32_t i)
     const int cx = (int)(c % (uint32_t)a.g.n[0]);
     const int cy = (int)((c / (uint32_t)a.g.n[0]) % (uint32_t)a.g.n[1]);
     const int cz = (int)(c / ((uint32_t)a.g.n[0] * (uint32_t)a.g.n[1]));
-    // neighbor offsets per axis, deduplicated when the grid is narrower than 3
+    // neighbor cells per axis, deduplicated when the grid is narrower than 3
     int ox[3], oy[3], oz[3];
-    int nx = 0, ny = 0, nz = 0;
-    for (int d = -1; d <= 1; ++d)
-        {
-        const int x = (cx + d + a.g.n[0]) % a.g.n[0];
-        bool dup = false;
-        for (int m = 0; m < nx; ++m)
-            dup |= (ox[m] == x);
-        if (!dup)
-            ox[nx++] = x;
-        const int y = (cy + d + a.g.n[1]) % a.g.n[1];
-        dup = false;
-        for (int m = 0; m < ny; ++m)
-            dup |= (oy[m] == y);
-        if (!dup)
-            oy[ny++] = y;
-        const int z = (cz + d + a.g.n[2]) % a.g.n[2];
-        dup = false;
-        for (int m = 0; m < nz; ++m)
-            dup |= (oz[m] == z);
-        if (!dup)
-            oz[nz++] = z;
-        }
+    const int nx = stencil_axis(cx, a.g.n[0], ox);
+    const int ny = stencil_axis(cy, a.g.n[1], oy);
+    const int nz = stencil_axis(cz, a.g.n[2], oz);
     uint32_t count = 0;
     for (int iz = 0; iz < nz; ++iz)
         for (int iy = 0; iy < ny; ++iy)
@@ -243,7 +201,9 @@ __global__ void __launch_bounds__(256) k_walk(const WalkArgs a)
     const uint32_t count = active ? walk_one(a, i) : 0u;
     if (active)
         {
-        a.n_neigh[i] = count;
+        // a list longer than the capacity is truncated (and reported), so
+        // consumers never read past the allocation
+        a.n_neigh[i] = min(count, a.max_neighbors);
         if (count > a.max_neighbors)
             atomicMax(a.overflow, count);
         }
@@ -335,28 +295,9 @@ static int nlist_build(azp_nlist* nl, const azp_particles* p, hipStream_t s)
     const uint32_t N = p->N;
     const DevBox box = make_box(p->box);
     const double rmax = nl->r_list_max();
-    CellGrid g;
     const double L[3] = {p->box.Lx, p->box.Ly, p->box.Lz};
-    size_t ncell = 1;
-    for (int k = 0; k < 3; ++k)
-        {
-        int n = (rmax > 0.0) ? (int)std::floor(L[k] / (rmax * (1.0 + 1e-9))) : 1;
-        n = std::max(1, std::min(n, 1024));
-        g.n[k] = n;
-        g.w[k] = L[k] / n;
-        ncell *= (size_t)n;
-        }
-    // cap the grid at ~4 cells per particle (dilute systems)
-    while (ncell > 4 * (size_t)std::max<uint32_t>(N, 64) + 64)
-        {
-        int kk = 0;
-        for (int k = 1; k < 3; ++k)
-            if (g.n[k] > g.n[kk])
-                kk = k;
-        g.n[kk] = std::max(1, g.n[kk] / 2);
-        g.w[kk] = L[kk] / g.n[kk];
-        ncell = (size_t)g.n[0] * g.n[1] * g.n[2];
-        }
+    const CellGrid g = cell_grid_for(L, rmax, N);
+    const size_t ncell = (size_t)g.n[0] * g.n[1] * g.n[2];
 
     AZP_TRY(nl->cell_of.reserve(N));
     AZP_TRY(nl->slot_of.reserve(N));

@@ -263,7 +263,9 @@ int azp_gpu_integrate_vv_two(double* d_vel,
                              uint16_t seed,
                              uint64_t timestep,
                              void* stream);
-/* spherical tops: angmom double4[N] space-frame L, inertia double[N] (isotropic I) */
+/* spherical tops: angmom double4[N] space-frame L, inertia double[N] (isotropic I).
+ * rot_one: L += dt tau / 2, then rotate q by omega dt (omega = L / I).
+ * rot_two: L += dt tau / 2 [+ dt (-gamma_r L / I + sqrt(6 gamma_r kT / dt) U(-1,1))] */
 int azp_gpu_integrate_rot_one(double* d_orientation,
                               double* d_angmom,
                               const double* d_torque,
@@ -344,6 +346,89 @@ int azp_pair_compute(azp_pair* pp,
                      uint64_t timestep,
                      int compute_virial,
                      void* stream);
+
+/* ---- batched replicas (no single reference counterpart: R independent
+ * AnisoPotentialPair<AnisoPairEvaluatorTwoPatchMorse> systems,
+ * AnisoPotentialPairGPUKernel.cu.inc:21-25, each with its own periodic box,
+ * size and Langevin kT, advanced together) ----
+ * Particles of replica r occupy [offset_r, offset_r + size_r) of every array,
+ * offsets being the prefix sums of the sizes; positions are in the replica's
+ * own box frame.  The arrays stay owned by the caller and must outlive the
+ * binding: captured graphs hold their addresses. */
+typedef struct azp_batch azp_batch;
+typedef struct
+    {
+    double* d_pos;         /* double4[N_total] x, y, z, type */
+    int32_t* d_pos_fixed;  /* int4[N_total], mixed precision (written by the library) */
+    double* d_orientation; /* double4[N_total] quaternion s, x, y, z */
+    double* d_vel;         /* double4[N_total] vx, vy, vz, mass      (time steps) */
+    double* d_angmom;      /* double4[N_total] angular momentum      (time steps) */
+    double* d_accel;       /* double4[N_total] acceleration          (time steps) */
+    double* d_force;       /* double4[N_total] out: F, energy */
+    double* d_torque;      /* double4[N_total] out: torque */
+    double* d_virial;      /* double[6 * virial_pitch] out (may be NULL without virial) */
+    uint32_t virial_pitch;
+    } azp_batch_arrays;
+/* Launch configuration of the persistent force kernel and of the walk. */
+typedef struct
+    {
+    uint32_t block_size;        /* force kernel workgroup: 64..512, multiple of 64 */
+    uint32_t grab;              /* 64-particle chunks taken per work-queue fetch */
+    uint32_t workgroups_per_cu; /* persistent grid = CUs x this; 0 = one wave per chunk */
+    uint32_t walk_block;        /* neighbor walk workgroup: 64, 128 or 256 */
+    double force_us;            /* force-kernel time measured by the autotuner */
+    } azp_batch_tuning;
+typedef struct
+    {
+    uint64_t timestep;
+    uint64_t n_builds;
+    uint64_t graph_launches;
+    uint32_t max_neighbors;
+    uint32_t n_cells;
+    } azp_batch_stats_t;
+/* Host-side validation only; device memory is allocated on first use. */
+int azp_batch_create(azp_batch** out,
+                     uint32_t n_replicas,
+                     const uint32_t* sizes,
+                     const azp_box* boxes,
+                     uint32_t ntypes,
+                     int evaluator);
+int azp_batch_destroy(azp_batch* b);
+int azp_batch_size(const azp_batch* b, uint32_t* n_replicas, uint32_t* n_total);
+int azp_batch_set_params(azp_batch* b, uint32_t ti, uint32_t tj, const double* values);
+int azp_batch_set_r_cut(azp_batch* b, uint32_t ti, uint32_t tj, double r_cut);
+int azp_batch_set_r_buff(azp_batch* b, double r_buff);
+int azp_batch_set_mode(azp_batch* b, int shift_mode, int precision, int compute_virial);
+/* kT: host array, one value per replica */
+int azp_batch_set_langevin(azp_batch* b,
+                           const double* kT,
+                           double dt,
+                           double gamma,
+                           double gamma_r,
+                           double inertia,
+                           uint16_t seed);
+int azp_batch_bind(azp_batch* b, const azp_batch_arrays* arrays);
+int azp_batch_set_timestep(azp_batch* b, uint64_t timestep, void* stream);
+/* Forces, torques, energies and virials of every replica in one persistent
+ * launch (the neighbor list is rebuilt when forced or when a particle moved
+ * more than r_buff / 2). */
+int azp_batch_compute(azp_batch* b, int force_rebuild, void* stream);
+/* nsteps Langevin steps (translation + rotation); use_graph replays captured
+ * HIP graphs of 10 steps (and of 1 step for the remainder). */
+int azp_batch_run(azp_batch* b, uint32_t nsteps, int use_graph, void* stream);
+/* d_out: double[4 * R]: translational KE, rotational KE, energy, virial trace */
+int azp_batch_thermo(azp_batch* b, double* d_out, void* stream);
+/* Times the launch candidates on the live batch and keeps the fastest. */
+int azp_batch_autotune(azp_batch* b, azp_batch_tuning* out, void* stream);
+int azp_batch_get_tuning(const azp_batch* b, azp_batch_tuning* out);
+int azp_batch_set_tuning(azp_batch* b, const azp_batch_tuning* tuning);
+int azp_batch_stats(azp_batch* b, azp_batch_stats_t* out);
+int azp_batch_nlist_download(azp_batch* b,
+                             uint32_t* h_n_neigh,
+                             uint32_t* h_nlist,
+                             uint64_t capacity,
+                             uint32_t* pitch,
+                             uint32_t* max_neighbors);
 
 #ifdef __cplusplus
 }

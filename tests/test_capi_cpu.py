@@ -120,3 +120,57 @@ def test_harness_api_shape():
         p.params[("A", "A")] = dict(M_d=1.0)
     p.params[("A", "A")] = dict(M_d=1, M_r=0.2, r_eq=1.0, omega=20, alpha=0.4, repulsion=True)
     assert p.params[("A", "A")]["repulsion"] is True
+
+
+def _batch(sizes, boxes, ntypes=1, evaluator=5):
+    h = ctypes.c_void_p()
+    R = len(sizes)
+    cs = (ctypes.c_uint32 * R)(*sizes)
+    cb = (_azp.Box * R)(*[_azp.Box(*b) for b in boxes])
+    st = _azp.lib().azp_batch_create(ctypes.byref(h), R, cs, cb, ntypes, evaluator)
+    return st, h
+
+
+def test_batch_host_side_validation():
+    """azp_batch_* argument checks run before any device work."""
+    L = _azp.lib()
+    assert _batch([], [])[0] == 1
+    assert _batch([10, 0], [(9.0,) * 3] * 2)[0] == 1
+    assert _batch([10], [(9.0, -1.0, 9.0)])[0] == 3
+    assert _batch([10], [(9.0,) * 3], evaluator=2)[0] == 4
+    st, h = _batch([500, 700, 1200], [(9.41,) * 3, (10.5,) * 3, (12.6,) * 3])
+    assert st == 0
+    try:
+        R = ctypes.c_uint32()
+        N = ctypes.c_uint32()
+        _azp.call("azp_batch_size", h, ctypes.byref(R), ctypes.byref(N))
+        assert (R.value, N.value) == (3, 2400)
+        assert L.azp_batch_set_mode(h, 2, 1, 1) == 4  # xplor: anisotropic pairs take none/shift
+        assert L.azp_batch_set_mode(h, 1, 7, 1) == 1
+        _azp.call("azp_batch_set_mode", h, 0, 1, 1)
+        assert L.azp_batch_set_tuning(h, ctypes.byref(_azp.BatchTuning(96, 1, 4, 256, 0.0))) == 1
+        assert L.azp_batch_set_tuning(h, ctypes.byref(_azp.BatchTuning(256, 0, 4, 256, 0.0))) == 1
+        assert L.azp_batch_set_tuning(h, ctypes.byref(_azp.BatchTuning(256, 2, 4, 32, 0.0))) == 1
+        _azp.call("azp_batch_set_tuning", h, ctypes.byref(_azp.BatchTuning(128, 2, 8, 64, 0.0)))
+        t = _azp.BatchTuning()
+        _azp.call("azp_batch_get_tuning", h, ctypes.byref(t))
+        assert (t.block_size, t.grab, t.workgroups_per_cu, t.walk_block) == (128, 2, 8, 64)
+        # compute before binding arrays / setting parameters
+        assert L.azp_batch_compute(h, 1, None) == 6
+        a = _azp.BatchArrays()
+        for f, _ in a._fields_[:-1]:
+            setattr(a, f, 0x1000)  # never dereferenced: validation fails first
+        a.virial_pitch = 2432
+        _azp.call("azp_batch_bind", h, ctypes.byref(a))
+        assert L.azp_batch_compute(h, 1, None) == 6  # params not set
+        vals = _azp.dbl_array([1.0, 0.2, 1.0, 20.0, 0.4, 1.0])
+        _azp.call("azp_batch_set_params", h, 0, 0, vals)
+        assert L.azp_batch_compute(h, 1, None) == 6  # r_cut not set
+        _azp.call("azp_batch_set_r_cut", h, 0, 0, 2.5)
+        _azp.call("azp_batch_set_r_buff", h, 2.5)  # r_list 5.0 > 9.41 / 2
+        assert L.azp_batch_compute(h, 1, None) == 3
+        assert b"replica 0" in L.azp_last_error()
+        assert L.azp_batch_set_langevin(h, _azp.dbl_array([1.0, -1.0, 1.0]), 0.005, 1.0, 1.0, 1.0, 0) == 1
+        assert L.azp_batch_set_langevin(h, _azp.dbl_array([1.0] * 3), 0.0, 1.0, 1.0, 1.0, 0) == 1
+    finally:
+        L.azp_batch_destroy(h)

@@ -1,0 +1,217 @@
+"""Batched TwoPatchMorse replicas (azp_batch_*): parity, autotuning, graphs.
+
+* neighbor sets of every replica equal the oracle's cell-list sets,
+* forces, energies, torques and virials of every replica from the one
+  persistent launch agree with the FP64 oracle (1e-12, plain form), with the
+  single-system anisotropic launcher (1e-12) and, in the mixed form, with the
+  plain form (1e-5),
+* the autotuner picks a candidate and leaves the results bit-identical,
+* captured graphs replay the eager steps bit for bit, replay after replay,
+* Langevin at a different kT per replica gives each replica its own
+  kinetic temperature,
+* the full 512-replica workload (log-uniform sizes in [500, 20000]).
+"""
+
+import numpy as np
+import pytest
+
+import oracle
+from azplugins_amd import _azp, configs
+from azplugins_amd.batch import ReplicaBatch, from_config
+from gpu_util import GpuSystem, gpu_pair, rel_err, say
+
+torch = _azp.torch
+pytestmark = pytest.mark.gpu
+
+TPM = {(0, 0): {"M_d": 1.0, "M_r": 0.2, "r_eq": 1.0, "omega": 20.0, "alpha": 0.4, "repulsion": True}}
+RC = 2.5
+
+
+def small_config(sizes=(500, 777, 1500, 2600, 4100), seed=11, density=0.6):
+    rng = np.random.default_rng(seed)
+    reps = []
+    for n in sizes:
+        pos4, Lv = configs.tpm_replica(int(n), density, rng)
+        reps.append(dict(pos4=pos4, L=Lv, quat=configs.random_unit_quaternions(int(n), rng),
+                         kT=float(rng.uniform(0.2, 1.0))))
+    return dict(replicas=reps, ntypes=1, r_cut=RC, r_buff=0.4, params=TPM, dt=0.005)
+
+
+def host(t):
+    return t.detach().cpu().numpy()
+
+
+def test_batch_nlist_and_forces_match_oracle():
+    cfg = small_config()
+    say(f"[batch] small batch {[r['pos4'].shape[0] for r in cfg['replicas']]}")
+    b = from_config(cfg, precision="fp64")
+    b.compute(force_rebuild=True)
+    torch.cuda.synchronize()
+    nn, lst = b.nlist()
+    f = host(b.force)
+    t = host(b.torque)
+    v = host(b.virial)
+    packed = oracle.pack_matrix(oracle.TWO_PATCH_MORSE, TPM, 1)
+    for r, rep in enumerate(cfg["replicas"]):
+        sl = b.replica(r)
+        lo = sl.start
+        n = sl.stop - sl.start
+        pairs = oracle.nlist(rep["pos4"], rep["L"], 1, oracle.r_listsq_matrix([RC], 0.4), "cell")
+        ref_set = set(zip(pairs[0].tolist(), pairs[1].tolist()))
+        got = set()
+        for i in range(n):
+            js = lst[: nn[lo + i], lo + i].astype(np.int64)
+            assert np.all((js >= lo) & (js < lo + n)), "neighbor outside its replica"
+            for j in (js - lo).tolist():
+                got.add((min(i, j), max(i, j)))
+        assert got == ref_set, f"replica {r}: neighbor sets differ"
+        fo, to, vo = oracle.aniso_forces(rep["pos4"], rep["quat"], rep["L"], pairs, 1, packed, np.array([RC * RC]))
+        errs = [rel_err(f[sl, :3], fo[:, :3]), rel_err(f[sl, 3], fo[:, 3]), rel_err(t[sl, :3], to[:, :3]),
+                rel_err(v[:, sl], vo)]
+        say(f"  replica {r} N={n}: fp64 rel err force/energy/torque/virial", ["%.1e" % e for e in errs])
+        assert max(errs) < 1e-12
+
+
+def test_batch_matches_single_system_launcher_and_mixed():
+    cfg = small_config(sizes=(900, 3000, 1200), seed=12)
+    b = from_config(cfg, precision="fp64")
+    b.compute(force_rebuild=True)
+    f64, t64, v64 = host(b.force).copy(), host(b.torque).copy(), host(b.virial).copy()
+    for r, rep in enumerate(cfg["replicas"]):
+        sl = b.replica(r)
+        single = gpu_pair("AnisoPotentialPairTwoPatchMorse", GpuSystem(rep["pos4"], rep["L"], quat=rep["quat"]), 1,
+                          TPM, RC)
+        errs = [rel_err(f64[sl], single.force), rel_err(t64[sl, :3], single.torque[:, :3]),
+                rel_err(v64[:, sl], single.virial)]
+        say(f"[batch] replica {r} vs single-system launcher", ["%.1e" % e for e in errs])
+        assert max(errs) < 1e-12
+    b.set_precision("mixed")
+    b.compute()
+    errs = [rel_err(host(b.force)[:, :3], f64[:, :3]), rel_err(host(b.force)[:, 3], f64[:, 3]),
+            rel_err(host(b.torque)[:, :3], t64[:, :3]), rel_err(host(b.virial), v64)]
+    say("[batch] mixed vs fp64 force/energy/torque/virial", ["%.1e" % e for e in errs])
+    assert max(errs) < 1e-5
+
+
+def test_autotune_keeps_results_bit_identical():
+    cfg = small_config(sizes=(500, 2500, 7000, 1000), seed=13)
+    b = from_config(cfg, precision="mixed")
+    b.set_tuning(256, 1, 4, 256)
+    b.compute(force_rebuild=True)
+    f0, t0 = b.force.clone(), b.torque.clone()
+    tune = b.autotune()
+    say(f"[batch] autotune -> {tune}")
+    assert tune["block_size"] in (64, 128, 256, 512)
+    assert tune["grab"] in (1, 2, 4, 8)
+    assert tune["workgroups_per_cu"] in (0, 1, 2, 4, 8, 16)
+    assert tune["walk_block"] in (64, 128, 256)
+    assert tune["force_us"] > 0
+    assert b.tuning["block_size"] == tune["block_size"]
+    torch.cuda.synchronize()
+    assert torch.equal(b.force, f0) and torch.equal(b.torque, t0)
+    for blk, grab, wg in ((64, 8, 2), (512, 1, 1), (128, 2, 0)):
+        b.set_tuning(blk, grab, wg, 64)
+        b.compute(force_rebuild=True)
+        torch.cuda.synchronize()
+        assert torch.equal(b.force, f0) and torch.equal(b.torque, t0), (blk, grab, wg)
+
+
+def _state(b):
+    return [b.pos.clone(), b.orientation.clone(), b.vel.clone(), b.angmom.clone()]
+
+
+def _restore(b, s):
+    for dst, src in zip((b.pos, b.orientation, b.vel, b.angmom), s):
+        dst.copy_(src)
+    b.bind()
+    b.set_timestep(0)
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_graph_replays_match_eager_steps(precision):
+    cfg = small_config(sizes=(600, 1800, 3000), seed=14)
+    b = from_config(cfg, precision=precision)
+    rng = np.random.default_rng(5)
+    b.vel[:, :3] = torch.tensor(rng.normal(scale=0.5, size=(b.N, 3)), device="cuda")
+    b.set_timestep(0)
+    s0 = _state(b)
+    # eager: 3 rounds of 12 steps (one 10-step chunk + 2 single steps each in the graph form)
+    eager = []
+    for _ in range(3):
+        b.run(12, use_graph=False)
+        torch.cuda.synchronize()
+        eager.append(_state(b))
+    builds_eager = b.stats()["n_builds"]
+    _restore(b, s0)
+    for k in range(3):
+        b.run(12, use_graph=True)
+        # the program's own eager work between replays
+        _ = (b.force[:, :3] ** 2).sum().item()
+        torch.cuda.synchronize()
+        got = _state(b)
+        for name, x, y in zip(("pos", "orientation", "vel", "angmom"), got, eager[k]):
+            assert torch.equal(x, y), f"replay {k}: {name} differs from the eager steps"
+    st = b.stats()
+    say(f"[batch] graph == eager over 36 steps ({precision}); stats {st}; eager builds {builds_eager}")
+    assert st["graph_launches"] == 9
+    assert st["timestep"] == 36
+
+
+def test_langevin_temperature_per_replica():
+    sizes = (1500, 2000, 2500, 3000)
+    cfg = small_config(sizes=sizes, seed=15)
+    kT = [0.25, 0.5, 0.75, 1.0]
+    for rep, k in zip(cfg["replicas"], kT):
+        rep["kT"] = k
+    b = from_config(cfg, precision="mixed")
+    b.set_langevin(kT, dt=0.005, gamma=1.0, gamma_r=1.0, inertia=1.0, seed=3)
+    b.set_timestep(0)
+    b.run(1000)
+    acc = np.zeros((len(sizes), 2))
+    ns = 0
+    for _ in range(40):
+        b.run(25)
+        th = b.thermo()
+        acc += th[:, :2]
+        ns += 1
+    acc /= ns
+    n = np.array(sizes, dtype=np.float64)
+    T_trans = 2.0 * acc[:, 0] / (3.0 * n)
+    T_rot = 2.0 * acc[:, 1] / (3.0 * n)
+    say(f"[batch] kT {kT}: translational {np.round(T_trans, 3)}, rotational {np.round(T_rot, 3)}; "
+        f"stats {b.stats()}")
+    assert np.all(np.abs(T_trans / np.array(kT) - 1.0) < 0.08)
+    assert np.all(np.abs(T_rot / np.array(kT) - 1.0) < 0.08)
+
+
+def test_full_batch_workload():
+    """512 replicas, log-uniform sizes in [500, 20000], density 0.6."""
+    cfg = configs.tpm_batch()
+    sizes = [r["pos4"].shape[0] for r in cfg["replicas"]]
+    say(f"[batch] full workload: {len(sizes)} replicas, {sum(sizes)} particles")
+    b = from_config(cfg, precision="fp64")
+    b.compute(force_rebuild=True)
+    f64, t64 = host(b.force).copy(), host(b.torque).copy()
+    assert np.all(np.isfinite(f64)) and np.all(np.isfinite(t64))
+    b.set_precision("mixed")
+    b.compute()
+    err = rel_err(host(b.force)[:, :3], f64[:, :3])
+    terr = rel_err(host(b.torque)[:, :3], t64[:, :3])
+    say(f"  mixed vs fp64: force {err:.2e} torque {terr:.2e}")
+    assert err < 1e-5 and terr < 1e-5
+    # the largest replica against the single-system launcher (mixed form)
+    r = int(np.argmax(sizes))
+    rep = cfg["replicas"][r]
+    single = gpu_pair("AnisoPotentialPairTwoPatchMorse", GpuSystem(rep["pos4"], rep["L"], quat=rep["quat"]), 1,
+                      TPM, RC, precision="mixed")
+    e1 = rel_err(host(b.force)[b.replica(r)], single.force)
+    say(f"  largest replica ({sizes[r]}) vs single-system launcher: {e1:.2e}")
+    assert e1 < 1e-12
+    tune = b.autotune()
+    say(f"  autotune {tune}")
+    b.set_timestep(0)
+    b.run(20)
+    st = b.stats()
+    say(f"  20 graph steps: {st}")
+    assert st["timestep"] == 20
+    assert np.all(np.isfinite(host(b.pos)))
