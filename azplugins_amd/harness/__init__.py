@@ -109,6 +109,7 @@ class Simulation:
 from . import md  # noqa: E402
 from .. import pair as _pair  # noqa: E402
 
-# ``hoomd.azplugins.pair`` spelling
+# ``hoomd.azplugins.pair`` and ``hoomd.device.GPU`` spellings
 azplugins = _types.SimpleNamespace(pair=_pair)
+device = _types.SimpleNamespace(GPU=GPU, Device=Device)
 _ = np
