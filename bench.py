@@ -47,8 +47,10 @@ def parse():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tps", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the DPD, adjoint, batched-replica and Hertz workloads")
     ap.add_argument("--tps-steps", type=int, default=200)
-    ap.add_argument("--cpu-cells", type=int, default=40, help="FCC cells per side of the CPU sample")
+    ap.add_argument("--cpu-cells", type=int, default=64, help="FCC cells per side of the CPU sample")
     ap.add_argument("--sweep-blocks", action="store_true", help="time the pair kernels per block size")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy (HBM counter calibration) and exit")
@@ -250,6 +252,16 @@ def main():
     }
     if not args.no_tps:
         out["tps"] = run_tps(c, args.tps_steps)
+    if not args.no_extras:
+        extras = {}
+        for name, fn in (("dpd_2m", bench_dpd), ("yukawa_262k_adjoint", bench_adjoint),
+                         ("tpm_batch_512", bench_batch), ("hertz_4096", bench_hertz)):
+            try:
+                extras[name] = fn(args)
+            except Exception as exc:  # a failing extra must not hide the headline line
+                log(f"bench: {name} failed: {exc!r}")
+                extras[name] = {"error": repr(exc)}
+        out["workloads"] = extras
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_cells)
     print(json.dumps(out))
@@ -258,20 +270,196 @@ def main():
 def count_pairs_in_cut(torch, pos, nn_host, nl, N, L, rcutsq):
     """Unordered pairs with r < r_cut, counted on the GPU from the full list."""
     _, lst = nl.download(N)
+    return count_list_pairs(torch, pos, nn_host, lst, N, L, rcutsq)
+
+
+def count_list_pairs(torch, pos, nn_host, lst, N, L, rcutsq, typeid=None, ntypes=1):
+    """Unordered pairs of a full list with r^2 < rcutsq[ti, tj] (L: (3,) or per-particle (N, 3))."""
     dev = pos.device
     total = 0
     chunk = 16
     nn_t = torch.tensor(nn_host.astype(np.int64), device=dev)
-    Lt = torch.tensor(L, device=dev)
+    Lt = torch.tensor(np.asarray(L, dtype=np.float64), device=dev)
+    rc = torch.tensor(np.atleast_1d(np.asarray(rcutsq, dtype=np.float64)), device=dev)
     for k0 in range(0, lst.shape[0], chunk):
         rows = torch.tensor(lst[k0:k0 + chunk, :N].astype(np.int64), device=dev)
         valid = (torch.arange(k0, k0 + rows.shape[0], device=dev)[:, None] < nn_t[None, :])
-        pj = pos[rows.clamp(max=N - 1), :3]
+        jj = rows.clamp(max=N - 1)
+        pj = pos[jj, :3]
         d = pos[None, :, :3] - pj
         d = d - Lt * torch.round(d / Lt)
         rsq = (d * d).sum(-1)
-        total += int(((rsq < rcutsq) & valid).sum().item())
+        if typeid is None:
+            lim = rc[0]
+        else:
+            lim = rc[(typeid[None, :] * ntypes + typeid[jj]).long()]
+        total += int(((rsq < lim) & valid).sum().item())
     return total // 2
+
+
+def _events(torch, fn, reps, warmup=2):
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    s = torch.cuda.Event(enable_timing=True)
+    e = torch.cuda.Event(enable_timing=True)
+    s.record()
+    for _ in range(reps):
+        fn()
+    e.record()
+    torch.cuda.synchronize()
+    return s.elapsed_time(e) / 1e3 / reps
+
+
+def bench_dpd(args):
+    """DPD general weight, N = 2,000,000 at density 3, A/B 50/50 (FP64, Philox random forces)."""
+    from azplugins_amd import _azp, _azplugins, configs
+
+    torch = _azp.torch
+    dev = torch.device("cuda:0")
+    c = configs.dpd_2m()
+    pos4, L = c["pos4"], c["L"]
+    N = pos4.shape[0]
+    rng = np.random.default_rng(7)
+    vel = np.zeros((N, 4))
+    vel[:, :3] = rng.normal(scale=np.sqrt(c["kT"]), size=(N, 3))
+    vel[:, 3] = 1.0
+    pos = torch.tensor(pos4, device=dev)
+    velt = torch.tensor(vel, device=dev)
+    tag = torch.arange(N, dtype=torch.int32, device=dev)
+    parts = _azp.Particles(N, _azp.Box(*L), pos.data_ptr(), None, velt.data_ptr(), tag.data_ptr(), None)
+    pp = _azplugins.PotentialPairDPDThermoGeneralWeightGPU(2)
+    for (a, b), v in c["params"].items():
+        pp.setParams(a, b, [v["A"], v["gamma"], v["s"]])
+        pp.setRCut(a, b, c["r_cut"])
+    pp.setThermostat(c["seed"], c["dt"], c["kT"])
+    nl = _azplugins.NeighborListGPU(c["r_buff"])
+    nl.set_r_cut(2, pp.getRCutMatrix())
+    nl.update(parts, force_rebuild=True)
+    vpitch = (N + 63) // 64 * 64
+    force = torch.zeros((N, 4), dtype=torch.float64, device=dev)
+    virial = torch.zeros((6, vpitch), dtype=torch.float64, device=dev)
+    out = _azp.Outputs(force.data_ptr(), virial.data_ptr(), vpitch, None)
+    step = [100]
+
+    def fn():
+        pp.compute(nl.handle, parts, out, step[0])
+        step[0] += 1
+
+    t = _events(torch, fn, max(5, args.steps // 5))
+    nn, lst = nl.download(N)
+    pairs = count_list_pairs(torch, pos, nn, lst, N, L, [1.0])
+    res = {"value": pairs / t, "unit": "pair interactions/s", "ms_per_eval": t * 1e3, "N": N,
+           "pairs_in_cut": pairs, "nlist_entries": int(nn.sum()), "precision": "fp64",
+           "note": "one DPD force evaluation (conservative + dissipative + Philox random) per step"}
+    log(f"bench: dpd_2m {t * 1e3:.3f} ms/eval, {pairs / t:.3e} pairs/s")
+    return res
+
+
+def bench_adjoint(args):
+    """ExpandedYukawa force matching at N = 262,144: forward + adjoint backward."""
+    from azplugins_amd import _azp, configs
+    from azplugins_amd.adjoint import PairForceField
+
+    torch = _azp.torch
+    dev = torch.device("cuda:0")
+    c = configs.yukawa_262k()
+    pos4, L = c["pos4"], c["L"]
+    keys = [(0, 0), (0, 1), (1, 1)]
+    theta = np.array([[c["params"][k]["epsilon"], c["params"][k]["kappa"], c["params"][k]["delta"]] for k in keys])
+    typeid = torch.tensor(pos4[:, 3], device=dev)
+    pos = torch.tensor(pos4[:, :3], device=dev)
+    res = {}
+    for precision in ("mixed", "fp64"):
+        ff = PairForceField("ExpandedYukawa", 2, L, c["r_cut"], precision=precision)
+        with torch.no_grad():
+            tgt, _ = ff(pos, typeid, torch.tensor(theta * 1.05, device=dev))
+        th = torch.tensor(theta, device=dev, requires_grad=True)
+        pr = pos.clone().requires_grad_(True)
+
+        def fwd_bwd():
+            th.grad = None
+            pr.grad = None
+            F, E = ff(pr, typeid, th)
+            loss = ((F - tgt) ** 2).mean() + 1e-3 * E.sum()
+            loss.backward()
+
+        t = _events(torch, fwd_bwd, 10)
+        with torch.no_grad():
+            t_f = _events(torch, lambda: ff(pos, typeid, th), 10)
+        res[precision] = {"value": 1.0 / t, "unit": "loss+gradient evaluations/s", "ms_forward_backward": t * 1e3,
+                          "ms_forward": t_f * 1e3}
+        log(f"bench: yukawa_262k adjoint {precision}: forward {t_f * 1e3:.3f} ms, forward+backward {t * 1e3:.3f} ms")
+    res["N"] = pos4.shape[0]
+    return res
+
+
+def bench_batch(args):
+    """512 TwoPatchMorse replicas (sizes log-uniform in [500, 20000]): graph-captured Langevin steps."""
+    from azplugins_amd import _azp, configs
+    from azplugins_amd.batch import from_config
+
+    torch = _azp.torch
+    cfg = configs.tpm_batch()
+    b = from_config(cfg, precision="mixed")
+    b.set_timestep(0)
+    b.compute(force_rebuild=True)
+    tune = b.autotune()
+    t_force = _events(torch, lambda: b.compute(), 10)
+    steps = 100
+    b.run(20)
+    torch.cuda.synchronize()
+    st0 = b.stats()
+    t0 = time.perf_counter()
+    b.run(steps)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    st1 = b.stats()
+    nn, lst = b.nlist()
+    Lper = np.repeat(np.stack([r["L"] for r in cfg["replicas"]]), b.sizes.astype(np.int64), axis=0)
+    pairs = count_list_pairs(torch, b.pos, nn, lst, b.N, Lper, [cfg["r_cut"] ** 2])
+    res = {"value": steps / dt, "unit": "timesteps/s (all replicas)", "replica_steps_per_s": b.R * steps / dt,
+           "particle_steps_per_s": b.N * steps / dt, "pairs_in_cut": pairs,
+           "pair_interactions_per_s_in_steps": pairs * steps / dt, "force_ms": t_force * 1e3,
+           "replicas": b.R, "N_total": b.N, "tuning": tune, "nlist_builds": st1["n_builds"] - st0["n_builds"],
+           "graph_launches": st1["graph_launches"] - st0["graph_launches"], "precision": "mixed",
+           "method": "Langevin (translation + rotation, I = 1), dt 0.005, per-replica kT"}
+    log(f"bench: tpm_batch_512 {steps / dt:.1f} steps/s ({b.N} particles), force {t_force * 1e3:.3f} ms, "
+        f"tuning {tune}")
+    return res
+
+
+def bench_hertz(args):
+    """Hertz N = 4096: 1000 NVE steps through the drop-in API (GPU)."""
+    import azplugins_amd.harness as hoomd
+    from azplugins_amd import _azp, configs
+
+    torch = _azp.torch
+    c = configs.hertz_4096()
+    snap = hoomd.Snapshot()
+    N = c["pos4"].shape[0]
+    snap.configuration.box = [*c["L"], 0, 0, 0]
+    snap.particles.N = N
+    snap.particles.position[:] = c["pos4"][:, :3]
+    sim = hoomd.Simulation(seed=1)
+    sim.create_state_from_snapshot(snap)
+    sim.state.thermalize_particle_momenta(hoomd.filter.All(), kT=c["kT"])
+    integ = hoomd.md.Integrator(dt=c["dt"])
+    nl = hoomd.md.nlist.Cell(buffer=c["r_buff"])
+    hz = hoomd.azplugins.pair.Hertz(nlist=nl, default_r_cut=c["r_cut"])
+    hz.params[("A", "A")] = dict(epsilon=1.0)
+    integ.forces = [hz]
+    integ.methods = [hoomd.md.methods.ConstantVolume(hoomd.filter.All())]
+    sim.operations.integrator = integ
+    sim.run(10)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sim.run(1000)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    log(f"bench: hertz_4096 GPU NVE {1000 / dt:.1f} steps/s")
+    return {"value": 1000 / dt, "unit": "timesteps/s", "steps": 1000, "N": N, "precision": "fp64",
+            "method": "NVE velocity Verlet dt 0.005"}
 
 
 def run_tps(c, steps):
@@ -329,7 +517,7 @@ def cpu_baseline(cells):
         times.append(time.perf_counter() - t0)
     t = float(np.median(times))
     log(f"bench: CPU oracle {n_cut / t:.3e} pair interactions/s on N={N} ({t:.2f} s per evaluation)")
-    return {
+    res = {
         "value": n_cut / t,
         "unit": "pair interactions/s",
         "cores": 1,
@@ -338,6 +526,20 @@ def cpu_baseline(cells):
                   f"evaluations (median {t:.2f} s), FP64 half list with third law; host "
                   f"{os.cpu_count()} logical CPUs",
     }
+    # Hertz N = 4096, 1000 NVE steps (the CPU-only workload), same oracle, one thread
+    h = configs.hertz_4096()
+    rng = np.random.default_rng(1)
+    vel = np.zeros((h["pos4"].shape[0], 4))
+    vel[:, :3] = rng.normal(scale=np.sqrt(h["kT"]), size=(vel.shape[0], 3))
+    vel[:, :3] -= vel[:, :3].mean(axis=0)
+    vel[:, 3] = 1.0
+    t0 = time.perf_counter()
+    oracle.run_nve(oracle.HERTZ, h["pos4"], vel, h["L"], 1, oracle.pack_matrix(oracle.HERTZ, h["params"], 1),
+                   [h["r_cut"]], h["r_buff"], "none", h["dt"], 1000)
+    th = time.perf_counter() - t0
+    res["hertz_4096_nve"] = {"value": 1000 / th, "unit": "timesteps/s", "steps": 1000, "cores": 1}
+    log(f"bench: CPU oracle Hertz 4096 NVE {1000 / th:.1f} steps/s")
+    return res
 
 
 def load_traffic():
