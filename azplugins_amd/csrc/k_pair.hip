@@ -14,8 +14,9 @@
 // Two precision forms:
 //   plain (FP64):  FP64 positions, min image, evaluator and accumulation.
 //   mixed:         positions as wrapped 32-bit fixed point (16-byte gather),
-//                  dx exact in integers then FP32, FP32 evaluator, FP64
-//                  accumulation.  Pairs whose FP32 rsq lies within a relative
+//                  dx exact in integers then FP32, FP32 evaluator; each batch
+//                  of 8 list entries is summed in FP32 and added to FP64
+//                  accumulators.  Pairs whose FP32 rsq lies within a relative
 //                  band of 1e-5 around rcutsq are re-tested on the FP64
 //                  positions, so the in/out cutoff decision is the FP64 one.
 #include <type_traits>
@@ -120,6 +121,51 @@ __device__ __forceinline__ void accumulate(Acc& acc, double fd, double dx, doubl
         }
     }
 
+// Mixed form: the terms of one batch of list entries are summed in FP32 (at
+// most 8 terms), then each batch sum is added to the FP64 accumulators.
+struct Acc32
+    {
+    float fx = 0, fy = 0, fz = 0, e = 0;
+    float v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0, v5 = 0;
+    };
+
+template<bool VIRIAL>
+__device__ __forceinline__ void accumulate32(Acc32& acc, float fd, float dx, float dy, float dz, float e)
+    {
+    const float px = fd * dx, py = fd * dy, pz = fd * dz;
+    acc.fx += px;
+    acc.fy += py;
+    acc.fz += pz;
+    acc.e += e;
+    if (VIRIAL)
+        {
+        acc.v0 = fmaf(px, dx, acc.v0);
+        acc.v1 = fmaf(px, dy, acc.v1);
+        acc.v2 = fmaf(px, dz, acc.v2);
+        acc.v3 = fmaf(py, dy, acc.v3);
+        acc.v4 = fmaf(py, dz, acc.v4);
+        acc.v5 = fmaf(pz, dz, acc.v5);
+        }
+    }
+
+template<bool VIRIAL>
+__device__ __forceinline__ void flush32(Acc& acc, const Acc32& b)
+    {
+    acc.fx += (double)b.fx;
+    acc.fy += (double)b.fy;
+    acc.fz += (double)b.fz;
+    acc.e += (double)b.e;
+    if (VIRIAL)
+        {
+        acc.v0 += (double)b.v0;
+        acc.v1 += (double)b.v1;
+        acc.v2 += (double)b.v2;
+        acc.v3 += (double)b.v3;
+        acc.v4 += (double)b.v4;
+        acc.v5 += (double)b.v5;
+        }
+    }
+
 template<bool VIRIAL>
 __device__ __forceinline__ void store(const PairKernelArgs& a, uint32_t i, const Acc& acc)
     {
@@ -166,7 +212,7 @@ __device__ __forceinline__ uint32_t wave_max(uint32_t v)
 
 // Accumulators of the mixed form are kept in integer-grid units of the
 // fixed-point positions (dx = n * L / 2^32): sum f n and sum f n_a n_b are
-// formed from exact integer-to-FP64 conversions and scaled once at the end.
+// formed from the integer displacements (as FP32) and scaled once at the end.
 template<bool VIRIAL>
 __device__ __forceinline__ void store_scaled(const PairKernelArgs& a, uint32_t i, const Acc& acc)
     {
@@ -216,7 +262,7 @@ __device__ __forceinline__ void term_mixed(const PairKernelArgs& a,
                                            int32_t ix,
                                            int32_t iy,
                                            int32_t iz,
-                                           Acc& acc)
+                                           Acc32& acc)
     {
     const float fx = (float)ix, fy = (float)iy, fz = (float)iz;
     float rsq;
@@ -240,7 +286,7 @@ __device__ __forceinline__ void term_mixed(const PairKernelArgs& a,
     if constexpr (HasMasked<Eval>::value && !XPLOR)
         {
         Eval::compute_masked(q, rs, in, fdivr, e);
-        accumulate<VIRIAL>(acc, (double)fdivr, (double)ix, (double)iy, (double)iz, (double)e);
+        accumulate32<VIRIAL>(acc, fdivr, fx, fy, fz, e);
         return;
         }
     if constexpr (NeedsRsq64<Eval>::value)
@@ -262,7 +308,7 @@ __device__ __forceinline__ void term_mixed(const PairKernelArgs& a,
         apply_xplor<float>(rs, x.rcsq, x.ronsq, fdivr, e);
     fdivr = in ? fdivr : 0.0f;
     e = in ? e : 0.0f;
-    accumulate<VIRIAL>(acc, (double)fdivr, (double)ix, (double)iy, (double)iz, (double)e);
+    accumulate32<VIRIAL>(acc, fdivr, fx, fy, fz, e);
     }
 
 template<class Eval, bool MIXED, bool VIRIAL, bool SINGLE, bool XPLOR, bool CUBIC>
@@ -351,6 +397,7 @@ __global__ void __launch_bounds__(512) k_pair(const PairKernelArgs a)
 #pragma unroll
             for (uint32_t u = 0; u < U; ++u)
                 pj[u] = *reinterpret_cast<const int4*>(pf_base + (jj[u] << 4));
+            Acc32 part;
 #pragma unroll
             for (uint32_t u = 0; u < U; ++u)
                 {
@@ -358,8 +405,9 @@ __global__ void __launch_bounds__(512) k_pair(const PairKernelArgs a)
                 term_mixed<Eval, VIRIAL, XPLOR, CUBIC>(a, SINGLE ? q0 : s_prep[tp], SINGLE ? x0 : s_x[tp], ie,
                                                        jj[u], fixed_delta(pi.x, pj[u].x),
                                                        fixed_delta(pi.y, pj[u].y), fixed_delta(pi.z, pj[u].z),
-                                                       acc);
+                                                       part);
                 }
+            flush32<VIRIAL>(acc, part);
             }
         if (i < a.N)
             store_scaled<VIRIAL>(a, i, acc);
