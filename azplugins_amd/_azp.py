@@ -129,6 +129,7 @@ class BatchStats(ctypes.Structure):
     _fields_ = [
         ("timestep", ctypes.c_uint64),
         ("n_builds", ctypes.c_uint64),
+        ("replica_builds", ctypes.c_uint64),
         ("graph_launches", ctypes.c_uint64),
         ("max_neighbors", ctypes.c_uint32),
         ("n_cells", ctypes.c_uint32),

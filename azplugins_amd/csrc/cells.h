@@ -81,4 +81,114 @@ __device__ __forceinline__ int stencil_axis(int c, int n, int out[3])
     return m;
     }
 
+// ---------------------------------------------------------------------------
+// Exclusive scan of n uint32 counts in three launches: per-chunk sums, a scan
+// of the chunk sums by one workgroup, then each chunk's local scan plus its
+// offset.  Chunks are kScanChunk = 4 x 1024 counts; up to 1024 chunks per
+// pass of the middle kernel, which loops over larger inputs.  out[n] = total.
+// Kernels return at once when `gate` is non-null and its bit 0 is clear.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kScanThreads = 1024;
+constexpr uint32_t kScanPerThread = 4;
+constexpr uint32_t kScanChunk = kScanThreads * kScanPerThread;
+
+inline uint32_t scan_chunks(uint32_t n)
+    {
+    return (n + kScanChunk - 1) / kScanChunk;
+    }
+
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s, uint32_t& total)
+    {
+    // Hillis-Steele over kScanThreads values in LDS
+    const uint32_t t = threadIdx.x;
+    s[t] = v;
+    __syncthreads();
+    for (uint32_t off = 1; off < kScanThreads; off <<= 1)
+        {
+        const uint32_t w = (t >= off) ? s[t - off] : 0u;
+        __syncthreads();
+        s[t] += w;
+        __syncthreads();
+        }
+    total = s[kScanThreads - 1];
+    const uint32_t incl = s[t];
+    __syncthreads();
+    return incl - v;
+    }
+
+static __global__ void __launch_bounds__(kScanThreads)
+    k_scan_sums(const uint32_t* __restrict__ in, uint32_t n, uint32_t* __restrict__ sums, const uint32_t* gate)
+    {
+    if (gate && (gate[0] & 1u) == 0u)
+        return;
+    __shared__ uint32_t s[kScanThreads];
+    const uint32_t base = blockIdx.x * kScanChunk + threadIdx.x * kScanPerThread;
+    uint32_t v = 0;
+    for (uint32_t k = 0; k < kScanPerThread; ++k)
+        if (base + k < n)
+            v += in[base + k];
+    uint32_t total;
+    (void)block_exclusive_scan(v, s, total);
+    if (threadIdx.x == 0)
+        sums[blockIdx.x] = total;
+    }
+
+static __global__ void __launch_bounds__(kScanThreads)
+    k_scan_offsets(uint32_t* __restrict__ sums, uint32_t nchunks, const uint32_t* gate)
+    {
+    if (gate && (gate[0] & 1u) == 0u)
+        return;
+    __shared__ uint32_t s[kScanThreads];
+    uint32_t carry = 0;
+    for (uint32_t b0 = 0; b0 < nchunks; b0 += kScanThreads)
+        {
+        const uint32_t k = b0 + threadIdx.x;
+        const uint32_t v = k < nchunks ? sums[k] : 0u;
+        uint32_t total;
+        const uint32_t ex = block_exclusive_scan(v, s, total);
+        if (k < nchunks)
+            sums[k] = carry + ex;
+        carry += total;
+        }
+    }
+
+static __global__ void __launch_bounds__(kScanThreads) k_scan_apply(const uint32_t* __restrict__ in,
+                                                                   uint32_t n,
+                                                                   const uint32_t* __restrict__ offsets,
+                                                                   uint32_t* __restrict__ out,
+                                                                   const uint32_t* gate)
+    {
+    if (gate && (gate[0] & 1u) == 0u)
+        return;
+    __shared__ uint32_t s[kScanThreads];
+    const uint32_t base = blockIdx.x * kScanChunk + threadIdx.x * kScanPerThread;
+    uint32_t v[kScanPerThread];
+    uint32_t sum = 0;
+    for (uint32_t k = 0; k < kScanPerThread; ++k)
+        {
+        v[k] = base + k < n ? in[base + k] : 0u;
+        sum += v[k];
+        }
+    uint32_t total;
+    uint32_t run = offsets[blockIdx.x] + block_exclusive_scan(sum, s, total);
+    for (uint32_t k = 0; k < kScanPerThread; ++k)
+        {
+        if (base + k < n)
+            out[base + k] = run;
+        run += v[k];
+        }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == kScanThreads - 1)
+        out[n] = offsets[blockIdx.x] + total;
+    }
+
+// Enqueues the exclusive scan of in[0..n) into out[0..n] (n >= 1); `sums`
+// holds scan_chunks(n) values.
+inline void enqueue_scan(const uint32_t* in, uint32_t n, uint32_t* sums, uint32_t* out, const uint32_t* gate, hipStream_t s)
+    {
+    const uint32_t nc = scan_chunks(n);
+    hipLaunchKernelGGL(k_scan_sums, dim3(nc), dim3(kScanThreads), 0, s, in, n, sums, gate);
+    hipLaunchKernelGGL(k_scan_offsets, dim3(1), dim3(kScanThreads), 0, s, sums, nc, gate);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nc), dim3(kScanThreads), 0, s, in, n, sums, out, gate);
+    }
+
 } // namespace azp

@@ -17,9 +17,12 @@
 //   live batch.
 // * Time steps (Langevin, translation and rotation of spherical tops) are
 //   captured once into a HIP graph and replayed.  Everything that depends on
-//   data is decided on the device: the rebuild check writes a flag, and the
-//   rebuild kernels return at once when it is clear.  Counters and flags are
-//   cleared by fill kernels (no memset nodes in the graph).
+//   data is decided on the device: the rebuild check flags each replica that
+//   needs a new list (and a global "any" flag); the rebuild kernels return at
+//   once when no replica is flagged and otherwise touch only the flagged
+//   replicas' particles and cells (the others keep their lists, counts and
+//   cell ranges).  Counters and flags are cleared by fill kernels (no memset
+//   nodes in the graph).
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -63,10 +66,25 @@ __global__ void k_b_fill64(unsigned long long* p, unsigned long long v)
         p[0] = v;
     }
 
+// Clears the cell counts of the flagged replicas.
+__global__ void k_b_clear_counts(uint32_t* __restrict__ count,
+                                 uint32_t ncell,
+                                 const uint32_t* __restrict__ cell_rep,
+                                 const uint32_t* __restrict__ rep_flag,
+                                 const uint32_t* gate)
+    {
+    if (gated_off(gate))
+        return;
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < ncell && rep_flag[cell_rep[c]])
+        count[c] = 0u;
+    }
+
 __global__ void k_b_bin(const double4* __restrict__ pos,
                         uint32_t N,
                         const uint32_t* __restrict__ rep_of,
                         const RepInfo* __restrict__ reps,
+                        const uint32_t* __restrict__ rep_flag,
                         uint32_t* __restrict__ cell_of,
                         uint32_t* __restrict__ slot_of,
                         uint32_t* __restrict__ count,
@@ -77,8 +95,11 @@ __global__ void k_b_bin(const double4* __restrict__ pos,
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N)
         return;
+    const uint32_t r = rep_of[i];
+    if (!rep_flag[r])
+        return;
     const double4 p = pos[i];
-    const RepInfo& R = reps[rep_of[i]];
+    const RepInfo& R = reps[r];
     const int cx = cell_coord(p.x, R.L[0], R.Linv[0], R.n[0]);
     const int cy = cell_coord(p.y, R.L[1], R.Linv[1], R.n[1]);
     const int cz = cell_coord(p.z, R.L[2], R.Linv[2], R.n[2]);
@@ -87,40 +108,9 @@ __global__ void k_b_bin(const double4* __restrict__ pos,
     slot_of[i] = atomicAdd(&count[c], 1u);
     }
 
-// Exclusive scan of n counts by one workgroup of 1024 lanes.
-__global__ void __launch_bounds__(1024)
-    k_b_scan(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, uint32_t n, const uint32_t* gate)
-    {
-    if (gated_off(gate))
-        return;
-    __shared__ uint32_t s[1024];
-    const uint32_t t = threadIdx.x;
-    const uint32_t per = (n + 1023u) / 1024u;
-    const uint32_t b = min(t * per, n);
-    const uint32_t e = min(b + per, n);
-    uint32_t local = 0;
-    for (uint32_t k = b; k < e; ++k)
-        local += in[k];
-    s[t] = local;
-    __syncthreads();
-    for (uint32_t off = 1; off < 1024; off <<= 1)
-        {
-        const uint32_t v = (t >= off) ? s[t - off] : 0u;
-        __syncthreads();
-        s[t] += v;
-        __syncthreads();
-        }
-    uint32_t run = s[t] - local;
-    for (uint32_t k = b; k < e; ++k)
-        {
-        out[k] = run;
-        run += in[k];
-        }
-    if (t == 1023)
-        out[n] = s[1023];
-    }
-
 __global__ void k_b_scatter(uint32_t N,
+                            const uint32_t* __restrict__ rep_of,
+                            const uint32_t* __restrict__ rep_flag,
                             const uint32_t* __restrict__ cell_of,
                             const uint32_t* __restrict__ slot_of,
                             const uint32_t* __restrict__ start,
@@ -130,13 +120,15 @@ __global__ void k_b_scatter(uint32_t N,
     if (gated_off(gate))
         return;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N)
+    if (i >= N || !rep_flag[rep_of[i]])
         return;
     members[start[cell_of[i]] + slot_of[i]] = i;
     }
 
 // Insertion sort of each cell's members by particle index (deterministic order).
 __global__ void k_b_sort_cells(uint32_t ncell,
+                               const uint32_t* __restrict__ cell_rep,
+                               const uint32_t* __restrict__ rep_flag,
                                const uint32_t* __restrict__ start,
                                uint32_t* __restrict__ members,
                                const uint32_t* gate)
@@ -144,7 +136,7 @@ __global__ void k_b_sort_cells(uint32_t ncell,
     if (gated_off(gate))
         return;
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncell)
+    if (c >= ncell || !rep_flag[cell_rep[c]])
         return;
     const uint32_t b = start[c], e = start[c + 1];
     for (uint32_t k = b + 1; k < e; ++k)
@@ -177,6 +169,7 @@ struct BatchWalkArgs
     uint32_t max_neighbors;
     uint32_t* flags; // [0] largest count beyond capacity, [2] bad type (1) / non-finite (2)
     const uint32_t* gate;
+    const uint32_t* rep_flag;
     };
 
 // Particle i's neighbors inside its own replica (same cell order, predicate
@@ -241,7 +234,7 @@ __global__ void __launch_bounds__(256) k_b_walk(const BatchWalkArgs a)
     if (gated_off(a.gate))
         return;
     const uint32_t i = xcd_logical_block() * blockDim.x + threadIdx.x;
-    const bool active = i < a.N;
+    const bool active = i < a.N && a.rep_flag[a.rep_of[i]] != 0u;
     const uint32_t count = active ? b_walk_one(a, i) : 0u;
     if (active)
         {
@@ -260,49 +253,66 @@ __global__ void __launch_bounds__(256) k_b_walk(const BatchWalkArgs a)
             a.nlist[(size_t)k * a.pitch + i] = i;
     }
 
-// Snapshot of the positions of the last build (and the build counter).
+// Snapshot of the positions of the last build of each rebuilt replica, and
+// the counters of rebuild events and of replica rebuilds.
 __global__ void k_b_copy_last(const double4* __restrict__ pos,
                               double4* __restrict__ last,
                               uint32_t N,
-                              unsigned long long* __restrict__ n_builds,
+                              const uint32_t* __restrict__ rep_of,
+                              const RepInfo* __restrict__ reps,
+                              const uint32_t* __restrict__ rep_flag,
+                              unsigned long long* __restrict__ builds,
                               const uint32_t* gate)
     {
     if (gated_off(gate))
         return;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0)
-        atomicAdd(n_builds, 1ull);
-    if (i < N)
-        last[i] = pos[i];
+        atomicAdd(builds, 1ull);
+    if (i >= N)
+        return;
+    const uint32_t r = rep_of[i];
+    if (!rep_flag[r])
+        return;
+    last[i] = pos[i];
+    if (i == reps[r].first)
+        atomicAdd(builds + 1, 1ull);
     }
 
-// Rebuild flag: some particle moved farther than r_buff / 2 since the last
-// build (bit 0), or a position is not finite (bit 0 here, bit 1 in flags[2]).
+// Rebuild flags: replica r is flagged (and the global flags[1]) when one of
+// its particles moved farther than r_buff / 2 since its last build, or has a
+// position that is not finite (also reported in flags[2], bit 1).
 __global__ void k_b_check(const double4* __restrict__ pos,
                           const double4* __restrict__ last,
                           uint32_t N,
                           const uint32_t* __restrict__ rep_of,
                           const RepInfo* __restrict__ reps,
                           double thresh_sq,
+                          uint32_t* __restrict__ rep_flag,
                           uint32_t* __restrict__ flags)
     {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N)
         return;
+    const uint32_t r = rep_of[i];
     const double4 p = pos[i];
     if (!(isfinite(p.x) && isfinite(p.y) && isfinite(p.z)))
         {
-        atomicOr(flags + 1, 1u);
+        rep_flag[r] = 1u;
+        flags[1] = 1u;
         atomicOr(flags + 2, 2u);
         return;
         }
     const double4 q = last[i];
-    const RepInfo& R = reps[rep_of[i]];
+    const RepInfo& R = reps[r];
     const double dx = min_image_1d(p.x - q.x, R.L[0], R.Linv[0]);
     const double dy = min_image_1d(p.y - q.y, R.L[1], R.Linv[1]);
     const double dz = min_image_1d(p.z - q.z, R.L[2], R.Linv[2]);
     if (rsq3_nofma(dx, dy, dz) > thresh_sq)
-        atomicOr(flags + 1, 1u);
+        {
+        rep_flag[r] = 1u;
+        flags[1] = 1u;
+        }
     }
 
 __global__ void k_b_pack_fixed(const double4* __restrict__ pos,
@@ -650,10 +660,10 @@ struct azp_batch
     DeviceArray<uint32_t> rep_of;
     DeviceArray<azp_param_two_patch_morse> d_params;
     DeviceArray<double> d_rcutsq, d_rlistsq;
-    DeviceArray<uint32_t> cell_of, slot_of, cell_count, cell_start, members;
-    DeviceArray<uint32_t> n_neigh, list, flags, counter;
+    DeviceArray<uint32_t> cell_of, slot_of, cell_count, cell_start, members, cell_rep, scan_sums;
+    DeviceArray<uint32_t> n_neigh, list, flags, counter, rep_flag;
     DeviceArray<double4> last_pos;
-    DeviceArray<unsigned long long> d_timestep, d_builds;
+    DeviceArray<unsigned long long> d_timestep, d_builds; // builds: [0] rebuild events, [1] replica rebuilds
     uint32_t* h_flags = nullptr; // pinned
     uint32_t ncell = 0;
     uint32_t pitch = 0;
@@ -772,8 +782,17 @@ int batch_upload_tables(azp_batch* b, hipStream_t s)
         cells += (uint32_t)nc;
         }
     b->ncell = cells;
+    std::vector<uint32_t> cell_rep(cells);
+    for (uint32_t r = 0; r < b->R; ++r)
+        {
+        const uint32_t nc = (uint32_t)reps[r].n[0] * reps[r].n[1] * reps[r].n[2];
+        std::fill(cell_rep.begin() + reps[r].cell_base, cell_rep.begin() + reps[r].cell_base + nc, r);
+        }
     AZP_TRY(b->d_reps.reserve(b->R));
+    AZP_TRY(b->cell_rep.reserve(cells));
+    AZP_TRY(b->rep_flag.reserve(b->R));
     AZP_HIP(hipMemcpyAsync(b->d_reps.ptr, reps.data(), b->R * sizeof(RepInfo), hipMemcpyHostToDevice, s));
+    AZP_HIP(hipMemcpyAsync(b->cell_rep.ptr, cell_rep.data(), cells * sizeof(uint32_t), hipMemcpyHostToDevice, s));
     if (!b->layout_uploaded)
         {
         std::vector<uint32_t> rep_of(b->N);
@@ -809,14 +828,16 @@ int batch_upload_tables(azp_batch* b, hipStream_t s)
     AZP_TRY(b->last_pos.reserve(b->N));
     AZP_TRY(b->cell_count.reserve(b->ncell));
     AZP_TRY(b->cell_start.reserve((size_t)b->ncell + 1));
+    AZP_TRY(b->scan_sums.reserve(scan_chunks(b->ncell)));
     AZP_TRY(b->flags.reserve(kNumFlags));
     AZP_TRY(b->counter.reserve(1));
     if (!b->d_timestep.ptr)
         {
         AZP_TRY(b->d_timestep.reserve(1));
-        AZP_TRY(b->d_builds.reserve(1));
+        AZP_TRY(b->d_builds.reserve(2));
         hipLaunchKernelGGL(k_b_fill64, dim3(1), dim3(64), 0, s, b->d_timestep.ptr, 0ull);
         hipLaunchKernelGGL(k_b_fill64, dim3(1), dim3(64), 0, s, b->d_builds.ptr, 0ull);
+        hipLaunchKernelGGL(k_b_fill64, dim3(1), dim3(64), 0, s, b->d_builds.ptr + 1, 0ull);
         hipLaunchKernelGGL(k_b_fill, dim3(1), dim3(64), 0, s, b->flags.ptr, 0u, kNumFlags, nullptr);
         AZP_LAUNCH_CHECK();
         }
@@ -845,15 +866,19 @@ void enqueue_rebuild(azp_batch* b, hipStream_t s)
     const unsigned B = 256;
     const uint32_t N = b->N;
     const uint32_t* gate = b->flags.ptr + kFlagRebuild;
+    const uint32_t* rf = b->rep_flag.ptr;
     const double4* pos = reinterpret_cast<const double4*>(b->arr.d_pos);
-    hipLaunchKernelGGL(k_b_fill, dim3(grid_for(b->ncell, B)), dim3(B), 0, s, b->cell_count.ptr, 0u, b->ncell, gate);
-    hipLaunchKernelGGL(k_b_bin, dim3(grid_for(N, B)), dim3(B), 0, s, pos, N, b->rep_of.ptr, b->d_reps.ptr,
+    hipLaunchKernelGGL(k_b_clear_counts, dim3(grid_for(b->ncell, B)), dim3(B), 0, s, b->cell_count.ptr, b->ncell,
+                       b->cell_rep.ptr, rf, gate);
+    hipLaunchKernelGGL(k_b_bin, dim3(grid_for(N, B)), dim3(B), 0, s, pos, N, b->rep_of.ptr, b->d_reps.ptr, rf,
                        b->cell_of.ptr, b->slot_of.ptr, b->cell_count.ptr, gate);
-    hipLaunchKernelGGL(k_b_scan, dim3(1), dim3(1024), 0, s, b->cell_count.ptr, b->cell_start.ptr, b->ncell, gate);
-    hipLaunchKernelGGL(k_b_scatter, dim3(grid_for(N, B)), dim3(B), 0, s, N, b->cell_of.ptr, b->slot_of.ptr,
+    // the scan covers every replica; the counts of replicas not rebuilt are
+    // unchanged, so are their cell ranges
+    enqueue_scan(b->cell_count.ptr, b->ncell, b->scan_sums.ptr, b->cell_start.ptr, gate, s);
+    hipLaunchKernelGGL(k_b_scatter, dim3(grid_for(N, B)), dim3(B), 0, s, N, b->rep_of.ptr, rf, b->cell_of.ptr,
+                       b->slot_of.ptr, b->cell_start.ptr, b->members.ptr, gate);
+    hipLaunchKernelGGL(k_b_sort_cells, dim3(grid_for(b->ncell, B)), dim3(B), 0, s, b->ncell, b->cell_rep.ptr, rf,
                        b->cell_start.ptr, b->members.ptr, gate);
-    hipLaunchKernelGGL(k_b_sort_cells, dim3(grid_for(b->ncell, B)), dim3(B), 0, s, b->ncell, b->cell_start.ptr,
-                       b->members.ptr, gate);
     BatchWalkArgs wa;
     wa.pos = pos;
     wa.N = N;
@@ -870,24 +895,27 @@ void enqueue_rebuild(azp_batch* b, hipStream_t s)
     wa.max_neighbors = b->max_neighbors;
     wa.flags = b->flags.ptr;
     wa.gate = gate;
+    wa.rep_flag = rf;
     const unsigned wb = b->tune.walk_block ? b->tune.walk_block : 256;
     hipLaunchKernelGGL(k_b_walk, dim3(grid_for(N, wb)), dim3(wb), 0, s, wa);
-    hipLaunchKernelGGL(k_b_copy_last, dim3(grid_for(N, B)), dim3(B), 0, s, pos, b->last_pos.ptr, N,
-                       b->d_builds.ptr, gate);
+    hipLaunchKernelGGL(k_b_copy_last, dim3(grid_for(N, B)), dim3(B), 0, s, pos, b->last_pos.ptr, N, b->rep_of.ptr,
+                       b->d_reps.ptr, rf, b->d_builds.ptr, gate);
     }
 
-// Rebuild decision on the device: flags[1] = 1 when forced, else the displacement check.
+// Rebuild decision on the device: every replica flagged when forced, else
+// the per-replica displacement check.
 void enqueue_check(azp_batch* b, bool force, hipStream_t s)
     {
     const unsigned B = 256;
-    hipLaunchKernelGGL(k_b_fill, dim3(1), dim3(64), 0, s, b->flags.ptr + kFlagRebuild, force ? 1u : 0u, 1u,
-                       nullptr);
+    const uint32_t v = force ? 1u : 0u;
+    hipLaunchKernelGGL(k_b_fill, dim3(1), dim3(64), 0, s, b->flags.ptr + kFlagRebuild, v, 1u, nullptr);
+    hipLaunchKernelGGL(k_b_fill, dim3(grid_for(b->R, B)), dim3(B), 0, s, b->rep_flag.ptr, v, b->R, nullptr);
     if (!force)
         {
         const double half = 0.5 * b->r_buff;
         hipLaunchKernelGGL(k_b_check, dim3(grid_for(b->N, B)), dim3(B), 0, s,
                            reinterpret_cast<const double4*>(b->arr.d_pos), b->last_pos.ptr, b->N, b->rep_of.ptr,
-                           b->d_reps.ptr, half * half, b->flags.ptr);
+                           b->d_reps.ptr, half * half, b->rep_flag.ptr, b->flags.ptr);
         }
     }
 
@@ -1478,14 +1506,16 @@ extern "C" int azp_batch_stats(azp_batch* b, azp_batch_stats_t* out)
     out->graph_launches = b->graph_launches;
     out->timestep = 0;
     out->n_builds = 0;
+    out->replica_builds = 0;
     if (b->d_timestep.ptr && b->own)
         {
         AZP_HIP(hipStreamSynchronize(b->own));
-        unsigned long long v[2] = {0, 0};
+        unsigned long long v[3] = {0, 0, 0};
         AZP_HIP(hipMemcpy(&v[0], b->d_timestep.ptr, sizeof(unsigned long long), hipMemcpyDeviceToHost));
-        AZP_HIP(hipMemcpy(&v[1], b->d_builds.ptr, sizeof(unsigned long long), hipMemcpyDeviceToHost));
+        AZP_HIP(hipMemcpy(&v[1], b->d_builds.ptr, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
         out->timestep = v[0];
         out->n_builds = v[1];
+        out->replica_builds = v[2];
         }
     return AZP_OK;
     }

@@ -5,7 +5,7 @@
 //
 // Build (all kernels, no host round trip except the overflow / rebuild flags):
 //   1. bin:     cell of each particle, atomically counted per cell
-//   2. scan:    exclusive prefix sum of the counts (one workgroup)
+//   2. scan:    exclusive prefix sum of the counts (three launches)
 //   3. scatter: members written into per-cell ranges, then each cell's members
 //               sorted by index so the list order is deterministic
 //   4. walk:    one lane per particle scans its 27 neighbor cells (duplicates
@@ -52,37 +52,6 @@ __global__ void k_bin(const double4* __restrict__ pos,
     const uint32_t c = (uint32_t)((cz * g.n[1] + cy) * g.n[0] + cx);
     cell_of[i] = c;
     slot_of[i] = atomicAdd(&count[c], 1u);
-    }
-
-// Exclusive scan of n counts by a single workgroup of 1024 lanes.
-__global__ void __launch_bounds__(1024) k_scan(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, uint32_t n)
-    {
-    __shared__ uint32_t s[1024];
-    const uint32_t t = threadIdx.x;
-    const uint32_t per = (n + 1023u) / 1024u;
-    const uint32_t b = t * per;
-    const uint32_t e = min(b + per, n);
-    uint32_t local = 0;
-    for (uint32_t k = b; k < e; ++k)
-        local += in[k];
-    s[t] = local;
-    __syncthreads();
-    // Hillis-Steele inclusive scan over the 1024 partial sums
-    for (uint32_t off = 1; off < 1024; off <<= 1)
-        {
-        uint32_t v = (t >= off) ? s[t - off] : 0u;
-        __syncthreads();
-        s[t] += v;
-        __syncthreads();
-        }
-    uint32_t run = s[t] - local;
-    for (uint32_t k = b; k < e; ++k)
-        {
-        out[k] = run;
-        run += in[k];
-        }
-    if (t == 1023)
-        out[n] = s[1023];
     }
 
 __global__ void k_scatter(uint32_t N,
@@ -263,7 +232,7 @@ struct azp_nlist
     DeviceArray<uint32_t> n_neigh;
     DeviceArray<uint32_t> list;
     DeviceArray<double4> last_pos;
-    DeviceArray<uint32_t> cell_of, slot_of, count, start, members;
+    DeviceArray<uint32_t> cell_of, slot_of, count, start, members, scan_sums;
     DeviceArray<uint32_t> flags; // [0] overflow, [1] rebuild, [2] bad type
     uint32_t* h_flags = nullptr;  // pinned
 
@@ -303,6 +272,7 @@ static int nlist_build(azp_nlist* nl, const azp_particles* p, hipStream_t s)
     AZP_TRY(nl->slot_of.reserve(N));
     AZP_TRY(nl->count.reserve(ncell));
     AZP_TRY(nl->start.reserve(ncell + 1));
+    AZP_TRY(nl->scan_sums.reserve(scan_chunks((uint32_t)ncell)));
     AZP_TRY(nl->members.reserve(N));
     AZP_TRY(nl->n_neigh.reserve(N));
     AZP_TRY(nl->last_pos.reserve(N));
@@ -313,7 +283,7 @@ static int nlist_build(azp_nlist* nl, const azp_particles* p, hipStream_t s)
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(ncell, B)), dim3(B), 0, s, nl->count.ptr, 0u, ncell);
     hipLaunchKernelGGL(k_bin, dim3(grid_for(N, B)), dim3(B), 0, s, pos, N, box, g,
                        nl->cell_of.ptr, nl->slot_of.ptr, nl->count.ptr);
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, nl->count.ptr, nl->start.ptr, (uint32_t)ncell);
+    enqueue_scan(nl->count.ptr, (uint32_t)ncell, nl->scan_sums.ptr, nl->start.ptr, nullptr, s);
     hipLaunchKernelGGL(k_scatter, dim3(grid_for(N, B)), dim3(B), 0, s, N, nl->cell_of.ptr,
                        nl->slot_of.ptr, nl->start.ptr, nl->members.ptr);
     hipLaunchKernelGGL(k_sort_cells, dim3(grid_for(ncell, B)), dim3(B), 0, s, (uint32_t)ncell,

@@ -422,6 +422,7 @@ def bench_batch(args):
            "particle_steps_per_s": b.N * steps / dt, "pairs_in_cut": pairs,
            "pair_interactions_per_s_in_steps": pairs * steps / dt, "force_ms": t_force * 1e3,
            "replicas": b.R, "N_total": b.N, "tuning": tune, "nlist_builds": st1["n_builds"] - st0["n_builds"],
+           "replica_rebuilds": st1["replica_builds"] - st0["replica_builds"],
            "graph_launches": st1["graph_launches"] - st0["graph_launches"], "precision": "mixed",
            "method": "Langevin (translation + rotation, I = 1), dt 0.005, per-replica kT"}
     log(f"bench: tpm_batch_512 {steps / dt:.1f} steps/s ({b.N} particles), force {t_force * 1e3:.3f} ms, "

@@ -381,7 +381,8 @@ typedef struct
 typedef struct
     {
     uint64_t timestep;
-    uint64_t n_builds;
+    uint64_t n_builds;       /* list builds (rebuild events) */
+    uint64_t replica_builds; /* replica lists rebuilt, summed over builds */
     uint64_t graph_launches;
     uint32_t max_neighbors;
     uint32_t n_cells;

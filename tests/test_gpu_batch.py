@@ -210,6 +210,15 @@ def test_short_tpm_dynamics_stay_bounded():
         f"{b.stats()}")
     assert np.all(np.isfinite(th))
     assert np.all(T < 2.0 * kT + 0.1) and np.all(T > 0.5 * kT)
+    st = b.stats()
+    assert st["replica_builds"] > st["n_builds"]  # rebuilds are per replica, several per event at most
+    # forces left by the last captured step (lists from partial per-replica
+    # rebuilds) equal those of a fresh full rebuild at the same positions
+    f_run, t_run = host(b.force).copy(), host(b.torque).copy()
+    b.compute(force_rebuild=True)
+    e_f, e_t = rel_err(f_run, host(b.force)), rel_err(t_run, host(b.torque))
+    say(f"  forces after partial rebuilds vs full rebuild: {e_f:.1e} {e_t:.1e}")
+    assert e_f < 1e-6 and e_t < 1e-6
 
 
 def test_full_batch_workload():
