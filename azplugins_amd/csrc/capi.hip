@@ -480,6 +480,12 @@ extern "C" int azp_pair_compute(azp_pair* pp,
         {
     case AZP_DRIVER_PAIR:
     case AZP_DRIVER_CONSERVATIVE:
+        if (pp->precision == AZP_PREC_MIXED)
+            {
+            AZP_TRY(azp_nlist_get_tiles(nl, &a.d_tile_count, &a.d_tile_jset, &a.d_tile_list, &a.tile_size));
+            if (a.tile_size == 0)
+                a.d_tile_list = nullptr;
+            }
         return azp_gpu_compute_pair_forces(&a, pp->evaluator, pp->d_params.ptr);
     case AZP_DRIVER_DPD_THERMO:
         return azp_gpu_compute_dpd_forces(&a, pp->d_params.ptr);

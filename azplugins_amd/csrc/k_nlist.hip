@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "cells.h"
+#include "tiles.h"
 
 namespace azp
 {
@@ -234,7 +235,10 @@ struct azp_nlist
     DeviceArray<double4> last_pos;
     DeviceArray<uint32_t> cell_of, slot_of, count, start, members, scan_sums;
     DeviceArray<uint32_t> flags; // [0] overflow, [1] rebuild, [2] bad type
-    uint32_t* h_flags = nullptr;  // pinned
+    uint32_t* h_flags = nullptr;  // pinned, 4 words ([3] tile overflow)
+
+    bool want_tiles = false;
+    TileList tiles;
 
     uint32_t pitch = 0;
     uint32_t max_neighbors = 0;
@@ -336,6 +340,18 @@ static int nlist_build(azp_nlist* nl, const azp_particles* p, hipStream_t s)
             {
             hipLaunchKernelGGL(k_copy_d4, dim3(grid_for(N, B)), dim3(B), 0, s, pos, nl->last_pos.ptr, N);
             AZP_LAUNCH_CHECK();
+            nl->tiles.valid = false;
+            if (nl->want_tiles)
+                {
+                // 512-particle tiles, else 256; without either the plain list serves
+                for (uint32_t ts : {512u, 256u})
+                    {
+                    AZP_TRY(tiles_build(nl->tiles, ts, N, nl->n_neigh.ptr, nl->list.ptr, nl->pitch,
+                                        nl->max_neighbors, nl->h_flags + 3, s));
+                    if (nl->tiles.valid)
+                        break;
+                    }
+                }
             nl->built_N = N;
             nl->built_box = p->box;
             nl->have_list = true;
@@ -485,6 +501,38 @@ extern "C" int azp_nlist_get(const azp_nlist* nl,
         *pitch = nl->pitch;
     if (max_neighbors)
         *max_neighbors = nl->max_neighbors;
+    return AZP_OK;
+    }
+
+extern "C" int azp_nlist_set_tiled(azp_nlist* nl, int enable)
+    {
+    if (!nl)
+        return set_error(AZP_ERR_INVALID, "null nlist");
+    if ((enable != 0) != nl->want_tiles)
+        nl->params_dirty = true; // next update rebuilds (and tiles) the list
+    nl->want_tiles = enable != 0;
+    if (!nl->want_tiles)
+        nl->tiles.valid = false;
+    return AZP_OK;
+    }
+
+extern "C" int azp_nlist_get_tiles(const azp_nlist* nl,
+                                   const uint32_t** d_tile_count,
+                                   const uint32_t** d_tile_jset,
+                                   const uint32_t** d_tile_list,
+                                   uint32_t* tile_size)
+    {
+    if (!nl)
+        return set_error(AZP_ERR_INVALID, "null nlist");
+    const bool ok = nl->want_tiles && nl->tiles.valid && nl->have_list;
+    if (d_tile_count)
+        *d_tile_count = ok ? nl->tiles.count.ptr : nullptr;
+    if (d_tile_jset)
+        *d_tile_jset = ok ? nl->tiles.jset.ptr : nullptr;
+    if (d_tile_list)
+        *d_tile_list = ok ? nl->tiles.list.ptr : nullptr;
+    if (tile_size)
+        *tile_size = ok ? nl->tiles.size : 0u;
     return AZP_OK;
     }
 

@@ -22,6 +22,7 @@
 #include <type_traits>
 
 #include "evaluators.h"
+#include "tiles.h"
 
 namespace azp
 {
@@ -48,6 +49,10 @@ struct PairKernelArgs
     double4* force;
     double* virial;
     uint32_t vpitch;
+    // tiled list (mixed form): per-tile neighbor sets and 16-bit entries
+    const uint32_t* tile_count;
+    const uint32_t* tile_jset;
+    const uint32_t* tile_list;
     };
 
 template<typename R> struct TypePairExtra
@@ -253,12 +258,17 @@ __device__ __forceinline__ void term_fp64(const typename Eval::template Prep<dou
     accumulate<VIRIAL>(acc, fdivr, dx, dy, dz, e);
     }
 
+// One list entry of particle i in the mixed form.  `valid` is false for
+// padding; `jkey` is the neighbor's index, or its rank in the tile's neighbor
+// set `jmap` (tiled list), needed only for the FP64 re-test near the cutoff.
 template<class Eval, bool VIRIAL, bool XPLOR, bool CUBIC>
 __device__ __forceinline__ void term_mixed(const PairKernelArgs& a,
                                            const typename Eval::template Prep<float>& q,
                                            const TypePairExtra<float>& x,
+                                           bool valid,
                                            uint32_t i,
-                                           uint32_t j,
+                                           uint32_t jkey,
+                                           const uint32_t* jmap,
                                            int32_t ix,
                                            int32_t iy,
                                            int32_t iz,
@@ -270,12 +280,12 @@ __device__ __forceinline__ void term_mixed(const PairKernelArgs& a,
         rsq = (fx * fx + fy * fy + fz * fz) * a.fx_scale2[0];
     else
         rsq = fx * fx * a.fx_scale2[0] + fy * fy * a.fx_scale2[1] + fz * fz * a.fx_scale2[2];
-    const bool valid = j != i;
     bool in = valid && rsq < x.rcsq_lo;
     double r64 = -1.0;
     if (valid && !in && rsq < x.rcsq_hi)
         {
         // FP32 cannot decide this one: apply the FP64 cutoff test
+        const uint32_t j = jmap ? jmap[jkey] : jkey;
         r64 = exact_rsq(a, i, j);
         in = r64 < x.rcsq64;
         rsq = (float)r64;
@@ -311,7 +321,14 @@ __device__ __forceinline__ void term_mixed(const PairKernelArgs& a,
     accumulate32<VIRIAL>(acc, fdivr, fx, fy, fz, e);
     }
 
-template<class Eval, bool MIXED, bool VIRIAL, bool SINGLE, bool XPLOR, bool CUBIC>
+// LDS bytes of the per-type-pair tables, rounded up to 16 (the tile's
+// positions follow them).
+template<class Eval, typename R> __host__ __device__ constexpr size_t table_bytes(uint32_t nt2)
+    {
+    return (nt2 * (sizeof(typename Eval::template Prep<R>) + sizeof(TypePairExtra<R>)) + 15) & ~(size_t)15;
+    }
+
+template<class Eval, bool MIXED, bool VIRIAL, bool SINGLE, bool XPLOR, bool CUBIC, bool TILED>
 __global__ void __launch_bounds__(512) k_pair(const PairKernelArgs a)
     {
     using R = std::conditional_t<MIXED, float, double>;
@@ -333,10 +350,26 @@ __global__ void __launch_bounds__(512) k_pair(const PairKernelArgs a)
         {
         for (uint32_t t = threadIdx.x; t < nt2; t += blockDim.x)
             prepare_pair<Eval, R>(a, t, s_prep[t], s_x[t]);
-        __syncthreads();
         }
 
     const uint32_t lb = xcd_logical_block();
+    int4* s_pos = nullptr;
+    const uint32_t* tile_js = nullptr;
+    if constexpr (TILED)
+        {
+        // stage the tile's neighbor positions (block lb is tile lb); every
+        // lane of the workgroup takes part before any wavefront may leave
+        s_pos = reinterpret_cast<int4*>(reinterpret_cast<char*>(smem) + (SINGLE ? 0 : table_bytes<Eval, R>(nt2)));
+        tile_js = a.tile_jset + (size_t)lb * kTileCap;
+        const uint32_t M = a.tile_count[lb];
+        for (uint32_t m = threadIdx.x; m < M; m += blockDim.x)
+            s_pos[m] = a.pos_fixed[tile_js[m]];
+        if (threadIdx.x == 0)
+            s_pos[kTileDummy] = make_int4(0, 0, 0, 0);
+        }
+    if (!SINGLE || TILED)
+        __syncthreads();
+
     const uint32_t i = lb * blockDim.x + threadIdx.x;
     // wavefronts entirely past N leave together; lanes past N in the last
     // wavefront shadow particle N-1 (same column, results not stored)
@@ -344,7 +377,8 @@ __global__ void __launch_bounds__(512) k_pair(const PairKernelArgs a)
         return;
     const uint32_t ie = min(i, a.N - 1);
     const uint32_t nn = a.n_neigh[ie];
-    const uint32_t P = (wave_max(nn) + U - 1) & ~(U - 1);
+    // wave-uniform trip count, held in a scalar register
+    const uint32_t P = __builtin_amdgcn_readfirstlane((wave_max(nn) + U - 1) & ~(U - 1));
     const size_t stride = a.nlist_pitch;
 
     Acc acc;
@@ -376,6 +410,46 @@ __global__ void __launch_bounds__(512) k_pair(const PairKernelArgs a)
         if (i < a.N)
             store<VIRIAL>(a, i, acc);
         }
+    else if constexpr (TILED)
+        {
+        const int4 pi = a.pos_fixed[ie];
+        const uint32_t ti = (uint32_t)pi.w;
+        const uint32_t col_off = ie * 4u;
+        for (uint32_t k = 0; k < P; k += U)
+            {
+            uint32_t w[U / 2];
+#pragma unroll
+            for (uint32_t u = 0; u < U / 2; ++u)
+                {
+                const char* row = reinterpret_cast<const char*>(a.tile_list + (size_t)((k >> 1) + u) * stride);
+                w[u] = *reinterpret_cast<const uint32_t*>(row + col_off);
+                }
+            uint32_t e[U];
+#pragma unroll
+            for (uint32_t u = 0; u < U / 2; ++u)
+                {
+                e[2 * u] = w[u] & 0xffffu;
+                e[2 * u + 1] = w[u] >> 16;
+                }
+            int4 pj[U];
+#pragma unroll
+            for (uint32_t u = 0; u < U; ++u)
+                pj[u] = s_pos[e[u]];
+            Acc32 part;
+#pragma unroll
+            for (uint32_t u = 0; u < U; ++u)
+                {
+                const uint32_t tp = SINGLE ? 0u : ti * nt + (uint32_t)pj[u].w;
+                term_mixed<Eval, VIRIAL, XPLOR, CUBIC>(a, SINGLE ? q0 : s_prep[tp], SINGLE ? x0 : s_x[tp],
+                                                       e[u] != kTileDummy, ie, e[u], tile_js,
+                                                       fixed_delta(pi.x, pj[u].x), fixed_delta(pi.y, pj[u].y),
+                                                       fixed_delta(pi.z, pj[u].z), part);
+                }
+            flush32<VIRIAL>(acc, part);
+            }
+        if (i < a.N)
+            store_scaled<VIRIAL>(a, i, acc);
+        }
     else
         {
         const int4 pi = a.pos_fixed[ie];
@@ -402,8 +476,8 @@ __global__ void __launch_bounds__(512) k_pair(const PairKernelArgs a)
             for (uint32_t u = 0; u < U; ++u)
                 {
                 const uint32_t tp = SINGLE ? 0u : ti * nt + (uint32_t)pj[u].w;
-                term_mixed<Eval, VIRIAL, XPLOR, CUBIC>(a, SINGLE ? q0 : s_prep[tp], SINGLE ? x0 : s_x[tp], ie,
-                                                       jj[u], fixed_delta(pi.x, pj[u].x),
+                term_mixed<Eval, VIRIAL, XPLOR, CUBIC>(a, SINGLE ? q0 : s_prep[tp], SINGLE ? x0 : s_x[tp],
+                                                       jj[u] != ie, ie, jj[u], nullptr, fixed_delta(pi.x, pj[u].x),
                                                        fixed_delta(pi.y, pj[u].y), fixed_delta(pi.z, pj[u].z),
                                                        part);
                 }
@@ -426,23 +500,24 @@ __global__ void k_pack_fixed(const double4* __restrict__ pos, uint32_t N, DevBox
                        (int)p.w);
     }
 
-template<class Eval, bool MIXED, bool VIRIAL, bool SINGLE, bool XPLOR, bool CUBIC>
+template<class Eval, bool MIXED, bool VIRIAL, bool SINGLE, bool XPLOR, bool CUBIC, bool TILED>
 static int launch_t(const PairKernelArgs& ka, unsigned block, hipStream_t stream)
     {
     using R = std::conditional_t<MIXED, float, double>;
     const size_t nt2 = (size_t)ka.ntypes * ka.ntypes;
-    const size_t shmem
-        = SINGLE ? 0 : nt2 * (sizeof(typename Eval::template Prep<R>) + sizeof(TypePairExtra<R>));
+    size_t shmem = SINGLE ? 0 : table_bytes<Eval, R>((uint32_t)nt2);
+    if (TILED)
+        shmem += (size_t)kTileCap * sizeof(int4);
     if (shmem > 64 * 1024)
         return set_error(AZP_ERR_UNSUPPORTED, "too many particle types (%u) for LDS staging", ka.ntypes);
     const unsigned grid = grid_for(ka.N, block);
-    hipLaunchKernelGGL((k_pair<Eval, MIXED, VIRIAL, SINGLE, XPLOR, CUBIC>), dim3(grid), dim3(block), shmem,
+    hipLaunchKernelGGL((k_pair<Eval, MIXED, VIRIAL, SINGLE, XPLOR, CUBIC, TILED>), dim3(grid), dim3(block), shmem,
                        stream, ka);
     AZP_LAUNCH_CHECK();
     return AZP_OK;
     }
 
-template<class Eval, bool MIXED, bool VIRIAL, bool SINGLE>
+template<class Eval, bool MIXED, bool VIRIAL, bool SINGLE, bool TILED>
 static int launch_mode(const PairKernelArgs& ka, unsigned block, hipStream_t s)
     {
     const bool xplor = ka.shift_mode == AZP_SHIFT_XPLOR;
@@ -450,27 +525,42 @@ static int launch_mode(const PairKernelArgs& ka, unsigned block, hipStream_t s)
         {
         const bool cubic = ka.fx_scale2[0] == ka.fx_scale2[1] && ka.fx_scale2[0] == ka.fx_scale2[2];
         if (xplor)
-            return cubic ? launch_t<Eval, true, VIRIAL, SINGLE, true, true>(ka, block, s)
-                         : launch_t<Eval, true, VIRIAL, SINGLE, true, false>(ka, block, s);
-        return cubic ? launch_t<Eval, true, VIRIAL, SINGLE, false, true>(ka, block, s)
-                     : launch_t<Eval, true, VIRIAL, SINGLE, false, false>(ka, block, s);
+            return cubic ? launch_t<Eval, true, VIRIAL, SINGLE, true, true, TILED>(ka, block, s)
+                         : launch_t<Eval, true, VIRIAL, SINGLE, true, false, TILED>(ka, block, s);
+        return cubic ? launch_t<Eval, true, VIRIAL, SINGLE, false, true, TILED>(ka, block, s)
+                     : launch_t<Eval, true, VIRIAL, SINGLE, false, false, TILED>(ka, block, s);
         }
     else
         {
-        return xplor ? launch_t<Eval, false, VIRIAL, SINGLE, true, false>(ka, block, s)
-                     : launch_t<Eval, false, VIRIAL, SINGLE, false, false>(ka, block, s);
+        return xplor ? launch_t<Eval, false, VIRIAL, SINGLE, true, false, false>(ka, block, s)
+                     : launch_t<Eval, false, VIRIAL, SINGLE, false, false, false>(ka, block, s);
         }
     }
 
-template<class Eval, bool MIXED>
-static int launch_eval(const PairKernelArgs& ka, bool virial, unsigned block, hipStream_t s)
+template<class Eval, bool MIXED, bool TILED>
+static int launch_virial(const PairKernelArgs& ka, bool virial, unsigned block, hipStream_t s)
     {
     const bool single = ka.ntypes == 1;
     if (virial)
-        return single ? launch_mode<Eval, MIXED, true, true>(ka, block, s)
-                      : launch_mode<Eval, MIXED, true, false>(ka, block, s);
-    return single ? launch_mode<Eval, MIXED, false, true>(ka, block, s)
-                  : launch_mode<Eval, MIXED, false, false>(ka, block, s);
+        return single ? launch_mode<Eval, MIXED, true, true, TILED>(ka, block, s)
+                      : launch_mode<Eval, MIXED, true, false, TILED>(ka, block, s);
+    return single ? launch_mode<Eval, MIXED, false, true, TILED>(ka, block, s)
+                  : launch_mode<Eval, MIXED, false, false, TILED>(ka, block, s);
+    }
+
+// The tiled list serves the mixed form when the tile tables fit next to the
+// type-pair tables in 64 KB of LDS; otherwise the plain list is used.
+template<class Eval, bool MIXED>
+static int launch_eval(const PairKernelArgs& ka, bool virial, unsigned block, unsigned tile_size, hipStream_t s)
+    {
+    if constexpr (MIXED)
+        {
+        const size_t nt2 = (size_t)ka.ntypes * ka.ntypes;
+        const size_t tables = ka.ntypes == 1 ? 0 : table_bytes<Eval, float>((uint32_t)nt2);
+        if (tile_size && ka.tile_list && tables + (size_t)kTileCap * sizeof(int4) <= 64 * 1024)
+            return launch_virial<Eval, true, true>(ka, virial, tile_size, s);
+        }
+    return launch_virial<Eval, MIXED, false>(ka, virial, block, s);
     }
 
 int pack_fixed(const double* d_pos, uint32_t N, const azp_box& box, int32_t* d_out, hipStream_t stream)
@@ -567,6 +657,18 @@ extern "C" int azp_gpu_compute_pair_forces(const azp_pair_args* args, int evalua
     ka.force = reinterpret_cast<double4*>(args->out.d_force);
     ka.virial = args->out.d_virial;
     ka.vpitch = args->out.virial_pitch;
+    ka.tile_count = args->d_tile_count;
+    ka.tile_jset = args->d_tile_jset;
+    ka.tile_list = args->d_tile_list;
+    unsigned tile = 0;
+    if (args->d_tile_list)
+        {
+        if (!args->d_tile_count || !args->d_tile_jset)
+            return set_error(AZP_ERR_INVALID, "a tiled list needs its counts, neighbor sets and entries");
+        if (args->tile_size != 256 && args->tile_size != 512)
+            return set_error(AZP_ERR_INVALID, "tile size %u must be 256 or 512", args->tile_size);
+        tile = args->tile_size;
+        }
     // 512-lane workgroups: measured fastest for both forms on the 1M-particle
     // PerturbedLJ workload (block sweep 64..512)
     const unsigned block = args->block_size ? args->block_size : 512;
@@ -578,24 +680,24 @@ extern "C" int azp_gpu_compute_pair_forces(const azp_pair_args* args, int evalua
     switch (evaluator)
         {
     case AZP_EVAL_HERTZ:
-        return mixed ? launch_eval<EvalHertz, true>(ka, vir, block, s)
-                     : launch_eval<EvalHertz, false>(ka, vir, block, s);
+        return mixed ? launch_eval<EvalHertz, true>(ka, vir, block, tile, s)
+                     : launch_eval<EvalHertz, false>(ka, vir, block, 0, s);
     case AZP_EVAL_EXPANDED_YUKAWA:
-        return mixed ? launch_eval<EvalExpandedYukawa, true>(ka, vir, block, s)
-                     : launch_eval<EvalExpandedYukawa, false>(ka, vir, block, s);
+        return mixed ? launch_eval<EvalExpandedYukawa, true>(ka, vir, block, tile, s)
+                     : launch_eval<EvalExpandedYukawa, false>(ka, vir, block, 0, s);
     case AZP_EVAL_PERTURBED_LJ:
-        return mixed ? launch_eval<EvalPerturbedLJ, true>(ka, vir, block, s)
-                     : launch_eval<EvalPerturbedLJ, false>(ka, vir, block, s);
+        return mixed ? launch_eval<EvalPerturbedLJ, true>(ka, vir, block, tile, s)
+                     : launch_eval<EvalPerturbedLJ, false>(ka, vir, block, 0, s);
     case AZP_EVAL_COLLOID:
         if (mixed)
             return set_error(AZP_ERR_UNSUPPORTED,
                              "Colloid runs in the plain FP64 form only (its colloid-colloid "
                              "branch cancels catastrophically in FP32)");
-        return launch_eval<EvalColloid, false>(ka, vir, block, s);
+        return launch_eval<EvalColloid, false>(ka, vir, block, 0, s);
     case AZP_EVAL_DPD_GENERAL_WEIGHT:
         // PotentialPairConservative<DPDPairEvaluatorGeneralWeight>
-        return mixed ? launch_eval<EvalDPDConservative, true>(ka, vir, block, s)
-                     : launch_eval<EvalDPDConservative, false>(ka, vir, block, s);
+        return mixed ? launch_eval<EvalDPDConservative, true>(ka, vir, block, tile, s)
+                     : launch_eval<EvalDPDConservative, false>(ka, vir, block, 0, s);
     default:
         return set_error(AZP_ERR_INVALID, "evaluator %d is not an isotropic pair evaluator", evaluator);
         }

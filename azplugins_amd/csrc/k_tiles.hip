@@ -1,0 +1,195 @@
+// Build of the tiled neighbor list (see tiles.h): one workgroup per tile.
+//
+//   1. every lane inserts its particle's neighbor indices into an LDS hash
+//      set (open addressing, linear probing); insertion stops once the set
+//      holds more than the tile capacity, which is then reported,
+//   2. the set is compacted and bitonic-sorted in LDS (deterministic order,
+//      ascending indices so the force kernel's staging reads are near
+//      contiguous), written to jset, and each key's hash slot records its
+//      rank,
+//   3. every lane rewrites its list entries as ranks, two per word.
+#include "device_common.h"
+#include "tiles.h"
+
+namespace azp
+{
+
+constexpr uint32_t kHashSlots = 7680;   // load factor <= 0.6 at capacity
+constexpr uint32_t kSortSlots = 4096;   // >= kTileCap, power of two
+constexpr uint32_t kEmpty = 0xffffffffu;
+
+__device__ __forceinline__ uint32_t tile_hash(uint32_t j)
+    {
+    return (uint32_t)(((uint64_t)(j * 2654435761u) * kHashSlots) >> 32);
+    }
+
+__device__ __forceinline__ void tile_insert(uint32_t* keys, uint32_t* count, uint32_t j)
+    {
+    uint32_t h = tile_hash(j);
+    for (uint32_t probe = 0; probe < kHashSlots; ++probe)
+        {
+        const uint32_t prev = atomicCAS(&keys[h], kEmpty, j);
+        if (prev == kEmpty)
+            {
+            atomicAdd(count, 1u);
+            return;
+            }
+        if (prev == j)
+            return;
+        h = (h + 1 == kHashSlots) ? 0u : h + 1;
+        }
+    }
+
+__device__ __forceinline__ uint32_t tile_find(const uint32_t* keys, uint32_t j)
+    {
+    uint32_t h = tile_hash(j);
+    for (uint32_t probe = 0; probe < kHashSlots; ++probe)
+        {
+        if (keys[h] == j)
+            return h;
+        h = (h + 1 == kHashSlots) ? 0u : h + 1;
+        }
+    return 0u; // not reached: every listed index was inserted
+    }
+
+__global__ void __launch_bounds__(512) k_tile_build(uint32_t N,
+                                                    const uint32_t* __restrict__ n_neigh,
+                                                    const uint32_t* __restrict__ nlist,
+                                                    uint32_t pitch,
+                                                    uint32_t* __restrict__ tile_count,
+                                                    uint32_t* __restrict__ tile_jset,
+                                                    uint32_t* __restrict__ tlist,
+                                                    uint32_t* __restrict__ overflow)
+    {
+    __shared__ uint32_t keys[kHashSlots];
+    __shared__ uint16_t rank[kHashSlots];
+    __shared__ uint32_t sorted[kSortSlots];
+    __shared__ uint32_t s_count, s_next;
+    const uint32_t T = blockDim.x;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t t = blockIdx.x;
+    for (uint32_t k = tid; k < kHashSlots; k += T)
+        keys[k] = kEmpty;
+    if (tid == 0)
+        {
+        s_count = 0u;
+        s_next = 0u;
+        }
+    __syncthreads();
+
+    const uint32_t i = t * T + tid;
+    const uint32_t nn = i < N ? n_neigh[i] : 0u;
+    volatile uint32_t* vcount = &s_count;
+    for (uint32_t k = 0; k < nn; ++k)
+        {
+        if (*vcount >= kTileCap)
+            break;
+        tile_insert(keys, &s_count, nlist[(size_t)k * pitch + i]);
+        }
+    __syncthreads();
+    const uint32_t M = s_count;
+    if (M > kTileCap - 1)
+        {
+        if (tid == 0)
+            atomicMax(overflow, M);
+        return; // the whole workgroup leaves (M is the same for every lane)
+        }
+
+    for (uint32_t k = tid; k < kHashSlots; k += T)
+        {
+        const uint32_t key = keys[k];
+        if (key != kEmpty)
+            sorted[atomicAdd(&s_next, 1u)] = key;
+        }
+    for (uint32_t k = M + tid; k < kSortSlots; k += T)
+        sorted[k] = kEmpty;
+    __syncthreads();
+    for (uint32_t size = 2; size <= kSortSlots; size <<= 1)
+        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1)
+            {
+            for (uint32_t e = tid; e < kSortSlots; e += T)
+                {
+                const uint32_t p = e ^ stride;
+                if (p > e)
+                    {
+                    const bool up = (e & size) == 0u;
+                    const uint32_t a = sorted[e], b = sorted[p];
+                    if ((a > b) == up)
+                        {
+                        sorted[e] = b;
+                        sorted[p] = a;
+                        }
+                    }
+                }
+            __syncthreads();
+            }
+    for (uint32_t m = tid; m < M; m += T)
+        {
+        const uint32_t key = sorted[m];
+        tile_jset[(size_t)t * kTileCap + m] = key;
+        rank[tile_find(keys, key)] = (uint16_t)m;
+        }
+    if (tid == 0)
+        tile_count[t] = M;
+    __syncthreads();
+
+    // entries as ranks, two per word, padded to the wavefront's longest list
+    // rounded up to 8
+    uint32_t wmax = nn;
+    for (int off = 32; off > 0; off >>= 1)
+        wmax = max(wmax, (uint32_t)__shfl_xor((int)wmax, off));
+    const uint32_t P = (wmax + 7u) & ~7u;
+    if (i >= N)
+        return;
+    for (uint32_t k = 0; k < P; k += 2)
+        {
+        const uint32_t e0 = k < nn ? rank[tile_find(keys, nlist[(size_t)k * pitch + i])] : kTileDummy;
+        const uint32_t e1 = k + 1 < nn ? rank[tile_find(keys, nlist[(size_t)(k + 1) * pitch + i])] : kTileDummy;
+        tlist[(size_t)(k >> 1) * pitch + i] = e0 | (e1 << 16);
+        }
+    }
+
+__global__ void k_tile_flag_clear(uint32_t* flag)
+    {
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        flag[0] = 0u;
+    }
+
+int tiles_build(TileList& t,
+                uint32_t tile_size,
+                uint32_t N,
+                const uint32_t* d_n_neigh,
+                const uint32_t* d_nlist,
+                uint32_t pitch,
+                uint32_t max_neighbors,
+                uint32_t* h_flag,
+                hipStream_t s)
+    {
+    t.valid = false;
+    t.size = 0;
+    if (tile_size != 256 && tile_size != 512)
+        return set_error(AZP_ERR_INVALID, "tile size %u must be 256 or 512", tile_size);
+    if (N == 0)
+        return AZP_OK;
+    const uint32_t ntiles = (N + tile_size - 1) / tile_size;
+    // words per column: the padded length (<= max_neighbors rounded up to 8) over 2
+    const uint32_t words = (((max_neighbors + 7u) & ~7u) + 1u) / 2u;
+    AZP_TRY(t.count.reserve(ntiles));
+    AZP_TRY(t.jset.reserve((size_t)ntiles * kTileCap));
+    AZP_TRY(t.list.reserve((size_t)words * pitch));
+    AZP_TRY(t.flag.reserve(1));
+    hipLaunchKernelGGL(k_tile_flag_clear, dim3(1), dim3(64), 0, s, t.flag.ptr);
+    hipLaunchKernelGGL(k_tile_build, dim3(ntiles), dim3(tile_size), 0, s, N, d_n_neigh, d_nlist, pitch, t.count.ptr,
+                       t.jset.ptr, t.list.ptr, t.flag.ptr);
+    AZP_LAUNCH_CHECK();
+    AZP_HIP(hipMemcpyAsync(h_flag, t.flag.ptr, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    AZP_HIP(hipStreamSynchronize(s));
+    if (h_flag[0] != 0u)
+        return AZP_OK; // some tile has too many distinct neighbors
+    t.ntiles = ntiles;
+    t.size = tile_size;
+    t.valid = true;
+    return AZP_OK;
+    }
+
+} // namespace azp

@@ -26,6 +26,7 @@ class NeighborList:
             self._sim = sim
             self._users = []
             self._last = None
+            self._tiled = False
 
     def _register(self, pair):
         if pair not in self._users:
@@ -43,12 +44,23 @@ class NeighborList:
                 m[k] = max(m[k], v)
         return nt, m
 
+    def _wants_tiles(self):
+        """The tiled list serves isotropic pair forces in the mixed form."""
+        from ... import _azp
+
+        return any(getattr(u, "_precision", "fp64") == "mixed" and u._cpp is not None
+                   and u._cpp.driver in (_azp.DRIVER_PAIR, _azp.DRIVER_CONSERVATIVE) for u in self._users)
+
     def _update(self, timestep):
         if self._last == timestep:
             return
         nt, m = self._r_cut_matrix()
         self._cpp.set_r_buff(self.buffer)
         self._cpp.set_r_cut(nt, m)
+        tiled = self._wants_tiles()
+        if tiled != getattr(self, "_tiled", False):
+            self._cpp.set_tiled(tiled)
+            self._tiled = tiled
         self._cpp.update(self._sim.state.particles())
         self._last = timestep
 

@@ -96,6 +96,14 @@ def main():
     nl.update(parts, force_rebuild=True)
     torch.cuda.synchronize()
     t_nlist = time.perf_counter() - t0
+    # the mixed form reads the tiled list (16-bit entries, tile neighbors in LDS)
+    nl.set_tiled(True)
+    t0 = time.perf_counter()
+    nl.update(parts, force_rebuild=True)
+    torch.cuda.synchronize()
+    t_nlist_tiled = time.perf_counter() - t0
+    tiles = nl.get_tiles()
+    log(f"bench: tile size {tiles[3]} (0 = plain list)")
     d_nn, d_list, pitch, mx = nl.get()
     nn_host, _ = nl.download(N)
     nlist_entries = int(nn_host.sum())
@@ -131,6 +139,7 @@ def main():
         return a
 
     args_mixed = make_args(1)
+    args_mixed.d_tile_count, args_mixed.d_tile_jset, args_mixed.d_tile_list, args_mixed.tile_size = tiles
     args_fp64 = make_args(0)
 
     def step_mixed():
@@ -209,7 +218,7 @@ def main():
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": traffic,
-        "kernel": "k_pair<EvalPerturbedLJ, MIXED, VIRIAL, SINGLE>",
+        "kernel": "k_pair<EvalPerturbedLJ, MIXED, VIRIAL, SINGLE%s>" % (", TILED" if tiles[3] else ""),
         "kernel_us": round(k_mixed * 1e6, 2),
         "algorithmic_bytes": bytes_mixed,
     }
@@ -249,6 +258,8 @@ def main():
         "max_rel_force_err_vs_fp64": df,
         "l2_rel_force_err_vs_fp64": l2,
         "nlist_build_ms": t_nlist * 1e3,
+        "nlist_build_with_tiles_ms": t_nlist_tiled * 1e3,
+        "tile_size": tiles[3],
     }
     if not args.no_tps:
         out["tps"] = run_tps(c, args.tps_steps)

@@ -21,7 +21,9 @@
  *   virial       double[6*pitch] xx, xy, xz, yy, yz, zz, component-major
  *   torque       double4[N]  tx, ty, tz, 0           (anisotropic output)
  *   neighbor list: n_neigh uint32[N]; nlist uint32[max_neighbors * pitch],
- *                  entry k of particle i at nlist[k * pitch + i] (full list).
+ *                  entry k of particle i at nlist[k * pitch + i] (full list);
+ *                  each 64-particle group's columns are padded with the
+ *                  particle's own index up to its longest list rounded up to 8.
  * The box is orthorhombic and periodic in x, y, z; positions in [-L/2, L/2).
  */
 #ifndef AZP_H
@@ -182,6 +184,12 @@ typedef struct
     double dt;
     double kT;
     void* stream;
+    /* optional tiled list for the mixed form (azp_nlist_get_tiles); NULL
+     * d_tile_list = use d_nlist only */
+    const uint32_t* d_tile_count;
+    const uint32_t* d_tile_jset;
+    const uint32_t* d_tile_list;
+    uint32_t tile_size;
     } azp_pair_args;
 
 /* ---- library ---- */
@@ -317,6 +325,18 @@ int azp_nlist_get(const azp_nlist* nl,
                   uint32_t* pitch,
                   uint32_t* max_neighbors);
 int azp_nlist_num_builds(const azp_nlist* nl, uint64_t* n);
+/* Tiled form of the list for the mixed-precision pair kernel (this
+ * library's own addition; no reference counterpart): the full list of each
+ * block of 512 (or 256) consecutive particles rewritten as 16-bit indices
+ * into the block's sorted set of distinct neighbors.  Built with every
+ * rebuild once enabled; tile_size = 0 when some block has more than 3967
+ * distinct neighbors (the plain list is then used). */
+int azp_nlist_set_tiled(azp_nlist* nl, int enable);
+int azp_nlist_get_tiles(const azp_nlist* nl,
+                        const uint32_t** d_tile_count,
+                        const uint32_t** d_tile_jset,
+                        const uint32_t** d_tile_list,
+                        uint32_t* tile_size);
 /* Copy n_neigh[N] and the list (max_neighbors * pitch entries) to host memory
  * (synchronizes the device). */
 int azp_nlist_download(const azp_nlist* nl,

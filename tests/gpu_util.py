@@ -69,7 +69,7 @@ class Result:
 
 
 def gpu_pair(cls_name, system, ntypes, params, r_cut, r_buff=0.4, mode="none", precision="fp64",
-             r_on=None, thermostat=None, timestep=0, virial=True, evaluator=None):
+             r_on=None, thermostat=None, timestep=0, virial=True, evaluator=None, tiled=False):
     """One force evaluation through azp_nlist_update + azp_pair_compute."""
     pp = getattr(_azplugins, cls_name + "GPU")(ntypes)
     ev = pp.evaluator
@@ -88,6 +88,8 @@ def gpu_pair(cls_name, system, ntypes, params, r_cut, r_buff=0.4, mode="none", p
         pp.setThermostat(*thermostat)
     nl = _azplugins.NeighborListGPU(r_buff)
     nl.set_r_cut(ntypes, pp.getRCutMatrix())
+    if tiled:
+        nl.set_tiled(True)
     parts = system.particles()
     nl.update(parts)
     N = system.N
