@@ -68,12 +68,13 @@ def test_tiled_equals_plain_two_types(mode):
 
 
 def test_tiles_fall_back_when_a_block_has_too_many_neighbors():
-    """r_cut 6 at density 0.85: ~900 neighbors per particle, far more than a tile holds."""
-    L = 16.0
-    pos4, _ = jittered_lattice(17, L, 0.1, 1, seed=5)
+    """r_cut 7 at density 1.37: ~2300 neighbors per particle; the distinct neighbors of
+    256 consecutive particles (~7000) exceed what a tile holds."""
+    L = 18.0
+    pos4, _ = jittered_lattice(20, L, 0.1, 1, seed=5)
     Lv = np.array([L, L, L])
     sysm = GpuSystem(pos4, Lv)
-    plain, tiled = _both("PotentialPairPerturbedLennardJones", sysm, 1, PLJ, 6.0)
+    plain, tiled = _both("PotentialPairPerturbedLennardJones", sysm, 1, PLJ, 7.0)
     assert tiled.nl.get_tiles()[3] == 0
     assert np.array_equal(plain.force, tiled.force)
 
@@ -102,9 +103,9 @@ def test_tiled_launcher_entry_point():
     a = raw_args(sysm, ref.nl, rcsq, None, 1, force, vir, pitch, precision=1)
     a.particles.d_pos_fixed = pf.data_ptr()
     a.d_tile_count, a.d_tile_jset, a.d_tile_list, a.tile_size = ref.nl.get_tiles()
-    _azp.call("azp_gpu_compute_pair_forces", ctypes.byref(a), 2, d_params)
+    _azp.call("azp_gpu_compute_pair_forces", ctypes.byref(a), 2, _azp.dptr(d_params))
     torch.cuda.synchronize()
     assert np.array_equal(force.cpu().numpy(), ref.force)
     a.tile_size = 128
     with pytest.raises(_azp.AzpError):
-        _azp.call("azp_gpu_compute_pair_forces", ctypes.byref(a), 2, d_params)
+        _azp.call("azp_gpu_compute_pair_forces", ctypes.byref(a), 2, _azp.dptr(d_params))
