@@ -98,6 +98,7 @@ class PairArgs(ctypes.Structure):
         ("d_tile_jset", ctypes.c_void_p),
         ("d_tile_list", ctypes.c_void_p),
         ("tile_size", ctypes.c_uint32),
+        ("tile_slots", ctypes.c_uint32),
     ]
 
 
@@ -179,7 +180,7 @@ SIGNATURES = [
     ("azp_nlist_download", _i, [_vp, _u32, _vp, _vp, _u64]),
     ("azp_nlist_set_tiled", _i, [_vp, _i]),
     ("azp_nlist_get_tiles", _i, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp),
-                                 ctypes.POINTER(_u32)]),
+                                 ctypes.POINTER(_u32), ctypes.POINTER(_u32)]),
     ("azp_pair_create", _i, [ctypes.POINTER(_vp), ctypes.c_char_p, _u32]),
     ("azp_pair_destroy", _i, [_vp]),
     ("azp_pair_set_params", _i, [_vp, _u32, _u32, _pd]),

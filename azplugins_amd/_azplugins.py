@@ -150,19 +150,24 @@ class NeighborListGPU:
         _azp.call("azp_nlist_get", self._h, ctypes.byref(nn), ctypes.byref(nl), ctypes.byref(pitch), ctypes.byref(mx))
         return nn.value, nl.value, pitch.value, mx.value
 
-    def set_tiled(self, enable=True):
-        """Also build the tiled list (16-bit entries per block of particles) for the mixed form."""
-        _azp.call("azp_nlist_set_tiled", self._h, 1 if enable else 0)
+    def set_tiled(self, mode=True):
+        """Also build the tiled list (16-bit entries per block of particles) for the mixed form.
+
+        mode: False / 0 off, True / 1 blocks of 512 else 256, or 256 / 512 for that size only.
+        """
+        m = int(mode) if not isinstance(mode, bool) else (1 if mode else 0)
+        _azp.call("azp_nlist_set_tiled", self._h, m)
 
     def get_tiles(self):
-        """(d_tile_count, d_tile_jset, d_tile_list, tile_size); tile_size 0 when not available."""
+        """(d_tile_count, d_tile_jset, d_tile_list, tile_size, tile_slots); tile_size 0 when not available."""
         c = ctypes.c_void_p()
         j = ctypes.c_void_p()
         t = ctypes.c_void_p()
         size = ctypes.c_uint32()
+        slots = ctypes.c_uint32()
         _azp.call("azp_nlist_get_tiles", self._h, ctypes.byref(c), ctypes.byref(j), ctypes.byref(t),
-                  ctypes.byref(size))
-        return c.value, j.value, t.value, size.value
+                  ctypes.byref(size), ctypes.byref(slots))
+        return c.value, j.value, t.value, size.value, slots.value
 
     def download(self, N):
         """(n_neigh[N], list[max_neighbors, pitch]) as numpy arrays."""

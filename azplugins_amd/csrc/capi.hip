@@ -470,7 +470,12 @@ extern "C" int azp_pair_compute(azp_pair* pp,
     a.dt = pp->dt;
     a.kT = pp->kT;
     a.stream = stream;
-    if (pp->precision == AZP_PREC_MIXED && !p->d_pos_fixed && p->N > 0)
+    uint32_t tiled = 0;
+    if (pp->precision == AZP_PREC_MIXED
+        && (pp->driver == AZP_DRIVER_PAIR || pp->driver == AZP_DRIVER_CONSERVATIVE))
+        AZP_TRY(azp_nlist_get_tiles(nl, nullptr, nullptr, nullptr, &tiled, nullptr));
+    // the tiled kernel converts positions itself
+    if (pp->precision == AZP_PREC_MIXED && !p->d_pos_fixed && p->N > 0 && tiled == 0)
         {
         AZP_TRY(pp->d_pos_fixed.reserve((size_t)p->N * 4));
         AZP_TRY(pack_fixed(p->d_pos, p->N, p->box, pp->d_pos_fixed.ptr, s));
@@ -482,7 +487,8 @@ extern "C" int azp_pair_compute(azp_pair* pp,
     case AZP_DRIVER_CONSERVATIVE:
         if (pp->precision == AZP_PREC_MIXED)
             {
-            AZP_TRY(azp_nlist_get_tiles(nl, &a.d_tile_count, &a.d_tile_jset, &a.d_tile_list, &a.tile_size));
+            AZP_TRY(azp_nlist_get_tiles(nl, &a.d_tile_count, &a.d_tile_jset, &a.d_tile_list, &a.tile_size,
+                                        &a.tile_slots));
             if (a.tile_size == 0)
                 a.d_tile_list = nullptr;
             }

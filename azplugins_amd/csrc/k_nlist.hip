@@ -235,9 +235,10 @@ struct azp_nlist
     DeviceArray<double4> last_pos;
     DeviceArray<uint32_t> cell_of, slot_of, count, start, members, scan_sums;
     DeviceArray<uint32_t> flags; // [0] overflow, [1] rebuild, [2] bad type
-    uint32_t* h_flags = nullptr;  // pinned, 4 words ([3] tile overflow)
+    uint32_t* h_flags = nullptr;  // pinned, 8 words ([4], [5] tile build flags)
 
     bool want_tiles = false;
+    uint32_t tile_mode = 0; // 1: 512 else 256; 256 / 512: that size only
     TileList tiles;
 
     uint32_t pitch = 0;
@@ -346,8 +347,10 @@ static int nlist_build(azp_nlist* nl, const azp_particles* p, hipStream_t s)
                 // 512-particle tiles, else 256; without either the plain list serves
                 for (uint32_t ts : {512u, 256u})
                     {
+                    if (nl->tile_mode != 1 && nl->tile_mode != ts)
+                        continue;
                     AZP_TRY(tiles_build(nl->tiles, ts, N, nl->n_neigh.ptr, nl->list.ptr, nl->pitch,
-                                        nl->max_neighbors, nl->h_flags + 3, s));
+                                        nl->max_neighbors, nl->h_flags + 4, s));
                     if (nl->tiles.valid)
                         break;
                     }
@@ -368,7 +371,7 @@ extern "C" int azp_nlist_create(azp_nlist** out)
     if (!out)
         return set_error(AZP_ERR_INVALID, "null out");
     azp_nlist* nl = new azp_nlist();
-    if (hipHostMalloc((void**)&nl->h_flags, 4 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess)
+    if (hipHostMalloc((void**)&nl->h_flags, 8 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess)
         {
         nl->h_flags = nullptr;
         delete nl;
@@ -504,13 +507,16 @@ extern "C" int azp_nlist_get(const azp_nlist* nl,
     return AZP_OK;
     }
 
-extern "C" int azp_nlist_set_tiled(azp_nlist* nl, int enable)
+extern "C" int azp_nlist_set_tiled(azp_nlist* nl, int mode)
     {
     if (!nl)
         return set_error(AZP_ERR_INVALID, "null nlist");
-    if ((enable != 0) != nl->want_tiles)
+    if (mode != 0 && mode != 1 && mode != 256 && mode != 512)
+        return set_error(AZP_ERR_INVALID, "tile mode %d must be 0, 1, 256 or 512", mode);
+    if ((uint32_t)mode != nl->tile_mode)
         nl->params_dirty = true; // next update rebuilds (and tiles) the list
-    nl->want_tiles = enable != 0;
+    nl->tile_mode = (uint32_t)mode;
+    nl->want_tiles = mode != 0;
     if (!nl->want_tiles)
         nl->tiles.valid = false;
     return AZP_OK;
@@ -520,7 +526,8 @@ extern "C" int azp_nlist_get_tiles(const azp_nlist* nl,
                                    const uint32_t** d_tile_count,
                                    const uint32_t** d_tile_jset,
                                    const uint32_t** d_tile_list,
-                                   uint32_t* tile_size)
+                                   uint32_t* tile_size,
+                                   uint32_t* tile_slots)
     {
     if (!nl)
         return set_error(AZP_ERR_INVALID, "null nlist");
@@ -533,6 +540,8 @@ extern "C" int azp_nlist_get_tiles(const azp_nlist* nl,
         *d_tile_list = ok ? nl->tiles.list.ptr : nullptr;
     if (tile_size)
         *tile_size = ok ? nl->tiles.size : 0u;
+    if (tile_slots)
+        *tile_slots = ok ? nl->tiles.slots : 0u;
     return AZP_OK;
     }
 

@@ -55,6 +55,12 @@ struct PairKernelArgs
     const uint32_t* tile_list;
     };
 
+// Position (double4 x, y, z, type) -> wrapped 32-bit fixed point of the box.
+__device__ __forceinline__ int4 fixed4(const double4& p, const DevBox& box)
+    {
+    return make_int4(to_fixed(p.x, box.Linv[0]), to_fixed(p.y, box.Linv[1]), to_fixed(p.z, box.Linv[2]), (int)p.w);
+    }
+
 template<typename R> struct TypePairExtra
     {
     double rcsq64;
@@ -355,17 +361,19 @@ __global__ void __launch_bounds__(512) k_pair(const PairKernelArgs a)
     const uint32_t lb = xcd_logical_block();
     int4* s_pos = nullptr;
     const uint32_t* tile_js = nullptr;
+    uint32_t tile_m = 0;
     if constexpr (TILED)
         {
-        // stage the tile's neighbor positions (block lb is tile lb); every
-        // lane of the workgroup takes part before any wavefront may leave
+        // stage the tile's neighbor positions, converted to fixed point here
+        // (block lb is tile lb); every lane of the workgroup takes part before
+        // any wavefront may leave.  Slot M is the padding entries' dummy.
         s_pos = reinterpret_cast<int4*>(reinterpret_cast<char*>(smem) + (SINGLE ? 0 : table_bytes<Eval, R>(nt2)));
         tile_js = a.tile_jset + (size_t)lb * kTileCap;
-        const uint32_t M = a.tile_count[lb];
-        for (uint32_t m = threadIdx.x; m < M; m += blockDim.x)
-            s_pos[m] = a.pos_fixed[tile_js[m]];
+        tile_m = a.tile_count[lb];
+        for (uint32_t m = threadIdx.x; m < tile_m; m += blockDim.x)
+            s_pos[m] = fixed4(a.pos[tile_js[m]], a.box);
         if (threadIdx.x == 0)
-            s_pos[kTileDummy] = make_int4(0, 0, 0, 0);
+            s_pos[tile_m] = make_int4(0, 0, 0, 0);
         }
     if (!SINGLE || TILED)
         __syncthreads();
@@ -412,7 +420,7 @@ __global__ void __launch_bounds__(512) k_pair(const PairKernelArgs a)
         }
     else if constexpr (TILED)
         {
-        const int4 pi = a.pos_fixed[ie];
+        const int4 pi = fixed4(a.pos[ie], a.box);
         const uint32_t ti = (uint32_t)pi.w;
         const uint32_t col_off = ie * 4u;
         for (uint32_t k = 0; k < P; k += U)
@@ -441,7 +449,7 @@ __global__ void __launch_bounds__(512) k_pair(const PairKernelArgs a)
                 {
                 const uint32_t tp = SINGLE ? 0u : ti * nt + (uint32_t)pj[u].w;
                 term_mixed<Eval, VIRIAL, XPLOR, CUBIC>(a, SINGLE ? q0 : s_prep[tp], SINGLE ? x0 : s_x[tp],
-                                                       e[u] != kTileDummy, ie, e[u], tile_js,
+                                                       e[u] != tile_m, ie, e[u], tile_js,
                                                        fixed_delta(pi.x, pj[u].x), fixed_delta(pi.y, pj[u].y),
                                                        fixed_delta(pi.z, pj[u].z), part);
                 }
@@ -493,21 +501,17 @@ __global__ void k_pack_fixed(const double4* __restrict__ pos, uint32_t N, DevBox
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N)
         return;
-    const double4 p = pos[i];
-    out[i] = make_int4(to_fixed(p.x, box.Linv[0]),
-                       to_fixed(p.y, box.Linv[1]),
-                       to_fixed(p.z, box.Linv[2]),
-                       (int)p.w);
+    out[i] = fixed4(pos[i], box);
     }
 
 template<class Eval, bool MIXED, bool VIRIAL, bool SINGLE, bool XPLOR, bool CUBIC, bool TILED>
-static int launch_t(const PairKernelArgs& ka, unsigned block, hipStream_t stream)
+static int launch_t(const PairKernelArgs& ka, unsigned block, unsigned slots, hipStream_t stream)
     {
     using R = std::conditional_t<MIXED, float, double>;
     const size_t nt2 = (size_t)ka.ntypes * ka.ntypes;
     size_t shmem = SINGLE ? 0 : table_bytes<Eval, R>((uint32_t)nt2);
     if (TILED)
-        shmem += (size_t)kTileCap * sizeof(int4);
+        shmem += (size_t)slots * sizeof(int4);
     if (shmem > 64 * 1024)
         return set_error(AZP_ERR_UNSUPPORTED, "too many particle types (%u) for LDS staging", ka.ntypes);
     const unsigned grid = grid_for(ka.N, block);
@@ -518,49 +522,56 @@ static int launch_t(const PairKernelArgs& ka, unsigned block, hipStream_t stream
     }
 
 template<class Eval, bool MIXED, bool VIRIAL, bool SINGLE, bool TILED>
-static int launch_mode(const PairKernelArgs& ka, unsigned block, hipStream_t s)
+static int launch_mode(const PairKernelArgs& ka, unsigned block, unsigned slots, hipStream_t s)
     {
     const bool xplor = ka.shift_mode == AZP_SHIFT_XPLOR;
     if constexpr (MIXED)
         {
         const bool cubic = ka.fx_scale2[0] == ka.fx_scale2[1] && ka.fx_scale2[0] == ka.fx_scale2[2];
         if (xplor)
-            return cubic ? launch_t<Eval, true, VIRIAL, SINGLE, true, true, TILED>(ka, block, s)
-                         : launch_t<Eval, true, VIRIAL, SINGLE, true, false, TILED>(ka, block, s);
-        return cubic ? launch_t<Eval, true, VIRIAL, SINGLE, false, true, TILED>(ka, block, s)
-                     : launch_t<Eval, true, VIRIAL, SINGLE, false, false, TILED>(ka, block, s);
+            return cubic ? launch_t<Eval, true, VIRIAL, SINGLE, true, true, TILED>(ka, block, slots, s)
+                         : launch_t<Eval, true, VIRIAL, SINGLE, true, false, TILED>(ka, block, slots, s);
+        return cubic ? launch_t<Eval, true, VIRIAL, SINGLE, false, true, TILED>(ka, block, slots, s)
+                     : launch_t<Eval, true, VIRIAL, SINGLE, false, false, TILED>(ka, block, slots, s);
         }
     else
         {
-        return xplor ? launch_t<Eval, false, VIRIAL, SINGLE, true, false, false>(ka, block, s)
-                     : launch_t<Eval, false, VIRIAL, SINGLE, false, false, false>(ka, block, s);
+        return xplor ? launch_t<Eval, false, VIRIAL, SINGLE, true, false, false>(ka, block, slots, s)
+                     : launch_t<Eval, false, VIRIAL, SINGLE, false, false, false>(ka, block, slots, s);
         }
     }
 
 template<class Eval, bool MIXED, bool TILED>
-static int launch_virial(const PairKernelArgs& ka, bool virial, unsigned block, hipStream_t s)
+static int launch_virial(const PairKernelArgs& ka, bool virial, unsigned block, unsigned slots, hipStream_t s)
     {
     const bool single = ka.ntypes == 1;
     if (virial)
-        return single ? launch_mode<Eval, MIXED, true, true, TILED>(ka, block, s)
-                      : launch_mode<Eval, MIXED, true, false, TILED>(ka, block, s);
-    return single ? launch_mode<Eval, MIXED, false, true, TILED>(ka, block, s)
-                  : launch_mode<Eval, MIXED, false, false, TILED>(ka, block, s);
+        return single ? launch_mode<Eval, MIXED, true, true, TILED>(ka, block, slots, s)
+                      : launch_mode<Eval, MIXED, true, false, TILED>(ka, block, slots, s);
+    return single ? launch_mode<Eval, MIXED, false, true, TILED>(ka, block, slots, s)
+                  : launch_mode<Eval, MIXED, false, false, TILED>(ka, block, slots, s);
     }
 
 // The tiled list serves the mixed form when the tile tables fit next to the
 // type-pair tables in 64 KB of LDS; otherwise the plain list is used.
 template<class Eval, bool MIXED>
-static int launch_eval(const PairKernelArgs& ka, bool virial, unsigned block, unsigned tile_size, hipStream_t s)
+static int launch_eval(const PairKernelArgs& ka,
+                       bool virial,
+                       unsigned block,
+                       unsigned tile_size,
+                       unsigned tile_slots,
+                       hipStream_t s)
     {
     if constexpr (MIXED)
         {
         const size_t nt2 = (size_t)ka.ntypes * ka.ntypes;
         const size_t tables = ka.ntypes == 1 ? 0 : table_bytes<Eval, float>((uint32_t)nt2);
-        if (tile_size && ka.tile_list && tables + (size_t)kTileCap * sizeof(int4) <= 64 * 1024)
-            return launch_virial<Eval, true, true>(ka, virial, tile_size, s);
+        if (tile_size && ka.tile_list && tables + (size_t)tile_slots * sizeof(int4) <= 64 * 1024)
+            return launch_virial<Eval, true, true>(ka, virial, tile_size, tile_slots, s);
+        if (!ka.pos_fixed)
+            return set_error(AZP_ERR_INVALID, "mixed precision without a tiled list needs d_pos_fixed");
         }
-    return launch_virial<Eval, MIXED, false>(ka, virial, block, s);
+    return launch_virial<Eval, MIXED, false>(ka, virial, block, 0, s);
     }
 
 int pack_fixed(const double* d_pos, uint32_t N, const azp_box& box, int32_t* d_out, hipStream_t stream)
@@ -630,7 +641,7 @@ extern "C" int azp_gpu_compute_pair_forces(const azp_pair_args* args, int evalua
     if (!d_params)
         return set_error(AZP_ERR_INVALID, "null params");
     const bool mixed = args->precision == AZP_PREC_MIXED;
-    if (mixed && !p.d_pos_fixed)
+    if (mixed && !p.d_pos_fixed && !args->d_tile_list)
         return set_error(AZP_ERR_INVALID, "mixed precision needs d_pos_fixed (azp_gpu_pack_fixed)");
     if (args->shift_mode == AZP_SHIFT_XPLOR && !args->d_ronsq)
         return set_error(AZP_ERR_INVALID, "xplor needs d_ronsq");
@@ -660,14 +671,17 @@ extern "C" int azp_gpu_compute_pair_forces(const azp_pair_args* args, int evalua
     ka.tile_count = args->d_tile_count;
     ka.tile_jset = args->d_tile_jset;
     ka.tile_list = args->d_tile_list;
-    unsigned tile = 0;
+    unsigned tile = 0, slots = 0;
     if (args->d_tile_list)
         {
         if (!args->d_tile_count || !args->d_tile_jset)
             return set_error(AZP_ERR_INVALID, "a tiled list needs its counts, neighbor sets and entries");
         if (args->tile_size != 256 && args->tile_size != 512)
             return set_error(AZP_ERR_INVALID, "tile size %u must be 256 or 512", args->tile_size);
+        if (args->tile_slots == 0 || args->tile_slots > kTileCap)
+            return set_error(AZP_ERR_INVALID, "tile slots %u must be in [1, %u]", args->tile_slots, kTileCap);
         tile = args->tile_size;
+        slots = args->tile_slots;
         }
     // 512-lane workgroups: measured fastest for both forms on the 1M-particle
     // PerturbedLJ workload (block sweep 64..512)
@@ -680,24 +694,24 @@ extern "C" int azp_gpu_compute_pair_forces(const azp_pair_args* args, int evalua
     switch (evaluator)
         {
     case AZP_EVAL_HERTZ:
-        return mixed ? launch_eval<EvalHertz, true>(ka, vir, block, tile, s)
-                     : launch_eval<EvalHertz, false>(ka, vir, block, 0, s);
+        return mixed ? launch_eval<EvalHertz, true>(ka, vir, block, tile, slots, s)
+                     : launch_eval<EvalHertz, false>(ka, vir, block, 0, 0, s);
     case AZP_EVAL_EXPANDED_YUKAWA:
-        return mixed ? launch_eval<EvalExpandedYukawa, true>(ka, vir, block, tile, s)
-                     : launch_eval<EvalExpandedYukawa, false>(ka, vir, block, 0, s);
+        return mixed ? launch_eval<EvalExpandedYukawa, true>(ka, vir, block, tile, slots, s)
+                     : launch_eval<EvalExpandedYukawa, false>(ka, vir, block, 0, 0, s);
     case AZP_EVAL_PERTURBED_LJ:
-        return mixed ? launch_eval<EvalPerturbedLJ, true>(ka, vir, block, tile, s)
-                     : launch_eval<EvalPerturbedLJ, false>(ka, vir, block, 0, s);
+        return mixed ? launch_eval<EvalPerturbedLJ, true>(ka, vir, block, tile, slots, s)
+                     : launch_eval<EvalPerturbedLJ, false>(ka, vir, block, 0, 0, s);
     case AZP_EVAL_COLLOID:
         if (mixed)
             return set_error(AZP_ERR_UNSUPPORTED,
                              "Colloid runs in the plain FP64 form only (its colloid-colloid "
                              "branch cancels catastrophically in FP32)");
-        return launch_eval<EvalColloid, false>(ka, vir, block, 0, s);
+        return launch_eval<EvalColloid, false>(ka, vir, block, 0, 0, s);
     case AZP_EVAL_DPD_GENERAL_WEIGHT:
         // PotentialPairConservative<DPDPairEvaluatorGeneralWeight>
-        return mixed ? launch_eval<EvalDPDConservative, true>(ka, vir, block, tile, s)
-                     : launch_eval<EvalDPDConservative, false>(ka, vir, block, 0, s);
+        return mixed ? launch_eval<EvalDPDConservative, true>(ka, vir, block, tile, slots, s)
+                     : launch_eval<EvalDPDConservative, false>(ka, vir, block, 0, 0, s);
     default:
         return set_error(AZP_ERR_INVALID, "evaluator %d is not an isotropic pair evaluator", evaluator);
         }

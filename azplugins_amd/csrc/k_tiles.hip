@@ -94,6 +94,8 @@ __global__ void __launch_bounds__(512) k_tile_build(uint32_t N,
             atomicMax(overflow, M);
         return; // the whole workgroup leaves (M is the same for every lane)
         }
+    if (tid == 0)
+        atomicMax(overflow + 1, M);
 
     for (uint32_t k = tid; k < kHashSlots; k += T)
         {
@@ -143,16 +145,16 @@ __global__ void __launch_bounds__(512) k_tile_build(uint32_t N,
         return;
     for (uint32_t k = 0; k < P; k += 2)
         {
-        const uint32_t e0 = k < nn ? rank[tile_find(keys, nlist[(size_t)k * pitch + i])] : kTileDummy;
-        const uint32_t e1 = k + 1 < nn ? rank[tile_find(keys, nlist[(size_t)(k + 1) * pitch + i])] : kTileDummy;
+        const uint32_t e0 = k < nn ? rank[tile_find(keys, nlist[(size_t)k * pitch + i])] : M;
+        const uint32_t e1 = k + 1 < nn ? rank[tile_find(keys, nlist[(size_t)(k + 1) * pitch + i])] : M;
         tlist[(size_t)(k >> 1) * pitch + i] = e0 | (e1 << 16);
         }
     }
 
 __global__ void k_tile_flag_clear(uint32_t* flag)
     {
-    if (blockIdx.x == 0 && threadIdx.x == 0)
-        flag[0] = 0u;
+    if (blockIdx.x == 0 && threadIdx.x < 2)
+        flag[threadIdx.x] = 0u;
     }
 
 int tiles_build(TileList& t,
@@ -177,17 +179,18 @@ int tiles_build(TileList& t,
     AZP_TRY(t.count.reserve(ntiles));
     AZP_TRY(t.jset.reserve((size_t)ntiles * kTileCap));
     AZP_TRY(t.list.reserve((size_t)words * pitch));
-    AZP_TRY(t.flag.reserve(1));
+    AZP_TRY(t.flag.reserve(2));
     hipLaunchKernelGGL(k_tile_flag_clear, dim3(1), dim3(64), 0, s, t.flag.ptr);
     hipLaunchKernelGGL(k_tile_build, dim3(ntiles), dim3(tile_size), 0, s, N, d_n_neigh, d_nlist, pitch, t.count.ptr,
                        t.jset.ptr, t.list.ptr, t.flag.ptr);
     AZP_LAUNCH_CHECK();
-    AZP_HIP(hipMemcpyAsync(h_flag, t.flag.ptr, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    AZP_HIP(hipMemcpyAsync(h_flag, t.flag.ptr, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     AZP_HIP(hipStreamSynchronize(s));
     if (h_flag[0] != 0u)
         return AZP_OK; // some tile has too many distinct neighbors
     t.ntiles = ntiles;
     t.size = tile_size;
+    t.slots = h_flag[1] + 1u;
     t.valid = true;
     return AZP_OK;
     }

@@ -47,6 +47,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tps", action="store_true")
+    ap.add_argument("--tile", type=int, default=1, choices=(0, 1, 256, 512),
+                    help="tiled list for the mixed form: 0 off, 1 auto (512 else 256), or a fixed size")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the DPD, adjoint, batched-replica and Hertz workloads")
     ap.add_argument("--tps-steps", type=int, default=200)
@@ -97,13 +99,13 @@ def main():
     torch.cuda.synchronize()
     t_nlist = time.perf_counter() - t0
     # the mixed form reads the tiled list (16-bit entries, tile neighbors in LDS)
-    nl.set_tiled(True)
+    nl.set_tiled(args.tile)
     t0 = time.perf_counter()
     nl.update(parts, force_rebuild=True)
     torch.cuda.synchronize()
     t_nlist_tiled = time.perf_counter() - t0
     tiles = nl.get_tiles()
-    log(f"bench: tile size {tiles[3]} (0 = plain list)")
+    log(f"bench: tile size {tiles[3]} (0 = plain list), LDS slots {tiles[4]}")
     d_nn, d_list, pitch, mx = nl.get()
     nn_host, _ = nl.download(N)
     nlist_entries = int(nn_host.sum())
@@ -139,11 +141,14 @@ def main():
         return a
 
     args_mixed = make_args(1)
-    args_mixed.d_tile_count, args_mixed.d_tile_jset, args_mixed.d_tile_list, args_mixed.tile_size = tiles
+    args_mixed.d_tile_count, args_mixed.d_tile_jset, args_mixed.d_tile_list, args_mixed.tile_size, \
+        args_mixed.tile_slots = tiles
     args_fp64 = make_args(0)
 
     def step_mixed():
-        _azp.call("azp_gpu_pack_fixed", _azp.dptr(pos), N, ctypes.byref(box), _azp.dptr(pos_fixed), stream)
+        # over the tiled list the pair kernel converts positions to fixed point itself
+        if not tiles[3]:
+            _azp.call("azp_gpu_pack_fixed", _azp.dptr(pos), N, ctypes.byref(box), _azp.dptr(pos_fixed), stream)
         _azp.call("azp_gpu_compute_pair_forces", ctypes.byref(args_mixed), 2, _azp.dptr(d_params))
 
     def step_fp64():
@@ -260,6 +265,7 @@ def main():
         "nlist_build_ms": t_nlist * 1e3,
         "nlist_build_with_tiles_ms": t_nlist_tiled * 1e3,
         "tile_size": tiles[3],
+        "tile_slots": tiles[4],
     }
     if not args.no_tps:
         out["tps"] = run_tps(c, args.tps_steps)

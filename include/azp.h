@@ -185,11 +185,13 @@ typedef struct
     double kT;
     void* stream;
     /* optional tiled list for the mixed form (azp_nlist_get_tiles); NULL
-     * d_tile_list = use d_nlist only */
+     * d_tile_list = use d_nlist (and d_pos_fixed) only.  The tiled kernel
+     * converts positions to fixed point itself (d_pos_fixed is not read). */
     const uint32_t* d_tile_count;
     const uint32_t* d_tile_jset;
     const uint32_t* d_tile_list;
     uint32_t tile_size;
+    uint32_t tile_slots;
     } azp_pair_args;
 
 /* ---- library ---- */
@@ -329,14 +331,17 @@ int azp_nlist_num_builds(const azp_nlist* nl, uint64_t* n);
  * library's own addition; no reference counterpart): the full list of each
  * block of 512 (or 256) consecutive particles rewritten as 16-bit indices
  * into the block's sorted set of distinct neighbors.  Built with every
- * rebuild once enabled; tile_size = 0 when some block has more than 3967
- * distinct neighbors (the plain list is then used). */
-int azp_nlist_set_tiled(azp_nlist* nl, int enable);
+ * rebuild once enabled.  mode: 0 off, 1 blocks of 512 else 256, 256 or 512
+ * that size only.  tile_size = 0 when some block has more than 3967
+ * distinct neighbors (the plain list is then used); tile_slots = LDS slots
+ * the pair kernel stages (largest distinct count + 1). */
+int azp_nlist_set_tiled(azp_nlist* nl, int mode);
 int azp_nlist_get_tiles(const azp_nlist* nl,
                         const uint32_t** d_tile_count,
                         const uint32_t** d_tile_jset,
                         const uint32_t** d_tile_list,
-                        uint32_t* tile_size);
+                        uint32_t* tile_size,
+                        uint32_t* tile_slots);
 /* Copy n_neigh[N] and the list (max_neighbors * pitch entries) to host memory
  * (synchronizes the device). */
 int azp_nlist_download(const azp_nlist* nl,

@@ -89,7 +89,7 @@ def gpu_pair(cls_name, system, ntypes, params, r_cut, r_buff=0.4, mode="none", p
     nl = _azplugins.NeighborListGPU(r_buff)
     nl.set_r_cut(ntypes, pp.getRCutMatrix())
     if tiled:
-        nl.set_tiled(True)
+        nl.set_tiled(tiled)
     parts = system.particles()
     nl.update(parts)
     N = system.N

@@ -21,10 +21,24 @@ pytestmark = pytest.mark.gpu
 PLJ = {(0, 0): {"epsilon": 1.0, "sigma": 1.0, "attraction_scale_factor": 0.5}}
 
 
-def _both(cls, sysm, nt, params, rc, **kw):
+def _both(cls, sysm, nt, params, rc, tiled=True, **kw):
     plain = gpu_pair(cls, sysm, nt, params, rc, precision="mixed", **kw)
-    tiled = gpu_pair(cls, sysm, nt, params, rc, precision="mixed", tiled=True, **kw)
+    tiled = gpu_pair(cls, sysm, nt, params, rc, precision="mixed", tiled=tiled, **kw)
     return plain, tiled
+
+
+@pytest.mark.parametrize("mode", [256, 512])
+def test_tiled_sizes_equal_plain(mode):
+    L = 20.0
+    pos4, _ = jittered_lattice(20, L, 0.1, 1, seed=3)
+    Lv = np.array([L, L, L])
+    sysm = GpuSystem(pos4, Lv)
+    plain, tiled = _both("PotentialPairPerturbedLennardJones", sysm, 1, PLJ, 3.0, tiled=mode)
+    size, slots = tiled.nl.get_tiles()[3:]
+    say(f"[tiles] forced tile size {mode}: got {size}, LDS slots {slots}")
+    assert size == mode and 0 < slots <= 3968
+    assert np.array_equal(plain.force, tiled.force)
+    assert np.array_equal(plain.virial, tiled.virial)
 
 
 @pytest.mark.parametrize("n,L", [(7, 7.3), (10, 10.4), (18, 18.9), (24, 25.3)])
@@ -34,8 +48,8 @@ def test_tiled_equals_plain_plj(n, L):
     Lv = np.array([L, L, L])
     sysm = GpuSystem(pos4, Lv)
     plain, tiled = _both("PotentialPairPerturbedLennardJones", sysm, 1, PLJ, 3.0)
-    _, _, _, size = tiled.nl.get_tiles()
-    say(f"[tiles] PLJ N={sysm.N}: tile size {size}")
+    _, _, _, size, slots = tiled.nl.get_tiles()
+    say(f"[tiles] PLJ N={sysm.N}: tile size {size}, LDS slots {slots}")
     assert size in (256, 512)
     assert np.array_equal(plain.force, tiled.force)
     assert np.array_equal(plain.virial, tiled.virial)
@@ -102,10 +116,15 @@ def test_tiled_launcher_entry_point():
     _azp.call("azp_gpu_pack_fixed", _azp.dptr(sysm.pos), N, ctypes.byref(box), _azp.dptr(pf), _azp.stream_ptr())
     a = raw_args(sysm, ref.nl, rcsq, None, 1, force, vir, pitch, precision=1)
     a.particles.d_pos_fixed = pf.data_ptr()
-    a.d_tile_count, a.d_tile_jset, a.d_tile_list, a.tile_size = ref.nl.get_tiles()
+    a.d_tile_count, a.d_tile_jset, a.d_tile_list, a.tile_size, a.tile_slots = ref.nl.get_tiles()
+    a.particles.d_pos_fixed = None  # the tiled kernel converts positions itself
     _azp.call("azp_gpu_compute_pair_forces", ctypes.byref(a), 2, _azp.dptr(d_params))
     torch.cuda.synchronize()
     assert np.array_equal(force.cpu().numpy(), ref.force)
     a.tile_size = 128
+    with pytest.raises(_azp.AzpError):
+        _azp.call("azp_gpu_compute_pair_forces", ctypes.byref(a), 2, _azp.dptr(d_params))
+    a.tile_size = 512
+    a.tile_slots = 5000
     with pytest.raises(_azp.AzpError):
         _azp.call("azp_gpu_compute_pair_forces", ctypes.byref(a), 2, _azp.dptr(d_params))
