@@ -165,16 +165,19 @@ def main():
         return (time.perf_counter() - t0) / steps
 
     def time_kernel(precision, reps):
-        """Average duration of the pair kernel alone, HIP events on its stream."""
+        """Average duration of the pair kernel alone: HIP events on its stream around
+        `reps` back-to-back launches (the sustained rate, as in a run of steps)."""
         a = args_mixed if precision == 1 else args_fp64
-        starts = [torch.cuda.Event(enable_timing=True) for _ in range(reps)]
-        ends = [torch.cuda.Event(enable_timing=True) for _ in range(reps)]
-        for k in range(reps):
-            starts[k].record()
+        for _ in range(3):
             _azp.call("azp_gpu_compute_pair_forces", ctypes.byref(a), 2, _azp.dptr(d_params))
-            ends[k].record()
+        start = torch.cuda.Event(enable_timing=True)
+        end = torch.cuda.Event(enable_timing=True)
+        start.record()
+        for _ in range(reps):
+            _azp.call("azp_gpu_compute_pair_forces", ctypes.byref(a), 2, _azp.dptr(d_params))
+        end.record()
         torch.cuda.synchronize()
-        return float(np.mean([s.elapsed_time(e) for s, e in zip(starts, ends)])) / 1e3
+        return start.elapsed_time(end) / 1e3 / reps
 
     # reference results (FP64 form) for the error of the mixed form
     step_fp64()
