@@ -9,10 +9,12 @@
 //   3. scatter: members written into per-cell ranges, then each cell's members
 //               sorted by index so the list order is deterministic
 //   4. walk:    one lane per particle scans its 27 neighbor cells (duplicates
-//               removed for grids narrower than 3 cells) and appends j with
-//               rsq <= r_list(ti,tj)^2 — the same FP64 predicate, evaluated
-//               without contraction, as the host oracle, so the pair sets
-//               agree exactly.  Entry k of particle i goes to
+//               removed for grids narrower than 3 cells) over a cell-ordered
+//               fixed-point copy of the positions and appends j with
+//               rsq <= r_list(ti,tj)^2 — FP32 on exact integer displacements,
+//               with every candidate near r_list decided by the same FP64
+//               predicate, evaluated without contraction, as the host oracle,
+//               so the pair sets agree exactly.  Entry k of particle i goes to
 //               nlist[k * pitch + i] (coalesced for one-lane-per-particle
 //               consumers).  Lanes that find more than max_neighbors entries
 //               keep counting and report the largest count; the host grows
@@ -87,16 +89,46 @@ __global__ void k_sort_cells(uint32_t ncell, const uint32_t* __restrict__ start,
         }
     }
 
+// Cell-ordered copy of the positions for the walk: slot m (the m-th member in
+// cell order) holds the member's position as wrapped 32-bit fixed point and,
+// in .w, its checked type (ntypes for an invalid type or a non-finite
+// position, which is then never listed); rank_of[j] = m.
+__global__ void k_cell_fixed(uint32_t N,
+                             const uint32_t* __restrict__ members,
+                             const double4* __restrict__ pos,
+                             DevBox box,
+                             uint32_t ntypes,
+                             int4* __restrict__ cell_fixed,
+                             uint32_t* __restrict__ rank_of)
+    {
+    const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= N)
+        return;
+    const uint32_t j = members[m];
+    const double4 p = pos[j];
+    const bool finite = isfinite(p.x) && isfinite(p.y) && isfinite(p.z);
+    const uint32_t t = finite ? checked_type(p.w, ntypes) : ntypes;
+    cell_fixed[m] = finite ? make_int4(to_fixed(p.x, box.Linv[0]), to_fixed(p.y, box.Linv[1]),
+                                       to_fixed(p.z, box.Linv[2]), (int)t)
+                           : make_int4(0, 0, 0, (int)t);
+    rank_of[j] = m;
+    }
+
 struct WalkArgs
     {
     const double4* pos;
     uint32_t N;
     DevBox box;
+    float fx_scale[3]; // L / 2^32 per axis
     CellGrid g;
     const uint32_t* cell_of;
     const uint32_t* start;
     const uint32_t* members;
+    const int4* cell_fixed;
+    const uint32_t* rank_of;
     const double* r_listsq;
+    const float* r_listsq_lo; // r_listsq (1 - 1e-5) in FP32
+    const float* r_listsq_hi; // r_listsq (1 + 1e-5) in FP32
     uint32_t ntypes;
     uint32_t* n_neigh;
     uint32_t* nlist;
@@ -105,8 +137,14 @@ struct WalkArgs
     uint32_t* overflow; // [0] largest count seen when > max_neighbors, [2] bad type seen
     };
 
-// Lists particle i's neighbors and returns how many it has.
-__device__ uint32_t walk_one(const WalkArgs& a, uint32_t i)
+// Lists particle i's neighbors and returns how many it has.  Candidates are
+// read from the cell-ordered fixed-point copy (16 bytes, contiguous per
+// cell) and tested in FP32 on the exact integer displacement; a candidate
+// whose FP32 r^2 lies within a relative 1e-5 of r_list^2 is decided by the
+// FP64 predicate of the host oracle (minimum image and r^2 without
+// contraction, rsq <= r_list^2), so the pair set is the FP64 one.  Outside
+// the band the FP32 value is within ~1e-6 relative of the FP64 one.
+template<bool SINGLE> __device__ uint32_t walk_one(const WalkArgs& a, uint32_t i)
     {
     const double4 pi = a.pos[i];
     const uint32_t ti = checked_type(pi.w, a.ntypes);
@@ -117,6 +155,10 @@ __device__ uint32_t walk_one(const WalkArgs& a, uint32_t i)
         atomicOr(a.overflow + 2, finite ? 1u : 2u);
         return 0;
         }
+    const int32_t fxi = to_fixed(pi.x, a.box.Linv[0]);
+    const int32_t fyi = to_fixed(pi.y, a.box.Linv[1]);
+    const int32_t fzi = to_fixed(pi.z, a.box.Linv[2]);
+    const uint32_t self = a.rank_of[i];
     const uint32_t c = a.cell_of[i];
     const int cx = (int)(c % (uint32_t)a.g.n[0]);
     const int cy = (int)((c / (uint32_t)a.g.n[0]) % (uint32_t)a.g.n[1]);
@@ -126,6 +168,8 @@ __device__ uint32_t walk_one(const WalkArgs& a, uint32_t i)
     const int nx = stencil_axis(cx, a.g.n[0], ox);
     const int ny = stencil_axis(cy, a.g.n[1], oy);
     const int nz = stencil_axis(cz, a.g.n[2], oz);
+    const double rl0 = a.r_listsq[0];
+    const float lo0 = a.r_listsq_lo[0], hi0 = a.r_listsq_hi[0];
     uint32_t count = 0;
     for (int iz = 0; iz < nz; ++iz)
         for (int iy = 0; iy < ny; ++iy)
@@ -133,28 +177,46 @@ __device__ uint32_t walk_one(const WalkArgs& a, uint32_t i)
                 {
                 const uint32_t nb = (uint32_t)((oz[iz] * a.g.n[1] + oy[iy]) * a.g.n[0] + ox[ix]);
                 const uint32_t b = a.start[nb], e = a.start[nb + 1];
-                for (uint32_t m = b; m < e; ++m)
+                // candidates 4 at a time: their loads are issued together
+                for (uint32_t m0 = b; m0 < e; m0 += 4)
                     {
-                    const uint32_t j = a.members[m];
-                    if (j == i)
+                    int4 pc[4];
+#pragma unroll
+                    for (uint32_t u = 0; u < 4; ++u)
+                        pc[u] = a.cell_fixed[min(m0 + u, e - 1)];
+#pragma unroll
+                    for (uint32_t u = 0; u < 4; ++u)
+                    {
+                    const uint32_t m = m0 + u;
+                    const int4 pj = pc[u];
+                    const uint32_t tj = (uint32_t)pj.w;
+                    if (m >= e || m == self || tj >= a.ntypes)
                         continue;
-                    const double4 pj = a.pos[j];
-                    const uint32_t tj = checked_type(pj.w, a.ntypes);
-                    if (tj == a.ntypes)
-                        continue;
-                    const double rl = a.r_listsq[ti * a.ntypes + tj];
-                    if (!(rl > 0.0))
-                        continue;
-                    const double dx = min_image_1d(pi.x - pj.x, a.box.L[0], a.box.Linv[0]);
-                    const double dy = min_image_1d(pi.y - pj.y, a.box.L[1], a.box.Linv[1]);
-                    const double dz = min_image_1d(pi.z - pj.z, a.box.L[2], a.box.Linv[2]);
-                    const double rsq = rsq3_nofma(dx, dy, dz);
-                    if (rsq <= rl)
+                    const uint32_t tp = SINGLE ? 0u : ti * a.ntypes + tj;
+                    const float lo = SINGLE ? lo0 : a.r_listsq_lo[tp];
+                    const float hi = SINGLE ? hi0 : a.r_listsq_hi[tp];
+                    const float dx = (float)fixed_delta(fxi, pj.x) * a.fx_scale[0];
+                    const float dy = (float)fixed_delta(fyi, pj.y) * a.fx_scale[1];
+                    const float dz = (float)fixed_delta(fzi, pj.z) * a.fx_scale[2];
+                    const float rsq = dx * dx + dy * dy + dz * dz;
+                    bool in = rsq < lo;
+                    if (!in && rsq <= hi)
+                        {
+                        const uint32_t j = a.members[m];
+                        const double4 p64 = a.pos[j];
+                        const double rl = SINGLE ? rl0 : a.r_listsq[tp];
+                        const double ex = min_image_1d(pi.x - p64.x, a.box.L[0], a.box.Linv[0]);
+                        const double ey = min_image_1d(pi.y - p64.y, a.box.L[1], a.box.Linv[1]);
+                        const double ez = min_image_1d(pi.z - p64.z, a.box.L[2], a.box.Linv[2]);
+                        in = rsq3_nofma(ex, ey, ez) <= rl;
+                        }
+                    if (in)
                         {
                         if (count < a.max_neighbors)
-                            a.nlist[(size_t)count * a.pitch + i] = j;
+                            a.nlist[(size_t)count * a.pitch + i] = a.members[m];
                         ++count;
                         }
+                    }
                     }
                 }
     return count;
@@ -164,11 +226,11 @@ __device__ uint32_t walk_one(const WalkArgs& a, uint32_t i)
 // columns with their own index up to the wave's longest list rounded up to a
 // multiple of 8, so consumers can run a wave-uniform loop over batches of 8
 // entries with no per-entry bounds test (padding entries have j == i).
-__global__ void __launch_bounds__(256) k_walk(const WalkArgs a)
+template<bool SINGLE> __global__ void __launch_bounds__(256) k_walk(const WalkArgs a)
     {
     const uint32_t i = xcd_logical_block() * blockDim.x + threadIdx.x;
     const bool active = i < a.N;
-    const uint32_t count = active ? walk_one(a, i) : 0u;
+    const uint32_t count = active ? walk_one<SINGLE>(a, i) : 0u;
     if (active)
         {
         // a list longer than the capacity is truncated (and reported), so
@@ -230,6 +292,9 @@ struct azp_nlist
     bool params_dirty = true;
 
     DeviceArray<double> r_listsq;
+    DeviceArray<float> r_listsq_lo, r_listsq_hi;
+    DeviceArray<int4> cell_fixed;
+    DeviceArray<uint32_t> rank_of;
     DeviceArray<uint32_t> n_neigh;
     DeviceArray<uint32_t> list;
     DeviceArray<double4> last_pos;
@@ -279,6 +344,8 @@ static int nlist_build(azp_nlist* nl, const azp_particles* p, hipStream_t s)
     AZP_TRY(nl->start.reserve(ncell + 1));
     AZP_TRY(nl->scan_sums.reserve(scan_chunks((uint32_t)ncell)));
     AZP_TRY(nl->members.reserve(N));
+    AZP_TRY(nl->cell_fixed.reserve(N));
+    AZP_TRY(nl->rank_of.reserve(N));
     AZP_TRY(nl->n_neigh.reserve(N));
     AZP_TRY(nl->last_pos.reserve(N));
     AZP_TRY(nl->flags.reserve(3));
@@ -293,6 +360,8 @@ static int nlist_build(azp_nlist* nl, const azp_particles* p, hipStream_t s)
                        nl->slot_of.ptr, nl->start.ptr, nl->members.ptr);
     hipLaunchKernelGGL(k_sort_cells, dim3(grid_for(ncell, B)), dim3(B), 0, s, (uint32_t)ncell,
                        nl->start.ptr, nl->members.ptr);
+    hipLaunchKernelGGL(k_cell_fixed, dim3(grid_for(N, B)), dim3(B), 0, s, N, nl->members.ptr, pos, box, nl->ntypes,
+                       nl->cell_fixed.ptr, nl->rank_of.ptr);
     AZP_LAUNCH_CHECK();
 
     // initial capacity from the density
@@ -315,18 +384,27 @@ static int nlist_build(azp_nlist* nl, const azp_particles* p, hipStream_t s)
         wa.pos = pos;
         wa.N = N;
         wa.box = box;
+        for (int k = 0; k < 3; ++k)
+            wa.fx_scale[k] = (float)(box.L[k] / 4294967296.0);
         wa.g = g;
         wa.cell_of = nl->cell_of.ptr;
         wa.start = nl->start.ptr;
         wa.members = nl->members.ptr;
+        wa.cell_fixed = nl->cell_fixed.ptr;
+        wa.rank_of = nl->rank_of.ptr;
         wa.r_listsq = nl->r_listsq.ptr;
+        wa.r_listsq_lo = nl->r_listsq_lo.ptr;
+        wa.r_listsq_hi = nl->r_listsq_hi.ptr;
         wa.ntypes = nl->ntypes;
         wa.n_neigh = nl->n_neigh.ptr;
         wa.nlist = nl->list.ptr;
         wa.pitch = nl->pitch;
         wa.max_neighbors = nl->max_neighbors;
         wa.overflow = nl->flags.ptr;
-        hipLaunchKernelGGL(k_walk, dim3(grid_for(N, B)), dim3(B), 0, s, wa);
+        if (nl->ntypes == 1)
+            hipLaunchKernelGGL(k_walk<true>, dim3(grid_for(N, B)), dim3(B), 0, s, wa);
+        else
+            hipLaunchKernelGGL(k_walk<false>, dim3(grid_for(N, B)), dim3(B), 0, s, wa);
         AZP_LAUNCH_CHECK();
         AZP_HIP(hipMemcpyAsync(nl->h_flags, nl->flags.ptr, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         AZP_HIP(hipStreamSynchronize(s));
@@ -444,8 +522,21 @@ extern "C" int azp_nlist_update(azp_nlist* nl, const azp_particles* p, int force
             const double r = nl->r_cut[k] + nl->r_buff;
             rl[k] = nl->r_cut[k] > 0.0 ? r * r : 0.0;
             }
+        std::vector<float> lo(nt2), hi(nt2);
+        for (size_t k = 0; k < nt2; ++k)
+            {
+            // excluded type pairs (r_cut <= 0) can never pass either test
+            lo[k] = rl[k] > 0.0 ? (float)(rl[k] * (1.0 - 1e-5)) : -1.0f;
+            hi[k] = rl[k] > 0.0 ? (float)(rl[k] * (1.0 + 1e-5)) : -1.0f;
+            }
         AZP_TRY(nl->r_listsq.reserve(nt2));
+        AZP_TRY(nl->r_listsq_lo.reserve(nt2));
+        AZP_TRY(nl->r_listsq_hi.reserve(nt2));
         AZP_HIP(hipMemcpyAsync(nl->r_listsq.ptr, rl.data(), nt2 * sizeof(double), hipMemcpyHostToDevice, s));
+        AZP_HIP(hipMemcpyAsync(nl->r_listsq_lo.ptr, lo.data(), nt2 * sizeof(float), hipMemcpyHostToDevice, s));
+        AZP_HIP(hipMemcpyAsync(nl->r_listsq_hi.ptr, hi.data(), nt2 * sizeof(float), hipMemcpyHostToDevice, s));
+        // the host vectors go out of scope: wait for the copies
+        AZP_HIP(hipStreamSynchronize(s));
         }
     bool need = force_rebuild || !nl->have_list || nl->params_dirty || p->N != nl->built_N
                 || p->box.Lx != nl->built_box.Lx || p->box.Ly != nl->built_box.Ly

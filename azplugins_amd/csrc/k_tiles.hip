@@ -17,6 +17,7 @@ namespace azp
 constexpr uint32_t kHashSlots = 7680;   // load factor <= 0.6 at capacity
 constexpr uint32_t kSortSlots = 4096;   // >= kTileCap, power of two
 constexpr uint32_t kEmpty = 0xffffffffu;
+constexpr uint32_t kBatch = 8;          // list words loaded per batch (the padding granule)
 
 __device__ __forceinline__ uint32_t tile_hash(uint32_t j)
     {
@@ -80,11 +81,19 @@ __global__ void __launch_bounds__(512) k_tile_build(uint32_t N,
     const uint32_t i = t * T + tid;
     const uint32_t nn = i < N ? n_neigh[i] : 0u;
     volatile uint32_t* vcount = &s_count;
-    for (uint32_t k = 0; k < nn; ++k)
+    // list words are loaded 8 at a time (independent loads in flight), then inserted
+    for (uint32_t k0 = 0; k0 < nn; k0 += kBatch)
         {
         if (*vcount >= kTileCap)
             break;
-        tile_insert(keys, &s_count, nlist[(size_t)k * pitch + i]);
+        uint32_t jj[kBatch];
+#pragma unroll
+        for (uint32_t u = 0; u < kBatch; ++u)
+            jj[u] = k0 + u < nn ? nlist[(size_t)(k0 + u) * pitch + i] : kEmpty;
+#pragma unroll
+        for (uint32_t u = 0; u < kBatch; ++u)
+            if (jj[u] != kEmpty)
+                tile_insert(keys, &s_count, jj[u]);
         }
     __syncthreads();
     const uint32_t M = s_count;
@@ -143,11 +152,19 @@ __global__ void __launch_bounds__(512) k_tile_build(uint32_t N,
     const uint32_t P = (wmax + 7u) & ~7u;
     if (i >= N)
         return;
-    for (uint32_t k = 0; k < P; k += 2)
+    for (uint32_t k0 = 0; k0 < P; k0 += kBatch)
         {
-        const uint32_t e0 = k < nn ? rank[tile_find(keys, nlist[(size_t)k * pitch + i])] : M;
-        const uint32_t e1 = k + 1 < nn ? rank[tile_find(keys, nlist[(size_t)(k + 1) * pitch + i])] : M;
-        tlist[(size_t)(k >> 1) * pitch + i] = e0 | (e1 << 16);
+        uint32_t jj[kBatch];
+#pragma unroll
+        for (uint32_t u = 0; u < kBatch; ++u)
+            jj[u] = k0 + u < nn ? nlist[(size_t)(k0 + u) * pitch + i] : kEmpty;
+        uint32_t e[kBatch];
+#pragma unroll
+        for (uint32_t u = 0; u < kBatch; ++u)
+            e[u] = jj[u] != kEmpty ? (uint32_t)rank[tile_find(keys, jj[u])] : M;
+#pragma unroll
+        for (uint32_t w = 0; w < kBatch / 2; ++w)
+            tlist[(size_t)((k0 >> 1) + w) * pitch + i] = e[2 * w] | (e[2 * w + 1] << 16);
         }
     }
 
