@@ -52,11 +52,14 @@ __global__ void __launch_bounds__(512) k_aniso(const AnisoKernelArgs a)
             s_q[t] = tpm_prepare<R>(a.params[t], a.rcutsq[t], a.shift_mode == AZP_SHIFT_SHIFT);
         __syncthreads();
         }
-    const uint32_t i = xcd_logical_block() * blockDim.x + threadIdx.x;
-    if (i >= a.N)
+    const uint32_t lb = xcd_logical_block();
+    const uint32_t i = lb * blockDim.x + threadIdx.x;
+    // wavefronts entirely past N leave together; lanes past N in the last
+    // wavefront walk no entries and store nothing
+    if (lb * blockDim.x + (threadIdx.x & ~63u) >= a.N)
         return;
-
-    const AnisoSelf<R> me = aniso_self<MIXED>(a.pos, a.pos_fixed, a.orientation, i);
+    const uint32_t ie = min(i, a.N - 1);
+    const AnisoSelf<R> me = aniso_self<MIXED>(a.pos, a.pos_fixed, a.orientation, ie);
     LaneBox box;
     for (int k = 0; k < 3; ++k)
         {
@@ -65,11 +68,12 @@ __global__ void __launch_bounds__(512) k_aniso(const AnisoKernelArgs a)
         box.fx_scale[k] = a.fx_scale[k];
         }
     AnisoAcc acc;
-    const uint32_t nn = a.n_neigh[i];
-    for (uint32_t k = 0; k < nn; ++k)
-        tpm_entry<MIXED, VIRIAL, SINGLE>(a.nlist[(size_t)k * a.nlist_pitch + i], a.pos, a.pos_fixed,
-                                         a.orientation, me, box, nt, q0, s_q, acc);
-    aniso_store<VIRIAL>(a.force, a.torque, a.virial, a.vpitch, i, acc);
+    const uint32_t nn = i < a.N ? a.n_neigh[i] : 0u;
+    const uint32_t trips = __builtin_amdgcn_readfirstlane((tpm_wave_max(nn) + 3u) & ~3u);
+    tpm_list<MIXED, VIRIAL, SINGLE, false>(ie, nn, trips, a.nlist, a.nlist_pitch, a.pos, a.pos_fixed, a.orientation,
+                                           nullptr, me, box, nt, q0, s_q, acc);
+    if (i < a.N)
+        aniso_store<VIRIAL>(a.force, a.torque, a.virial, a.vpitch, i, acc);
     }
 
 template<bool MIXED, bool VIRIAL, bool SINGLE>

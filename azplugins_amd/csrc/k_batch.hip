@@ -315,11 +315,16 @@ __global__ void k_b_check(const double4* __restrict__ pos,
         }
     }
 
+// Mixed form: positions to fixed point in each replica's box, and the
+// FP32 director of every particle (read by its neighbors instead of the
+// quaternion).
 __global__ void k_b_pack_fixed(const double4* __restrict__ pos,
+                               const double4* __restrict__ orientation,
                                uint32_t N,
                                const uint32_t* __restrict__ rep_of,
                                const RepInfo* __restrict__ reps,
-                               int4* __restrict__ out)
+                               int4* __restrict__ out,
+                               float4* __restrict__ dirs)
     {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N)
@@ -327,6 +332,9 @@ __global__ void k_b_pack_fixed(const double4* __restrict__ pos,
     const double4 p = pos[i];
     const RepInfo& R = reps[rep_of[i]];
     out[i] = make_int4(to_fixed(p.x, R.Linv[0]), to_fixed(p.y, R.Linv[1]), to_fixed(p.z, R.Linv[2]), (int)p.w);
+    float n[3];
+    director_of<float>(orientation[i], n);
+    dirs[i] = make_float4(n[0], n[1], n[2], 0.0f);
     }
 
 struct BatchForceArgs
@@ -344,6 +352,7 @@ struct BatchForceArgs
     const double* rcutsq;
     uint32_t ntypes;
     int shift_mode;
+    const float4* dirs; // mixed form: FP32 directors
     double4* force;
     double4* torque;
     double* virial;
@@ -387,9 +396,9 @@ __global__ void __launch_bounds__(512) k_b_forces(const BatchForceArgs a)
         for (uint32_t c = c0; c < c1; ++c)
             {
             const uint32_t i = c * 64u + lane;
-            if (i >= a.N)
-                continue;
-            const RepInfo& rep = a.reps[a.rep_of[i]];
+            const bool active = i < a.N;
+            const uint32_t ie = min(i, a.N - 1);
+            const RepInfo& rep = a.reps[a.rep_of[ie]];
             LaneBox box;
 #pragma unroll
             for (int k = 0; k < 3; ++k)
@@ -398,13 +407,14 @@ __global__ void __launch_bounds__(512) k_b_forces(const BatchForceArgs a)
                 box.Linv[k] = rep.Linv[k];
                 box.fx_scale[k] = rep.fx_scale[k];
                 }
-            const AnisoSelf<R> me = aniso_self<MIXED>(a.pos, a.pos_fixed, a.orientation, i);
+            const AnisoSelf<R> me = aniso_self<MIXED>(a.pos, a.pos_fixed, a.orientation, ie);
             AnisoAcc acc;
-            const uint32_t nn = a.n_neigh[i];
-            for (uint32_t k = 0; k < nn; ++k)
-                tpm_entry<MIXED, VIRIAL, SINGLE>(a.nlist[(size_t)k * a.pitch + i], a.pos, a.pos_fixed,
-                                                 a.orientation, me, box, nt, q0, s_q, acc);
-            aniso_store<VIRIAL>(a.force, a.torque, a.virial, a.vpitch, i, acc);
+            const uint32_t nn = active ? a.n_neigh[i] : 0u;
+            const uint32_t trips = __builtin_amdgcn_readfirstlane((tpm_wave_max(nn) + 3u) & ~3u);
+            tpm_list<MIXED, VIRIAL, SINGLE, MIXED>(ie, nn, trips, a.nlist, a.pitch, a.pos, a.pos_fixed, a.orientation,
+                                                   a.dirs, me, box, nt, q0, s_q, acc);
+            if (active)
+                aniso_store<VIRIAL>(a.force, a.torque, a.virial, a.vpitch, i, acc);
             }
         }
     }
@@ -663,6 +673,7 @@ struct azp_batch
     DeviceArray<uint32_t> cell_of, slot_of, cell_count, cell_start, members, cell_rep, scan_sums;
     DeviceArray<uint32_t> n_neigh, list, flags, counter, rep_flag;
     DeviceArray<double4> last_pos;
+    DeviceArray<float4> dirs;
     DeviceArray<unsigned long long> d_timestep, d_builds; // builds: [0] rebuild events, [1] replica rebuilds
     uint32_t* h_flags = nullptr; // pinned
     uint32_t ncell = 0;
@@ -826,6 +837,7 @@ int batch_upload_tables(azp_batch* b, hipStream_t s)
     AZP_TRY(b->members.reserve(b->N));
     AZP_TRY(b->n_neigh.reserve(b->N));
     AZP_TRY(b->last_pos.reserve(b->N));
+    AZP_TRY(b->dirs.reserve(b->N));
     AZP_TRY(b->cell_count.reserve(b->ncell));
     AZP_TRY(b->cell_start.reserve((size_t)b->ncell + 1));
     AZP_TRY(b->scan_sums.reserve(scan_chunks(b->ncell)));
@@ -937,6 +949,7 @@ int launch_forces_t(azp_batch* b, const azp_batch_tuning& t, hipStream_t s)
     fa.rcutsq = b->d_rcutsq.ptr;
     fa.ntypes = b->ntypes;
     fa.shift_mode = b->shift_mode;
+    fa.dirs = b->dirs.ptr;
     fa.force = reinterpret_cast<double4*>(b->arr.d_force);
     fa.torque = reinterpret_cast<double4*>(b->arr.d_torque);
     fa.virial = b->arr.d_virial;
@@ -966,8 +979,9 @@ int enqueue_forces(azp_batch* b, const azp_batch_tuning& t, hipStream_t s)
     if (b->precision == AZP_PREC_MIXED)
         {
         hipLaunchKernelGGL(k_b_pack_fixed, dim3(grid_for(b->N, 256)), dim3(256), 0, s,
-                           reinterpret_cast<const double4*>(b->arr.d_pos), b->N, b->rep_of.ptr, b->d_reps.ptr,
-                           reinterpret_cast<int4*>(b->arr.d_pos_fixed));
+                           reinterpret_cast<const double4*>(b->arr.d_pos),
+                           reinterpret_cast<const double4*>(b->arr.d_orientation), b->N, b->rep_of.ptr,
+                           b->d_reps.ptr, reinterpret_cast<int4*>(b->arr.d_pos_fixed), b->dirs.ptr);
         }
     const bool single = b->ntypes == 1;
     const bool mixed = b->precision == AZP_PREC_MIXED;

@@ -248,124 +248,6 @@ template<bool SINGLE> __global__ void __launch_bounds__(256) k_walk(const WalkAr
             a.nlist[(size_t)k * a.pitch + i] = i;
     }
 
-// Cell-centric walk for dense systems: one wavefront per cell.  The lanes
-// are the cell's members (64 at a time), and they scan the cell's 27
-// neighbor cells together: every candidate is the same for all lanes, so its
-// position and index come through scalar loads and only the test runs per
-// lane.  Candidates, their order and the FP32 / FP64 predicate are those of
-// walk_one, so the lists are identical; columns are padded afterwards by
-// k_pad_waves.
-template<bool SINGLE> __global__ void __launch_bounds__(64) k_walk_cells(const WalkArgs a, uint32_t ncell)
-    {
-    const uint32_t c = xcd_logical_block();
-    if (c >= ncell)
-        return;
-    const uint32_t lane = threadIdx.x;
-    const uint32_t cb = a.start[c], ce = a.start[c + 1];
-    const int cx = (int)(c % (uint32_t)a.g.n[0]);
-    const int cy = (int)((c / (uint32_t)a.g.n[0]) % (uint32_t)a.g.n[1]);
-    const int cz = (int)(c / ((uint32_t)a.g.n[0] * (uint32_t)a.g.n[1]));
-    int ox[3], oy[3], oz[3];
-    const int nx = stencil_axis(cx, a.g.n[0], ox);
-    const int ny = stencil_axis(cy, a.g.n[1], oy);
-    const int nz = stencil_axis(cz, a.g.n[2], oz);
-    const double rl0 = a.r_listsq[0];
-    const float lo0 = a.r_listsq_lo[0], hi0 = a.r_listsq_hi[0];
-    for (uint32_t q = cb; q < ce; q += 64)
-        {
-        const uint32_t self = q + lane;
-        bool act = self < ce;
-        const uint32_t i = act ? a.members[self] : 0u;
-        const double4 pi = a.pos[i];
-        const uint32_t ti = checked_type(pi.w, a.ntypes);
-        const bool finite = isfinite(pi.x) && isfinite(pi.y) && isfinite(pi.z);
-        if (act && (ti == a.ntypes || !finite))
-            {
-            atomicOr(a.overflow + 2, finite ? 1u : 2u);
-            a.n_neigh[i] = 0u;
-            act = false;
-            }
-        const int4 fi = a.cell_fixed[act ? self : cb];
-        uint32_t count = 0;
-        for (int iz = 0; iz < nz; ++iz)
-            for (int iy = 0; iy < ny; ++iy)
-                for (int ix = 0; ix < nx; ++ix)
-                    {
-                    // wave-uniform (scalar) cell bounds and candidate loads
-                    const uint32_t nb = __builtin_amdgcn_readfirstlane(
-                        (uint32_t)((oz[iz] * a.g.n[1] + oy[iy]) * a.g.n[0] + ox[ix]));
-                    const uint32_t b = __builtin_amdgcn_readfirstlane(a.start[nb]);
-                    const uint32_t e = __builtin_amdgcn_readfirstlane(a.start[nb + 1]);
-                    for (uint32_t m0 = b; m0 < e; m0 += 4)
-                        {
-                        int4 pc[4];
-#pragma unroll
-                        for (uint32_t u = 0; u < 4; ++u)
-                            pc[u] = a.cell_fixed[min(m0 + u, e - 1)];
-#pragma unroll
-                        for (uint32_t u = 0; u < 4; ++u)
-                        {
-                        const uint32_t m = m0 + u;
-                        const int4 pj = pc[u];
-                        const uint32_t tj = (uint32_t)pj.w;
-                        if (m >= e || tj >= a.ntypes)
-                            continue;
-                        const uint32_t tp = SINGLE ? 0u : ti * a.ntypes + tj;
-                        const float lo = SINGLE ? lo0 : a.r_listsq_lo[tp];
-                        const float hi = SINGLE ? hi0 : a.r_listsq_hi[tp];
-                        const float dx = (float)fixed_delta(fi.x, pj.x) * a.fx_scale[0];
-                        const float dy = (float)fixed_delta(fi.y, pj.y) * a.fx_scale[1];
-                        const float dz = (float)fixed_delta(fi.z, pj.z) * a.fx_scale[2];
-                        const float rsq = dx * dx + dy * dy + dz * dz;
-                        const bool cand = act && m != self;
-                        bool in = cand && rsq < lo;
-                        if (cand && !in && rsq <= hi)
-                            {
-                            const double4 p64 = a.pos[a.members[m]];
-                            const double rl = SINGLE ? rl0 : a.r_listsq[tp];
-                            const double ex = min_image_1d(pi.x - p64.x, a.box.L[0], a.box.Linv[0]);
-                            const double ey = min_image_1d(pi.y - p64.y, a.box.L[1], a.box.Linv[1]);
-                            const double ez = min_image_1d(pi.z - p64.z, a.box.L[2], a.box.Linv[2]);
-                            in = rsq3_nofma(ex, ey, ez) <= rl;
-                            }
-                        if (in)
-                            {
-                            if (count < a.max_neighbors)
-                                a.nlist[(size_t)count * a.pitch + i] = a.members[m];
-                            ++count;
-                            }
-                        }
-                        }
-                    }
-        if (act)
-            {
-            a.n_neigh[i] = min(count, a.max_neighbors);
-            if (count > a.max_neighbors)
-                atomicMax(a.overflow, count);
-            }
-        }
-    }
-
-// Pads the columns of every 64 consecutive particles with their own index up
-// to the group's longest list rounded up to 8 (after k_walk_cells).
-__global__ void __launch_bounds__(256) k_pad_waves(uint32_t N,
-                                                   const uint32_t* __restrict__ n_neigh,
-                                                   uint32_t* __restrict__ nlist,
-                                                   uint32_t pitch,
-                                                   uint32_t max_neighbors)
-    {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool active = i < N;
-    const uint32_t count = active ? n_neigh[i] : 0u;
-    uint32_t wmax = count;
-    for (int off = 32; off > 0; off >>= 1)
-        wmax = max(wmax, (uint32_t)__shfl_xor((int)wmax, off));
-    const uint32_t padded = (wmax + 7u) & ~7u;
-    if (active && padded <= max_neighbors)
-        for (uint32_t k = count; k < padded; ++k)
-            nlist[(size_t)k * pitch + i] = i;
-    }
-
 // Sets *flag when some particle moved farther than sqrt(thresh_sq).
 __global__ void k_displacement(const double4* __restrict__ pos,
                                const double4* __restrict__ last,
@@ -519,19 +401,7 @@ static int nlist_build(azp_nlist* nl, const azp_particles* p, hipStream_t s)
         wa.pitch = nl->pitch;
         wa.max_neighbors = nl->max_neighbors;
         wa.overflow = nl->flags.ptr;
-        // dense cells (>= 16 particles on average): one wavefront per cell with
-        // scalar candidate loads; otherwise one lane per particle
-        const bool by_cell = (double)N / (double)ncell >= 16.0;
-        if (by_cell)
-            {
-            if (nl->ntypes == 1)
-                hipLaunchKernelGGL(k_walk_cells<true>, dim3((unsigned)ncell), dim3(64), 0, s, wa, (uint32_t)ncell);
-            else
-                hipLaunchKernelGGL(k_walk_cells<false>, dim3((unsigned)ncell), dim3(64), 0, s, wa, (uint32_t)ncell);
-            hipLaunchKernelGGL(k_pad_waves, dim3(grid_for(N, B)), dim3(B), 0, s, N, nl->n_neigh.ptr, nl->list.ptr,
-                               nl->pitch, nl->max_neighbors);
-            }
-        else if (nl->ntypes == 1)
+        if (nl->ntypes == 1)
             hipLaunchKernelGGL(k_walk<true>, dim3(grid_for(N, B)), dim3(B), 0, s, wa);
         else
             hipLaunchKernelGGL(k_walk<false>, dim3(grid_for(N, B)), dim3(B), 0, s, wa);

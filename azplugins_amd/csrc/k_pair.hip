@@ -14,8 +14,8 @@
 // Two precision forms:
 //   plain (FP64):  FP64 positions, min image, evaluator and accumulation.
 //   mixed:         positions as wrapped 32-bit fixed point (16-byte gather),
-//                  dx exact in integers then FP32, FP32 evaluator; each batch
-//                  of 8 list entries is summed in FP32 and added to FP64
+//                  dx exact in integers then FP32, FP32 evaluator; every 16
+//                  list entries are summed in FP32 and added to FP64
 //                  accumulators.  Pairs whose FP32 rsq lies within a relative
 //                  band of 1e-5 around rcutsq are re-tested on the FP64
 //                  positions, so the in/out cutoff decision is the FP64 one.
@@ -132,8 +132,8 @@ __device__ __forceinline__ void accumulate(Acc& acc, double fd, double dx, doubl
         }
     }
 
-// Mixed form: the terms of one batch of list entries are summed in FP32 (at
-// most 8 terms), then each batch sum is added to the FP64 accumulators.
+// Mixed form: the terms of two batches of list entries are summed in FP32 (at
+// most 16 terms), then each such sum is added to the FP64 accumulators.
 struct Acc32
     {
     float fx = 0, fy = 0, fz = 0, e = 0;
@@ -423,6 +423,7 @@ __global__ void __launch_bounds__(512) k_pair(const PairKernelArgs a)
         const int4 pi = fixed4(a.pos[ie], a.box);
         const uint32_t ti = (uint32_t)pi.w;
         const uint32_t col_off = ie * 4u;
+        Acc32 part;
         for (uint32_t k = 0; k < P; k += U)
             {
             uint32_t w[U / 2];
@@ -443,7 +444,6 @@ __global__ void __launch_bounds__(512) k_pair(const PairKernelArgs a)
 #pragma unroll
             for (uint32_t u = 0; u < U; ++u)
                 pj[u] = s_pos[e[u]];
-            Acc32 part;
 #pragma unroll
             for (uint32_t u = 0; u < U; ++u)
                 {
@@ -453,7 +453,11 @@ __global__ void __launch_bounds__(512) k_pair(const PairKernelArgs a)
                                                        fixed_delta(pi.x, pj[u].x), fixed_delta(pi.y, pj[u].y),
                                                        fixed_delta(pi.z, pj[u].z), part);
                 }
-            flush32<VIRIAL>(acc, part);
+            if ((k & U) != 0 || k + U >= P)
+                {
+                flush32<VIRIAL>(acc, part);
+                part = Acc32();
+                }
             }
         if (i < a.N)
             store_scaled<VIRIAL>(a, i, acc);
@@ -466,6 +470,7 @@ __global__ void __launch_bounds__(512) k_pair(const PairKernelArgs a)
         // SGPR base and a VGPR offset: no per-entry 64-bit address math)
         const uint32_t col_off = ie * 4u;
         const char* pf_base = reinterpret_cast<const char*>(a.pos_fixed);
+        Acc32 part;
         for (uint32_t k = 0; k < P; k += U)
             {
             uint32_t jj[U];
@@ -479,7 +484,6 @@ __global__ void __launch_bounds__(512) k_pair(const PairKernelArgs a)
 #pragma unroll
             for (uint32_t u = 0; u < U; ++u)
                 pj[u] = *reinterpret_cast<const int4*>(pf_base + (jj[u] << 4));
-            Acc32 part;
 #pragma unroll
             for (uint32_t u = 0; u < U; ++u)
                 {
@@ -489,7 +493,11 @@ __global__ void __launch_bounds__(512) k_pair(const PairKernelArgs a)
                                                        fixed_delta(pi.y, pj[u].y), fixed_delta(pi.z, pj[u].z),
                                                        part);
                 }
-            flush32<VIRIAL>(acc, part);
+            if ((k & U) != 0 || k + U >= P)
+                {
+                flush32<VIRIAL>(acc, part);
+                part = Acc32();
+                }
             }
         if (i < a.N)
             store_scaled<VIRIAL>(a, i, acc);

@@ -142,47 +142,58 @@ aniso_self(const double4* pos, const int4* pos_fixed, const double4* orientation
     return s;
     }
 
-// One list entry (i, j): the cutoff test (rsq > rcsq is out; the mixed form
-// re-tests pairs within 1e-5 of the cutoff in FP64), the evaluator and the
-// accumulation of F, tau_i, E and the virial dx_b F_a.
-template<bool MIXED, bool VIRIAL, bool SINGLE>
-__device__ __forceinline__ void tpm_entry(uint32_t j,
-                                          const double4* pos,
-                                          const int4* pos_fixed,
-                                          const double4* orientation,
-                                          const AnisoSelf<std::conditional_t<MIXED, float, double>>& me,
-                                          const LaneBox& box,
-                                          uint32_t nt,
-                                          const TPMPrep<std::conditional_t<MIXED, float, double>>& q0,
-                                          const TPMPrep<std::conditional_t<MIXED, float, double>>* s_q,
-                                          AnisoAcc& acc)
+// Director of particle j from its quaternion (s, x, y, z), or a director
+// computed beforehand (x, y, z of a float4).
+template<typename R> __device__ __forceinline__ void director_of(const double4& o, R n[3])
+    {
+    director<R>((R)o.x, (R)o.y, (R)o.z, (R)o.w, n);
+    }
+template<typename R> __device__ __forceinline__ void director_of(const float4& d, R n[3])
+    {
+    n[0] = (R)d.x;
+    n[1] = (R)d.y;
+    n[2] = (R)d.z;
+    }
+
+// One list entry (i, j) from j's prefetched position (int4 fixed point in
+// the mixed form, double4 in the plain form) and orientation: the cutoff test
+// (rsq > rcsq is out; the mixed form re-tests pairs within 1e-5 of the cutoff
+// in FP64), the evaluator and the accumulation of F, tau_i, E and the virial
+// dx_b F_a.  Padding entries (valid false) contribute nothing.
+template<bool MIXED, bool VIRIAL, bool SINGLE, typename PosT, typename OriT>
+__device__ __forceinline__ void tpm_term(bool valid,
+                                         uint32_t j,
+                                         const PosT& pj,
+                                         const OriT& oj,
+                                         const double4* pos,
+                                         const AnisoSelf<std::conditional_t<MIXED, float, double>>& me,
+                                         const LaneBox& box,
+                                         uint32_t nt,
+                                         const TPMPrep<std::conditional_t<MIXED, float, double>>& q0,
+                                         const TPMPrep<std::conditional_t<MIXED, float, double>>* s_q,
+                                         AnisoAcc& acc)
     {
     using R = std::conditional_t<MIXED, float, double>;
     R dr[3];
     R rsq;
-    uint32_t tj;
+    const uint32_t tj = (uint32_t)pj.w;
+    const TPMPrep<R>& q = SINGLE ? q0 : s_q[me.type * nt + (valid ? tj : me.type)];
     bool in;
     if constexpr (!MIXED)
         {
-        const double4 pj = pos[j];
         dr[0] = min_image_1d(me.pos.x - pj.x, box.L[0], box.Linv[0]);
         dr[1] = min_image_1d(me.pos.y - pj.y, box.L[1], box.Linv[1]);
         dr[2] = min_image_1d(me.pos.z - pj.z, box.L[2], box.Linv[2]);
         rsq = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
-        tj = (uint32_t)pj.w;
-        const TPMPrep<R>& q = SINGLE ? q0 : s_q[me.type * nt + tj];
-        in = !(rsq > q.rcsq); // AnisoPairEvaluatorTwoPatchMorse.h:135
+        in = valid && !(rsq > q.rcsq); // AnisoPairEvaluatorTwoPatchMorse.h:135
         }
     else
         {
-        const int4 pj = pos_fixed[j];
         dr[0] = (R)fixed_delta(me.pfix.x, pj.x) * box.fx_scale[0];
         dr[1] = (R)fixed_delta(me.pfix.y, pj.y) * box.fx_scale[1];
         dr[2] = (R)fixed_delta(me.pfix.z, pj.z) * box.fx_scale[2];
         rsq = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
-        tj = (uint32_t)pj.w;
-        const TPMPrep<R>& q = SINGLE ? q0 : s_q[me.type * nt + tj];
-        in = !(rsq > q.rcsq_hi);
+        in = valid && !(rsq > q.rcsq_hi);
         if (in && rsq > q.rcsq_lo)
             {
             const double4 pj64 = pos[j];
@@ -195,10 +206,8 @@ __device__ __forceinline__ void tpm_entry(uint32_t j,
         }
     if (!in)
         return;
-    const TPMPrep<R>& q = SINGLE ? q0 : s_q[me.type * nt + tj];
-    const double4 oj = orientation[j];
     R n_j[3];
-    director<R>((R)oj.x, (R)oj.y, (R)oj.z, (R)oj.w, n_j);
+    director_of<R>(oj, n_j);
     R f[3], t[3], e;
     tpm_eval<R>(q, dr, rsq, me.n, n_j, f, t, e);
     const double Fx = f[0], Fy = f[1], Fz = f[2];
@@ -218,6 +227,64 @@ __device__ __forceinline__ void tpm_entry(uint32_t j,
         acc.v3 = fma(dy, Fy, acc.v3);
         acc.v4 = fma(dz, Fy, acc.v4);
         acc.v5 = fma(dz, Fz, acc.v5);
+        }
+    }
+
+__device__ __forceinline__ uint32_t tpm_wave_max(uint32_t v)
+    {
+    for (int off = 32; off > 0; off >>= 1)
+        v = max(v, (uint32_t)__shfl_xor((int)v, off));
+    return v;
+    }
+
+// Particle ie's list walked in a wave-uniform loop of `trips` entries (the
+// wavefront's longest list, rounded up to 4), four entries at a time: their
+// list words, positions and orientations (or precomputed directors, `dirs`)
+// are loaded together before the terms are evaluated.  Entries past the
+// lane's own count are padding.
+template<bool MIXED, bool VIRIAL, bool SINGLE, bool DIRS>
+__device__ __forceinline__ void tpm_list(uint32_t ie,
+                                         uint32_t nn,
+                                         uint32_t trips,
+                                         const uint32_t* nlist,
+                                         uint32_t pitch,
+                                         const double4* pos,
+                                         const int4* pos_fixed,
+                                         const double4* orientation,
+                                         const float4* dirs,
+                                         const AnisoSelf<std::conditional_t<MIXED, float, double>>& me,
+                                         const LaneBox& box,
+                                         uint32_t nt,
+                                         const TPMPrep<std::conditional_t<MIXED, float, double>>& q0,
+                                         const TPMPrep<std::conditional_t<MIXED, float, double>>* s_q,
+                                         AnisoAcc& acc)
+    {
+    constexpr uint32_t U = 4;
+    using PosT = std::conditional_t<MIXED, int4, double4>;
+    using OriT = std::conditional_t<DIRS, float4, double4>;
+    for (uint32_t k = 0; k < trips; k += U)
+        {
+        uint32_t jj[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u)
+            jj[u] = k + u < nn ? nlist[(size_t)(k + u) * pitch + ie] : ie;
+        PosT pj[U];
+        OriT oj[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u)
+            {
+            if constexpr (MIXED)
+                pj[u] = pos_fixed[jj[u]];
+            else
+                pj[u] = pos[jj[u]];
+            if constexpr (DIRS)
+                oj[u] = dirs[jj[u]];
+            else
+                oj[u] = orientation[jj[u]];
+            }
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u)
+            tpm_term<MIXED, VIRIAL, SINGLE>(k + u < nn, jj[u], pj[u], oj[u], pos, me, box, nt, q0, s_q, acc);
         }
     }
 

@@ -243,7 +243,9 @@ def test_full_batch_workload():
                       TPM, RC, precision="mixed")
     e1 = rel_err(host(b.force)[b.replica(r)], single.force)
     say(f"  largest replica ({sizes[r]}) vs single-system launcher: {e1:.2e}")
-    assert e1 < 1e-12
+    # both mixed; the batch reads directors computed once per particle, the
+    # single-system kernel forms them per pair, so FP32 roundings may differ
+    assert e1 < 1e-6
     tune = b.autotune()
     say(f"  autotune {tune}")
     b.set_timestep(0)
