@@ -14,11 +14,14 @@
 //               rsq <= r_list(ti,tj)^2 — FP32 on exact integer displacements,
 //               with every candidate near r_list decided by the same FP64
 //               predicate, evaluated without contraction, as the host oracle,
-//               so the pair sets agree exactly.  Entry k of particle i goes to
-//               nlist[k * pitch + i] (coalesced for one-lane-per-particle
-//               consumers).  Lanes that find more than max_neighbors entries
-//               keep counting and report the largest count; the host grows
-//               the list and reruns the walk.
+//               so the pair sets agree exactly.  Each lane appends to its own
+//               particle-major row (its successive 4-byte writes fill whole
+//               lines; scattered column writes cost 8x in DRAM traffic).
+//               Lanes that find more than max_neighbors entries keep counting
+//               and report the largest count; the host grows the list and
+//               reruns the walk.
+//   5. columns: rows transposed to nlist[k * pitch + i] (coalesced for
+//               one-lane-per-particle consumers), padded per 64 particles.
 // Rebuild check: a particle moved farther than r_buff/2 since the last build.
 #include <algorithm>
 #include <cmath>
@@ -131,8 +134,7 @@ struct WalkArgs
     const float* r_listsq_hi; // r_listsq (1 + 1e-5) in FP32
     uint32_t ntypes;
     uint32_t* n_neigh;
-    uint32_t* nlist;
-    uint32_t pitch;
+    uint32_t* rows; // particle-major staging: entry k of particle i at rows[i * max_neighbors + k]
     uint32_t max_neighbors;
     uint32_t* overflow; // [0] largest count seen when > max_neighbors, [2] bad type seen
     };
@@ -213,7 +215,7 @@ template<bool SINGLE> __device__ uint32_t walk_one(const WalkArgs& a, uint32_t i
                     if (in)
                         {
                         if (count < a.max_neighbors)
-                            a.nlist[(size_t)count * a.pitch + i] = a.members[m];
+                            a.rows[(size_t)i * a.max_neighbors + count] = a.members[m];
                         ++count;
                         }
                     }
@@ -222,30 +224,46 @@ template<bool SINGLE> __device__ uint32_t walk_one(const WalkArgs& a, uint32_t i
     return count;
     }
 
-// One lane per particle.  After listing, the 64 lanes of a wavefront pad their
-// columns with their own index up to the wave's longest list rounded up to a
-// multiple of 8, so consumers can run a wave-uniform loop over batches of 8
-// entries with no per-entry bounds test (padding entries have j == i).
+// One lane per particle.  Each lane writes its entries to its own row of the
+// particle-major staging array (successive writes of a lane fill whole cache
+// lines); k_list_columns then transposes into the column layout.
 template<bool SINGLE> __global__ void __launch_bounds__(256) k_walk(const WalkArgs a)
     {
     const uint32_t i = xcd_logical_block() * blockDim.x + threadIdx.x;
-    const bool active = i < a.N;
-    const uint32_t count = active ? walk_one<SINGLE>(a, i) : 0u;
-    if (active)
-        {
-        // a list longer than the capacity is truncated (and reported), so
-        // consumers never read past the allocation
-        a.n_neigh[i] = min(count, a.max_neighbors);
-        if (count > a.max_neighbors)
-            atomicMax(a.overflow, count);
-        }
+    if (i >= a.N)
+        return;
+    const uint32_t count = walk_one<SINGLE>(a, i);
+    // a list longer than the capacity is truncated (and reported), so
+    // consumers never read past the allocation
+    a.n_neigh[i] = min(count, a.max_neighbors);
+    if (count > a.max_neighbors)
+        atomicMax(a.overflow, count);
+    }
+
+// Particle-major rows -> column layout nlist[k * pitch + i], one wavefront per
+// 64 consecutive particles: each row k of the group is one contiguous 256-byte
+// store.  Columns are padded with the particle's own index up to the group's
+// longest list rounded up to 8, so consumers can run a wave-uniform loop over
+// batches of 8 entries (padding entries have j == i).
+__global__ void __launch_bounds__(64) k_list_columns(uint32_t N,
+                                                     const uint32_t* __restrict__ n_neigh,
+                                                     const uint32_t* __restrict__ rows,
+                                                     uint32_t max_neighbors,
+                                                     uint32_t* __restrict__ nlist,
+                                                     uint32_t pitch)
+    {
+    const uint32_t i = blockIdx.x * 64u + threadIdx.x;
+    const bool active = i < N;
+    const uint32_t count = active ? n_neigh[i] : 0u;
     uint32_t wmax = count;
     for (int off = 32; off > 0; off >>= 1)
         wmax = max(wmax, (uint32_t)__shfl_xor((int)wmax, off));
-    const uint32_t padded = (wmax + 7u) & ~7u;
-    if (active && wmax <= a.max_neighbors)
-        for (uint32_t k = count; k < padded; ++k)
-            a.nlist[(size_t)k * a.pitch + i] = i;
+    const uint32_t padded = min((wmax + 7u) & ~7u, max_neighbors);
+    if (!active)
+        return;
+    const uint32_t* row = rows + (size_t)i * max_neighbors;
+    for (uint32_t k = 0; k < padded; ++k)
+        nlist[(size_t)k * pitch + i] = k < count ? row[k] : i;
     }
 
 // Sets *flag when some particle moved farther than sqrt(thresh_sq).
@@ -297,6 +315,7 @@ struct azp_nlist
     DeviceArray<uint32_t> rank_of;
     DeviceArray<uint32_t> n_neigh;
     DeviceArray<uint32_t> list;
+    DeviceArray<uint32_t> rows; // particle-major staging of the walk
     DeviceArray<double4> last_pos;
     DeviceArray<uint32_t> cell_of, slot_of, count, start, members, scan_sums;
     DeviceArray<uint32_t> flags; // [0] overflow, [1] rebuild, [2] bad type
@@ -379,6 +398,7 @@ static int nlist_build(azp_nlist* nl, const azp_particles* p, hipStream_t s)
     for (int attempt = 0; attempt < 4; ++attempt)
         {
         AZP_TRY(nl->list.reserve((size_t)nl->max_neighbors * nl->pitch));
+        AZP_TRY(nl->rows.reserve((size_t)nl->max_neighbors * N));
         hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, nl->flags.ptr, 0u, (size_t)3);
         WalkArgs wa;
         wa.pos = pos;
@@ -397,14 +417,15 @@ static int nlist_build(azp_nlist* nl, const azp_particles* p, hipStream_t s)
         wa.r_listsq_hi = nl->r_listsq_hi.ptr;
         wa.ntypes = nl->ntypes;
         wa.n_neigh = nl->n_neigh.ptr;
-        wa.nlist = nl->list.ptr;
-        wa.pitch = nl->pitch;
+        wa.rows = nl->rows.ptr;
         wa.max_neighbors = nl->max_neighbors;
         wa.overflow = nl->flags.ptr;
         if (nl->ntypes == 1)
             hipLaunchKernelGGL(k_walk<true>, dim3(grid_for(N, B)), dim3(B), 0, s, wa);
         else
             hipLaunchKernelGGL(k_walk<false>, dim3(grid_for(N, B)), dim3(B), 0, s, wa);
+        hipLaunchKernelGGL(k_list_columns, dim3(grid_for(N, 64)), dim3(64), 0, s, N, nl->n_neigh.ptr, nl->rows.ptr,
+                           nl->max_neighbors, nl->list.ptr, nl->pitch);
         AZP_LAUNCH_CHECK();
         AZP_HIP(hipMemcpyAsync(nl->h_flags, nl->flags.ptr, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         AZP_HIP(hipStreamSynchronize(s));
