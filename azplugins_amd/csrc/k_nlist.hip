@@ -139,6 +139,15 @@ struct WalkArgs
     uint32_t* overflow; // [0] largest count seen when > max_neighbors, [2] bad type seen
     };
 
+// Eight buffered entries -> one 32-byte store (rows are 32-byte aligned:
+// max_neighbors is a multiple of 8).
+__device__ __forceinline__ void store_chunk(uint32_t* dst, const uint32_t* lbuf)
+    {
+    uint4* d = reinterpret_cast<uint4*>(dst);
+    d[0] = make_uint4(lbuf[0], lbuf[1], lbuf[2], lbuf[3]);
+    d[1] = make_uint4(lbuf[4], lbuf[5], lbuf[6], lbuf[7]);
+    }
+
 // Lists particle i's neighbors and returns how many it has.  Candidates are
 // read from the cell-ordered fixed-point copy (16 bytes, contiguous per
 // cell) and tested in FP32 on the exact integer displacement; a candidate
@@ -146,7 +155,7 @@ struct WalkArgs
 // FP64 predicate of the host oracle (minimum image and r^2 without
 // contraction, rsq <= r_list^2), so the pair set is the FP64 one.  Outside
 // the band the FP32 value is within ~1e-6 relative of the FP64 one.
-template<bool SINGLE> __device__ uint32_t walk_one(const WalkArgs& a, uint32_t i)
+template<bool SINGLE> __device__ uint32_t walk_one(const WalkArgs& a, uint32_t i, uint32_t* lbuf)
     {
     const double4 pi = a.pos[i];
     const uint32_t ti = checked_type(pi.w, a.ntypes);
@@ -214,13 +223,19 @@ template<bool SINGLE> __device__ uint32_t walk_one(const WalkArgs& a, uint32_t i
                         }
                     if (in)
                         {
-                        if (count < a.max_neighbors)
-                            a.rows[(size_t)i * a.max_neighbors + count] = a.members[m];
+                        // entries gather in the lane's LDS slots and leave
+                        // as whole 32-byte chunks of its row
+                        lbuf[count & 7u] = a.members[m];
+                        if ((count & 7u) == 7u && count < a.max_neighbors)
+                            store_chunk(a.rows + (size_t)i * a.max_neighbors + (count & ~7u), lbuf);
                         ++count;
                         }
                     }
                     }
                 }
+    // the last partial chunk (slots past the count are never read)
+    if ((count & 7u) != 0u && count < a.max_neighbors)
+        store_chunk(a.rows + (size_t)i * a.max_neighbors + (count & ~7u), lbuf);
     return count;
     }
 
@@ -229,10 +244,11 @@ template<bool SINGLE> __device__ uint32_t walk_one(const WalkArgs& a, uint32_t i
 // lines); k_list_columns then transposes into the column layout.
 template<bool SINGLE> __global__ void __launch_bounds__(256) k_walk(const WalkArgs a)
     {
+    __shared__ uint32_t s_buf[256 * 8];
     const uint32_t i = xcd_logical_block() * blockDim.x + threadIdx.x;
     if (i >= a.N)
         return;
-    const uint32_t count = walk_one<SINGLE>(a, i);
+    const uint32_t count = walk_one<SINGLE>(a, i, s_buf + threadIdx.x * 8u);
     // a list longer than the capacity is truncated (and reported), so
     // consumers never read past the allocation
     a.n_neigh[i] = min(count, a.max_neighbors);
