@@ -274,12 +274,30 @@ __global__ void __launch_bounds__(64) k_list_columns(uint32_t N,
     uint32_t wmax = count;
     for (int off = 32; off > 0; off >>= 1)
         wmax = max(wmax, (uint32_t)__shfl_xor((int)wmax, off));
+    // a multiple of 8 (max_neighbors is one)
     const uint32_t padded = min((wmax + 7u) & ~7u, max_neighbors);
     if (!active)
         return;
-    const uint32_t* row = rows + (size_t)i * max_neighbors;
-    for (uint32_t k = 0; k < padded; ++k)
-        nlist[(size_t)k * pitch + i] = k < count ? row[k] : i;
+    // the row is read in 32-byte chunks, two chunks in flight
+    const uint4* row = reinterpret_cast<const uint4*>(rows + (size_t)i * max_neighbors);
+    for (uint32_t k0 = 0; k0 < padded; k0 += 16)
+        {
+        const bool two = k0 + 8 < padded;
+        uint4 c[4];
+        c[0] = row[k0 / 4];
+        c[1] = row[k0 / 4 + 1];
+        if (two)
+            {
+            c[2] = row[k0 / 4 + 2];
+            c[3] = row[k0 / 4 + 3];
+            }
+        const uint32_t v[16] = {c[0].x, c[0].y, c[0].z, c[0].w, c[1].x, c[1].y, c[1].z, c[1].w,
+                                c[2].x, c[2].y, c[2].z, c[2].w, c[3].x, c[3].y, c[3].z, c[3].w};
+#pragma unroll
+        for (uint32_t u = 0; u < 16; ++u)
+            if (u < 8 || two)
+                nlist[(size_t)(k0 + u) * pitch + i] = k0 + u < count ? v[u] : i;
+        }
     }
 
 // Sets *flag when some particle moved farther than sqrt(thresh_sq).
@@ -464,7 +482,7 @@ static int nlist_build(azp_nlist* nl, const azp_particles* p, hipStream_t s)
                     {
                     if (nl->tile_mode != 1 && nl->tile_mode != ts)
                         continue;
-                    AZP_TRY(tiles_build(nl->tiles, ts, N, nl->n_neigh.ptr, nl->list.ptr, nl->pitch,
+                    AZP_TRY(tiles_build(nl->tiles, ts, N, nl->n_neigh.ptr, nl->rows.ptr, nl->pitch,
                                         nl->max_neighbors, nl->h_flags + 4, s));
                     if (nl->tiles.valid)
                         break;

@@ -53,9 +53,25 @@ __device__ __forceinline__ uint32_t tile_find(const uint32_t* keys, uint32_t j)
     return 0u; // not reached: every listed index was inserted
     }
 
+// Eight entries k0 .. k0 + 7 of particle i from its particle-major row.
+__device__ __forceinline__ void row_chunk(const uint32_t* rows, uint32_t stride, uint32_t i, uint32_t k0, uint32_t v[8])
+    {
+    const uint4* r = reinterpret_cast<const uint4*>(rows + (size_t)i * stride + k0);
+    const uint4 a = r[0], b = r[1];
+    v[0] = a.x;
+    v[1] = a.y;
+    v[2] = a.z;
+    v[3] = a.w;
+    v[4] = b.x;
+    v[5] = b.y;
+    v[6] = b.z;
+    v[7] = b.w;
+    }
+
 __global__ void __launch_bounds__(512) k_tile_build(uint32_t N,
                                                     const uint32_t* __restrict__ n_neigh,
-                                                    const uint32_t* __restrict__ nlist,
+                                                    const uint32_t* __restrict__ rows,
+                                                    uint32_t stride,
                                                     uint32_t pitch,
                                                     uint32_t* __restrict__ tile_count,
                                                     uint32_t* __restrict__ tile_jset,
@@ -87,12 +103,10 @@ __global__ void __launch_bounds__(512) k_tile_build(uint32_t N,
         if (*vcount >= kTileCap)
             break;
         uint32_t jj[kBatch];
+        row_chunk(rows, stride, i, k0, jj);
 #pragma unroll
         for (uint32_t u = 0; u < kBatch; ++u)
-            jj[u] = k0 + u < nn ? nlist[(size_t)(k0 + u) * pitch + i] : kEmpty;
-#pragma unroll
-        for (uint32_t u = 0; u < kBatch; ++u)
-            if (jj[u] != kEmpty)
+            if (k0 + u < nn)
                 tile_insert(keys, &s_count, jj[u]);
         }
     __syncthreads();
@@ -155,13 +169,12 @@ __global__ void __launch_bounds__(512) k_tile_build(uint32_t N,
     for (uint32_t k0 = 0; k0 < P; k0 += kBatch)
         {
         uint32_t jj[kBatch];
-#pragma unroll
-        for (uint32_t u = 0; u < kBatch; ++u)
-            jj[u] = k0 + u < nn ? nlist[(size_t)(k0 + u) * pitch + i] : kEmpty;
+        if (k0 < nn)
+            row_chunk(rows, stride, i, k0, jj);
         uint32_t e[kBatch];
 #pragma unroll
         for (uint32_t u = 0; u < kBatch; ++u)
-            e[u] = jj[u] != kEmpty ? (uint32_t)rank[tile_find(keys, jj[u])] : M;
+            e[u] = k0 + u < nn ? (uint32_t)rank[tile_find(keys, jj[u])] : M;
 #pragma unroll
         for (uint32_t w = 0; w < kBatch / 2; ++w)
             tlist[(size_t)((k0 >> 1) + w) * pitch + i] = e[2 * w] | (e[2 * w + 1] << 16);
@@ -178,7 +191,7 @@ int tiles_build(TileList& t,
                 uint32_t tile_size,
                 uint32_t N,
                 const uint32_t* d_n_neigh,
-                const uint32_t* d_nlist,
+                const uint32_t* d_rows,
                 uint32_t pitch,
                 uint32_t max_neighbors,
                 uint32_t* h_flag,
@@ -198,8 +211,8 @@ int tiles_build(TileList& t,
     AZP_TRY(t.list.reserve((size_t)words * pitch));
     AZP_TRY(t.flag.reserve(2));
     hipLaunchKernelGGL(k_tile_flag_clear, dim3(1), dim3(64), 0, s, t.flag.ptr);
-    hipLaunchKernelGGL(k_tile_build, dim3(ntiles), dim3(tile_size), 0, s, N, d_n_neigh, d_nlist, pitch, t.count.ptr,
-                       t.jset.ptr, t.list.ptr, t.flag.ptr);
+    hipLaunchKernelGGL(k_tile_build, dim3(ntiles), dim3(tile_size), 0, s, N, d_n_neigh, d_rows, max_neighbors, pitch,
+                       t.count.ptr, t.jset.ptr, t.list.ptr, t.flag.ptr);
     AZP_LAUNCH_CHECK();
     AZP_HIP(hipMemcpyAsync(h_flag, t.flag.ptr, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     AZP_HIP(hipStreamSynchronize(s));

@@ -37,7 +37,10 @@ struct TileList
     bool valid = false;
     };
 
-// Builds tiles of `tile_size` (256 or 512) particles from the full list.
+// Builds tiles of `tile_size` (256 or 512) particles from the full list in
+// particle-major rows (entry k of particle i at d_rows[i * max_neighbors + k],
+// max_neighbors a multiple of 8); `pitch` is the column pitch of the tiled
+// list.
 // Returns AZP_OK and sets t.valid when every tile fits kTileCap - 1 distinct
 // neighbors; t.valid is false otherwise (the caller keeps the plain list).
 // Synchronizes `stream` (reads two flag words through `h_flag`, pinned).
@@ -45,7 +48,7 @@ int tiles_build(TileList& t,
                 uint32_t tile_size,
                 uint32_t N,
                 const uint32_t* d_n_neigh,
-                const uint32_t* d_nlist,
+                const uint32_t* d_rows,
                 uint32_t pitch,
                 uint32_t max_neighbors,
                 uint32_t* h_flag,
