@@ -90,85 +90,46 @@ __global__ void __launch_bounds__(512) k_dpd(const DPDKernelArgs a)
         s_p[t].rcutsq = a.rcutsq[t];
         }
     __syncthreads();
-    const uint32_t lb = xcd_logical_block();
-    const uint32_t i = lb * blockDim.x + threadIdx.x;
-    // wavefronts entirely past N leave together; lanes past N walk nothing
-    if (lb * blockDim.x + (threadIdx.x & ~63u) >= a.N)
+    const uint32_t i = xcd_logical_block() * blockDim.x + threadIdx.x;
+    if (i >= a.N)
         return;
-    const uint32_t ie = min(i, a.N - 1);
-    const double4 pi = a.pos[ie];
-    const double4 vi = a.vel[ie];
-    const uint32_t tagi = a.tag[ie];
+    const double4 pi = a.pos[i];
+    const double4 vi = a.vel[i];
+    const uint32_t tagi = a.tag[i];
     const uint32_t ti = (uint32_t)pi.w;
     double fx = 0, fy = 0, fz = 0, pe = 0;
     double v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0, v5 = 0;
-    const uint32_t nn = i < a.N ? a.n_neigh[i] : 0u;
-    // Wave-uniform loop over batches of 4 list entries: the batch's list
-    // words and positions are loaded together, then the velocities and tags
-    // of the pairs inside the cutoff, then the pair terms in list order.
-    uint32_t wmax = nn;
-    for (int off = 32; off > 0; off >>= 1)
-        wmax = max(wmax, (uint32_t)__shfl_xor((int)wmax, off));
-    const uint32_t trips = __builtin_amdgcn_readfirstlane((wmax + 3u) & ~3u);
-    constexpr uint32_t U = 4;
-    for (uint32_t k0 = 0; k0 < trips; k0 += U)
+    const uint32_t nn = a.n_neigh[i];
+    for (uint32_t k = 0; k < nn; ++k)
         {
-        uint32_t jj[U];
-#pragma unroll
-        for (uint32_t u = 0; u < U; ++u)
-            jj[u] = k0 + u < nn ? a.nlist[(size_t)(k0 + u) * a.nlist_pitch + ie] : ie;
-        double4 pj[U];
-#pragma unroll
-        for (uint32_t u = 0; u < U; ++u)
-            pj[u] = a.pos[jj[u]];
-        double dx[U], dy[U], dz[U], rsq[U];
-        bool in[U];
-#pragma unroll
-        for (uint32_t u = 0; u < U; ++u)
+        const uint32_t j = a.nlist[(size_t)k * a.nlist_pitch + i];
+        const double4 pj = a.pos[j];
+        const double dx = min_image_1d(pi.x - pj.x, a.box.L[0], a.box.Linv[0]);
+        const double dy = min_image_1d(pi.y - pj.y, a.box.L[1], a.box.Linv[1]);
+        const double dz = min_image_1d(pi.z - pj.z, a.box.L[2], a.box.Linv[2]);
+        const double rsq = dx * dx + dy * dy + dz * dz;
+        const DPDPair q = s_p[ti * nt + (uint32_t)pj.w];
+        if (!(rsq < q.rcutsq))
+            continue;
+        const double4 vj = a.vel[j];
+        const double rdotv = dx * (vi.x - vj.x) + dy * (vi.y - vj.y) + dz * (vi.z - vj.z);
+        double fdivr, fcons, e;
+        dpd_eval(q, rsq, a.seed, tagi, a.tag[j], a.timestep, a.dt, a.kT, rdotv, fdivr, fcons, e);
+        fx += dx * fdivr;
+        fy += dy * fdivr;
+        fz += dz * fdivr;
+        pe += e * 0.5;
+        if (VIRIAL)
             {
-            dx[u] = min_image_1d(pi.x - pj[u].x, a.box.L[0], a.box.Linv[0]);
-            dy[u] = min_image_1d(pi.y - pj[u].y, a.box.L[1], a.box.Linv[1]);
-            dz[u] = min_image_1d(pi.z - pj[u].z, a.box.L[2], a.box.Linv[2]);
-            rsq[u] = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
-            in[u] = k0 + u < nn && rsq[u] < s_p[ti * nt + (uint32_t)pj[u].w].rcutsq;
-            }
-        double4 vj[U];
-        uint32_t tj[U];
-#pragma unroll
-        for (uint32_t u = 0; u < U; ++u)
-            if (in[u])
-                {
-                vj[u] = a.vel[jj[u]];
-                tj[u] = a.tag[jj[u]];
-                }
-#pragma unroll
-        for (uint32_t u = 0; u < U; ++u)
-            {
-            if (!in[u])
-                continue;
-            const DPDPair q = s_p[ti * nt + (uint32_t)pj[u].w];
-            const double rdotv
-                = dx[u] * (vi.x - vj[u].x) + dy[u] * (vi.y - vj[u].y) + dz[u] * (vi.z - vj[u].z);
-            double fdivr, fcons, e;
-            dpd_eval(q, rsq[u], a.seed, tagi, tj[u], a.timestep, a.dt, a.kT, rdotv, fdivr, fcons, e);
-            fx += dx[u] * fdivr;
-            fy += dy[u] * fdivr;
-            fz += dz[u] * fdivr;
-            pe += e * 0.5;
-            if (VIRIAL)
-                {
-                const double fd2 = fcons * 0.5;
-                v0 += fd2 * dx[u] * dx[u];
-                v1 += fd2 * dx[u] * dy[u];
-                v2 += fd2 * dx[u] * dz[u];
-                v3 += fd2 * dy[u] * dy[u];
-                v4 += fd2 * dy[u] * dz[u];
-                v5 += fd2 * dz[u] * dz[u];
-                }
+            const double fd2 = fcons * 0.5;
+            v0 += fd2 * dx * dx;
+            v1 += fd2 * dx * dy;
+            v2 += fd2 * dx * dz;
+            v3 += fd2 * dy * dy;
+            v4 += fd2 * dy * dz;
+            v5 += fd2 * dz * dz;
             }
         }
-    if (i >= a.N)
-        return;
     a.force[i] = make_double4(fx, fy, fz, pe);
     if (VIRIAL)
         {
