@@ -99,6 +99,8 @@ class PairArgs(ctypes.Structure):
         ("d_tile_list", ctypes.c_void_p),
         ("tile_size", ctypes.c_uint32),
         ("tile_slots", ctypes.c_uint32),
+        ("d_tile_order", ctypes.c_void_p),
+        ("d_tile_nn", ctypes.c_void_p),
     ]
 
 
@@ -181,6 +183,8 @@ SIGNATURES = [
     ("azp_nlist_set_tiled", _i, [_vp, _i]),
     ("azp_nlist_get_tiles", _i, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                  ctypes.POINTER(_u32), ctypes.POINTER(_u32)]),
+    ("azp_nlist_get_tile_order", _i, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
+    ("azp_nlist_walk_kind", _i, [_vp, ctypes.POINTER(_i)]),
     ("azp_pair_create", _i, [ctypes.POINTER(_vp), ctypes.c_char_p, _u32]),
     ("azp_pair_destroy", _i, [_vp]),
     ("azp_pair_set_params", _i, [_vp, _u32, _u32, _pd]),
@@ -213,6 +217,9 @@ SIGNATURES = [
     ("azp_batch_nlist_download", _i, [_vp, _vp, _vp, _u64, ctypes.POINTER(_u32), ctypes.POINTER(_u32)]),
 ]
 
+# must equal AZP_ABI_VERSION of include/azp.h (azp_pair_args layout)
+ABI_VERSION = 2
+
 _lib = None
 
 
@@ -230,6 +237,11 @@ def lib():
             fn = getattr(L, name)
             fn.restype = res
             fn.argtypes = args
+        if L.azp_abi_version() != ABI_VERSION:
+            raise ImportError(
+                f"{LIB_PATH} has ABI version {L.azp_abi_version()}, these bindings expect "
+                f"{ABI_VERSION}: rebuild it with `python -m azplugins_amd.build`"
+            )
         _lib = L
     return _lib
 

@@ -169,6 +169,19 @@ class NeighborListGPU:
                   ctypes.byref(size), ctypes.byref(slots))
         return c.value, j.value, t.value, size.value, slots.value
 
+    def get_tile_order(self):
+        """(d_tile_order, d_tile_nn) for slot-ordered tiles, (None, None) otherwise."""
+        o = ctypes.c_void_p()
+        n = ctypes.c_void_p()
+        _azp.call("azp_nlist_get_tile_order", self._h, ctypes.byref(o), ctypes.byref(n))
+        return o.value, n.value
+
+    def walk_kind(self):
+        """1 when the last build wrote the tiled list from the per-block walk, 0 for the plain walk."""
+        k = ctypes.c_int()
+        _azp.call("azp_nlist_walk_kind", self._h, ctypes.byref(k))
+        return k.value
+
     def download(self, N):
         """(n_neigh[N], list[max_neighbors, pitch]) as numpy arrays."""
         import numpy as np

@@ -32,6 +32,14 @@ void clear_error()
     }
 
 int pack_fixed(const double* d_pos, uint32_t N, const azp_box& box, int32_t* d_out, hipStream_t stream);
+bool pair_tile_fits(int evaluator, uint32_t ntypes, uint32_t tile_slots);
+int nlist_get_on_stream(const azp_nlist* nl,
+                        hipStream_t s,
+                        bool need_plain,
+                        const uint32_t** d_n_neigh,
+                        const uint32_t** d_nlist,
+                        uint32_t* pitch,
+                        uint32_t* max_neighbors);
 
 } // namespace azp
 
@@ -458,7 +466,18 @@ extern "C" int azp_pair_compute(azp_pair* pp,
     std::memset(&a, 0, sizeof(a));
     a.particles = *p;
     a.out = *out;
-    AZP_TRY(azp_nlist_get(nl, &a.d_n_neigh, &a.d_nlist, &a.nlist_pitch, &a.max_neighbors));
+    // the mixed isotropic drivers run over the tiled list when it is there
+    // and its neighbor positions fit in LDS; everything else reads the plain
+    // list (expanded from the tiled one on first use)
+    uint32_t tiled = 0, slots = 0;
+    if (pp->precision == AZP_PREC_MIXED
+        && (pp->driver == AZP_DRIVER_PAIR || pp->driver == AZP_DRIVER_CONSERVATIVE))
+        {
+        AZP_TRY(azp_nlist_get_tiles(nl, nullptr, nullptr, nullptr, &tiled, &slots));
+        if (tiled != 0 && !pair_tile_fits(pp->evaluator, pp->ntypes, slots))
+            tiled = 0;
+        }
+    AZP_TRY(nlist_get_on_stream(nl, s, tiled == 0, &a.d_n_neigh, &a.d_nlist, &a.nlist_pitch, &a.max_neighbors));
     a.d_rcutsq = pp->d_rcutsq.ptr;
     a.d_ronsq = pp->d_ronsq.ptr;
     a.ntypes = pp->ntypes;
@@ -470,10 +489,6 @@ extern "C" int azp_pair_compute(azp_pair* pp,
     a.dt = pp->dt;
     a.kT = pp->kT;
     a.stream = stream;
-    uint32_t tiled = 0;
-    if (pp->precision == AZP_PREC_MIXED
-        && (pp->driver == AZP_DRIVER_PAIR || pp->driver == AZP_DRIVER_CONSERVATIVE))
-        AZP_TRY(azp_nlist_get_tiles(nl, nullptr, nullptr, nullptr, &tiled, nullptr));
     // the tiled kernel converts positions itself
     if (pp->precision == AZP_PREC_MIXED && !p->d_pos_fixed && p->N > 0 && tiled == 0)
         {
@@ -485,12 +500,11 @@ extern "C" int azp_pair_compute(azp_pair* pp,
         {
     case AZP_DRIVER_PAIR:
     case AZP_DRIVER_CONSERVATIVE:
-        if (pp->precision == AZP_PREC_MIXED)
+        if (tiled != 0)
             {
             AZP_TRY(azp_nlist_get_tiles(nl, &a.d_tile_count, &a.d_tile_jset, &a.d_tile_list, &a.tile_size,
                                         &a.tile_slots));
-            if (a.tile_size == 0)
-                a.d_tile_list = nullptr;
+            AZP_TRY(azp_nlist_get_tile_order(nl, &a.d_tile_order, &a.d_tile_nn));
             }
         return azp_gpu_compute_pair_forces(&a, pp->evaluator, pp->d_params.ptr);
     case AZP_DRIVER_DPD_THERMO:

@@ -6,6 +6,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <utility>
+#include <vector>
 
 #include "device_common.h"
 
@@ -43,6 +45,35 @@ inline CellGrid cell_grid_for(const double L[3], double rmax, uint32_t N)
         ncell = (size_t)g.n[0] * g.n[1] * g.n[2];
         }
     return g;
+    }
+
+// Rank of every cell (geometric index (z * ny + y) * nx + x) along the
+// Morton (Z-order) curve of its coordinates.  The cell list stores cells in
+// this order, so any run of consecutive slots covers a compact block of
+// space whatever order the particles come in.
+inline void morton_cell_rank(const CellGrid& g, std::vector<uint32_t>& rank)
+    {
+    const size_t n = (size_t)g.n[0] * g.n[1] * g.n[2];
+    auto spread = [](uint64_t v) {
+        v &= 0x1fffffull;
+        v = (v | (v << 32)) & 0x1f00000000ffffull;
+        v = (v | (v << 16)) & 0x1f0000ff0000ffull;
+        v = (v | (v << 8)) & 0x100f00f00f00f00full;
+        v = (v | (v << 4)) & 0x10c30c30c30c30c3ull;
+        v = (v | (v << 2)) & 0x1249249249249249ull;
+        return v;
+    };
+    std::vector<std::pair<uint64_t, uint32_t>> key(n);
+    size_t c = 0;
+    for (int z = 0; z < g.n[2]; ++z)
+        for (int y = 0; y < g.n[1]; ++y)
+            for (int x = 0; x < g.n[0]; ++x, ++c)
+                key[c] = {spread((uint64_t)x) | (spread((uint64_t)y) << 1) | (spread((uint64_t)z) << 2),
+                          (uint32_t)c};
+    std::sort(key.begin(), key.end());
+    rank.resize(n);
+    for (size_t r = 0; r < n; ++r)
+        rank[key[r].second] = (uint32_t)r;
     }
 
 // Cell index along one axis for x in [-L/2, L/2) on an n-cell grid.

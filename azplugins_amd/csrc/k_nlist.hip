@@ -22,6 +22,9 @@
 //               reruns the walk.
 //   5. columns: rows transposed to nlist[k * pitch + i] (coalesced for
 //               one-lane-per-particle consumers), padded per 64 particles.
+// With tiles enabled, steps 4 and 5 are replaced by the fused per-tile walk
+// (k_tile_walk.hip), which writes the tiled list directly; the plain list is
+// expanded from it only when a consumer asks for it.
 // Rebuild check: a particle moved farther than r_buff/2 since the last build.
 #include <algorithm>
 #include <cmath>
@@ -40,10 +43,13 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n)
         p[k] = v;
     }
 
+// cell_of[i] is the geometric cell index; the cell's members occupy slots
+// start[cell_rank[c]] .. start[cell_rank[c] + 1] (Morton order of cells).
 __global__ void k_bin(const double4* __restrict__ pos,
                       uint32_t N,
                       DevBox box,
                       CellGrid g,
+                      const uint32_t* __restrict__ cell_rank,
                       uint32_t* __restrict__ cell_of,
                       uint32_t* __restrict__ slot_of,
                       uint32_t* __restrict__ count)
@@ -57,11 +63,12 @@ __global__ void k_bin(const double4* __restrict__ pos,
     const int cz = cell_coord(p.z, box.L[2], box.Linv[2], g.n[2]);
     const uint32_t c = (uint32_t)((cz * g.n[1] + cy) * g.n[0] + cx);
     cell_of[i] = c;
-    slot_of[i] = atomicAdd(&count[c], 1u);
+    slot_of[i] = atomicAdd(&count[cell_rank[c]], 1u);
     }
 
 __global__ void k_scatter(uint32_t N,
                           const uint32_t* __restrict__ cell_of,
+                          const uint32_t* __restrict__ cell_rank,
                           const uint32_t* __restrict__ slot_of,
                           const uint32_t* __restrict__ start,
                           uint32_t* __restrict__ members)
@@ -69,16 +76,34 @@ __global__ void k_scatter(uint32_t N,
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N)
         return;
-    members[start[cell_of[i]] + slot_of[i]] = i;
+    members[start[cell_rank[cell_of[i]]] + slot_of[i]] = i;
     }
 
-// Insertion sort of each cell's members by particle index.
-__global__ void k_sort_cells(uint32_t ncell, const uint32_t* __restrict__ start, uint32_t* __restrict__ members)
+// Each cell's members sorted by particle index, so that the lists do not
+// depend on the order the binning atomics took: one wavefront per cell ranks
+// up to 64 members by counting the smaller ones; a longer cell is
+// insertion-sorted by its first lane.
+__global__ void __launch_bounds__(256) k_sort_cells(uint32_t ncell,
+                                                    const uint32_t* __restrict__ start,
+                                                    uint32_t* __restrict__ members)
     {
-    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t c = blockIdx.x * 4u + (threadIdx.x >> 6);
+    const uint32_t lane = threadIdx.x & 63u;
     if (c >= ncell)
         return;
-    const uint32_t b = start[c], e = start[c + 1];
+    const uint32_t b = start[c], e = start[c + 1], n = e - b;
+    if (n <= 64u)
+        {
+        const uint32_t v = lane < n ? members[b + lane] : 0xffffffffu;
+        uint32_t r = 0;
+        for (uint32_t k = 0; k < n; ++k)
+            r += (uint32_t)__shfl((int)v, (int)k) < v ? 1u : 0u;
+        if (lane < n)
+            members[b + r] = v;
+        return;
+        }
+    if (lane != 0)
+        return;
     for (uint32_t k = b + 1; k < e; ++k)
         {
         const uint32_t v = members[k];
@@ -125,6 +150,7 @@ struct WalkArgs
     float fx_scale[3]; // L / 2^32 per axis
     CellGrid g;
     const uint32_t* cell_of;
+    const uint32_t* cell_rank;
     const uint32_t* start;
     const uint32_t* members;
     const int4* cell_fixed;
@@ -186,7 +212,7 @@ template<bool SINGLE> __device__ uint32_t walk_one(const WalkArgs& a, uint32_t i
         for (int iy = 0; iy < ny; ++iy)
             for (int ix = 0; ix < nx; ++ix)
                 {
-                const uint32_t nb = (uint32_t)((oz[iz] * a.g.n[1] + oy[iy]) * a.g.n[0] + ox[ix]);
+                const uint32_t nb = a.cell_rank[(oz[iz] * a.g.n[1] + oy[iy]) * a.g.n[0] + ox[ix]];
                 const uint32_t b = a.start[nb], e = a.start[nb + 1];
                 // candidates 4 at a time: their loads are issued together
                 for (uint32_t m0 = b; m0 < e; m0 += 4)
@@ -352,12 +378,20 @@ struct azp_nlist
     DeviceArray<uint32_t> rows; // particle-major staging of the walk
     DeviceArray<double4> last_pos;
     DeviceArray<uint32_t> cell_of, slot_of, count, start, members, scan_sums;
-    DeviceArray<uint32_t> flags; // [0] overflow, [1] rebuild, [2] bad type
+    DeviceArray<uint32_t> cell_rank; // Morton rank of each cell of the grid below
+    int rank_grid[3] = {0, 0, 0};
+    DeviceArray<uint32_t> flags; // [0] overflow, [1] rebuild, [2] bad type, [3] tile misfit, [4] largest tile set
     uint32_t* h_flags = nullptr;  // pinned, 8 words ([4], [5] tile build flags)
 
     bool want_tiles = false;
     uint32_t tile_mode = 0; // 1: 512 else 256; 256 / 512: that size only
     TileList tiles;
+    // The fused walk writes only the tiled list; the plain list is expanded
+    // from it when a consumer asks for it (plain_ready false until then).
+    bool plain_ready = true;
+    bool fused = false; // the last build used the fused walk
+    hipStream_t last_stream = nullptr;
+    uint32_t walk_skip[2] = {0, 0}; // builds to skip the fused walk of 512 / 256 after a tile did not fit
 
     uint32_t pitch = 0;
     uint32_t max_neighbors = 0;
@@ -401,17 +435,28 @@ static int nlist_build(azp_nlist* nl, const azp_particles* p, hipStream_t s)
     AZP_TRY(nl->rank_of.reserve(N));
     AZP_TRY(nl->n_neigh.reserve(N));
     AZP_TRY(nl->last_pos.reserve(N));
-    AZP_TRY(nl->flags.reserve(3));
+    AZP_TRY(nl->flags.reserve(5));
+
+    if (g.n[0] != nl->rank_grid[0] || g.n[1] != nl->rank_grid[1] || g.n[2] != nl->rank_grid[2])
+        {
+        std::vector<uint32_t> rank;
+        morton_cell_rank(g, rank);
+        AZP_TRY(nl->cell_rank.reserve(ncell));
+        AZP_HIP(hipMemcpyAsync(nl->cell_rank.ptr, rank.data(), ncell * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+        AZP_HIP(hipStreamSynchronize(s)); // the host vector goes out of scope
+        for (int k = 0; k < 3; ++k)
+            nl->rank_grid[k] = g.n[k];
+        }
 
     const unsigned B = 256;
     const double4* pos = reinterpret_cast<const double4*>(p->d_pos);
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(ncell, B)), dim3(B), 0, s, nl->count.ptr, 0u, ncell);
-    hipLaunchKernelGGL(k_bin, dim3(grid_for(N, B)), dim3(B), 0, s, pos, N, box, g,
+    hipLaunchKernelGGL(k_bin, dim3(grid_for(N, B)), dim3(B), 0, s, pos, N, box, g, nl->cell_rank.ptr,
                        nl->cell_of.ptr, nl->slot_of.ptr, nl->count.ptr);
     enqueue_scan(nl->count.ptr, (uint32_t)ncell, nl->scan_sums.ptr, nl->start.ptr, nullptr, s);
-    hipLaunchKernelGGL(k_scatter, dim3(grid_for(N, B)), dim3(B), 0, s, N, nl->cell_of.ptr,
+    hipLaunchKernelGGL(k_scatter, dim3(grid_for(N, B)), dim3(B), 0, s, N, nl->cell_of.ptr, nl->cell_rank.ptr,
                        nl->slot_of.ptr, nl->start.ptr, nl->members.ptr);
-    hipLaunchKernelGGL(k_sort_cells, dim3(grid_for(ncell, B)), dim3(B), 0, s, (uint32_t)ncell,
+    hipLaunchKernelGGL(k_sort_cells, dim3(grid_for(ncell, 4)), dim3(256), 0, s, (uint32_t)ncell,
                        nl->start.ptr, nl->members.ptr);
     hipLaunchKernelGGL(k_cell_fixed, dim3(grid_for(N, B)), dim3(B), 0, s, N, nl->members.ptr, pos, box, nl->ntypes,
                        nl->cell_fixed.ptr, nl->rank_of.ptr);
@@ -433,6 +478,89 @@ static int nlist_build(azp_nlist* nl, const azp_particles* p, hipStream_t s)
         {
         AZP_TRY(nl->list.reserve((size_t)nl->max_neighbors * nl->pitch));
         AZP_TRY(nl->rows.reserve((size_t)nl->max_neighbors * N));
+        nl->tiles.valid = false;
+        nl->plain_ready = true;
+        nl->fused = false;
+        if (nl->want_tiles)
+            {
+            // fused walk: the tiled list straight from the cell list, 512-
+            // particle tiles, else 256
+            bool grown = false, done = false;
+            for (uint32_t ts : {512u, 256u})
+                {
+                if (nl->tile_mode != 1 && nl->tile_mode != ts)
+                    continue;
+                uint32_t& skip = nl->walk_skip[ts == 512u ? 0 : 1];
+                if (skip > 0)
+                    {
+                    --skip;
+                    continue;
+                    }
+                hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, nl->flags.ptr, 0u, (size_t)5);
+                TileWalkArgs ta;
+                ta.pos = pos;
+                ta.N = N;
+                ta.box = box;
+                for (int k = 0; k < 3; ++k)
+                    ta.fx_scale[k] = (float)(box.L[k] / 4294967296.0);
+                ta.g = g;
+                ta.cell_of = nl->cell_of.ptr;
+                ta.cell_rank = nl->cell_rank.ptr;
+                ta.start = nl->start.ptr;
+                ta.members = nl->members.ptr;
+                ta.cell_fixed = nl->cell_fixed.ptr;
+                ta.rank_of = nl->rank_of.ptr;
+                ta.r_listsq = nl->r_listsq.ptr;
+                ta.r_listsq_lo = nl->r_listsq_lo.ptr;
+                ta.r_listsq_hi = nl->r_listsq_hi.ptr;
+                ta.ntypes = nl->ntypes;
+                ta.n_neigh = nl->n_neigh.ptr;
+                ta.rows = reinterpret_cast<uint16_t*>(nl->rows.ptr);
+                ta.max_neighbors = nl->max_neighbors;
+                ta.flags = nl->flags.ptr;
+                ta.pitch = nl->pitch;
+                AZP_TRY(tiles_walk(nl->tiles, ts, ta, s));
+                AZP_HIP(hipMemcpyAsync(nl->h_flags, nl->flags.ptr, 5 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+                AZP_HIP(hipStreamSynchronize(s));
+                if (nl->h_flags[2] & 2u)
+                    return set_error(AZP_ERR_INVALID, "a particle position is not finite (NaN or inf)");
+                if (nl->h_flags[2] & 1u)
+                    return set_error(AZP_ERR_INVALID,
+                                     "a particle type index is outside [0, %u) (pos.w must hold the type)",
+                                     nl->ntypes);
+                if (nl->h_flags[0] != 0u)
+                    {
+                    grown = true;
+                    break;
+                    }
+                if (nl->h_flags[3] == 0u)
+                    {
+                    nl->tiles.size = ts;
+                    nl->tiles.slots = nl->h_flags[4] + 1u;
+                    nl->tiles.valid = true;
+                    nl->plain_ready = false;
+                    nl->fused = true;
+                    done = true;
+                    break;
+                    }
+                skip = 8; // a tile did not fit: plain walk for the next builds
+                }
+            if (grown)
+                {
+                nl->max_neighbors = ((uint32_t)(nl->h_flags[0] * 1.1) + 8u) & ~7u;
+                continue;
+                }
+            if (done)
+                {
+                hipLaunchKernelGGL(k_copy_d4, dim3(grid_for(N, B)), dim3(B), 0, s, pos, nl->last_pos.ptr, N);
+                AZP_LAUNCH_CHECK();
+                nl->built_N = N;
+                nl->built_box = p->box;
+                nl->have_list = true;
+                nl->n_builds++;
+                return AZP_OK;
+                }
+            }
         hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, nl->flags.ptr, 0u, (size_t)3);
         WalkArgs wa;
         wa.pos = pos;
@@ -442,6 +570,7 @@ static int nlist_build(azp_nlist* nl, const azp_particles* p, hipStream_t s)
             wa.fx_scale[k] = (float)(box.L[k] / 4294967296.0);
         wa.g = g;
         wa.cell_of = nl->cell_of.ptr;
+        wa.cell_rank = nl->cell_rank.ptr;
         wa.start = nl->start.ptr;
         wa.members = nl->members.ptr;
         wa.cell_fixed = nl->cell_fixed.ptr;
@@ -474,10 +603,10 @@ static int nlist_build(azp_nlist* nl, const azp_particles* p, hipStream_t s)
             {
             hipLaunchKernelGGL(k_copy_d4, dim3(grid_for(N, B)), dim3(B), 0, s, pos, nl->last_pos.ptr, N);
             AZP_LAUNCH_CHECK();
-            nl->tiles.valid = false;
             if (nl->want_tiles)
                 {
-                // 512-particle tiles, else 256; without either the plain list serves
+                // tiles from the plain list: 512-particle tiles, else 256;
+                // without either the plain list serves
                 for (uint32_t ts : {512u, 256u})
                     {
                     if (nl->tile_mode != 1 && nl->tile_mode != ts)
@@ -568,6 +697,7 @@ extern "C" int azp_nlist_update(azp_nlist* nl, const azp_particles* p, int force
                              L[k],
                              rmax);
     hipStream_t s = (hipStream_t)stream;
+    nl->last_stream = s;
     if (nl->params_dirty)
         {
         const size_t nt2 = (size_t)nl->ntypes * nl->ntypes;
@@ -616,6 +746,9 @@ extern "C" int azp_nlist_update(azp_nlist* nl, const azp_particles* p, int force
         return AZP_OK;
     if (p->N == 0)
         {
+        nl->fused = false;
+        nl->plain_ready = true;
+        nl->tiles.valid = false;
         nl->built_N = 0;
         nl->built_box = p->box;
         nl->have_list = true;
@@ -632,6 +765,48 @@ extern "C" int azp_nlist_update(azp_nlist* nl, const azp_particles* p, int force
     return AZP_OK;
     }
 
+// Expands the tiled list into the plain one on stream s if that has not
+// happened since the last build.
+static int nlist_expand(const azp_nlist* cnl, hipStream_t s)
+    {
+    azp_nlist* nl = const_cast<azp_nlist*>(cnl);
+    if (nl->plain_ready)
+        return AZP_OK;
+    AZP_TRY(tiles_to_plain(nl->tiles, nl->built_N, nl->n_neigh.ptr, nl->rank_of.ptr, nl->pitch, nl->list.ptr, s));
+    nl->plain_ready = true;
+    return AZP_OK;
+    }
+
+namespace azp
+{
+// The list arrays for work enqueued on stream s.  With need_plain false the
+// plain list is returned only if it is current (else null: a tiled consumer).
+int nlist_get_on_stream(const azp_nlist* nl,
+                        hipStream_t s,
+                        bool need_plain,
+                        const uint32_t** d_n_neigh,
+                        const uint32_t** d_nlist,
+                        uint32_t* pitch,
+                        uint32_t* max_neighbors)
+    {
+    if (!nl)
+        return set_error(AZP_ERR_INVALID, "null nlist");
+    if (!nl->have_list)
+        return set_error(AZP_ERR_STATE, "neighbor list not built");
+    if (need_plain)
+        AZP_TRY(nlist_expand(nl, s));
+    if (d_n_neigh)
+        *d_n_neigh = nl->n_neigh.ptr;
+    if (d_nlist)
+        *d_nlist = nl->plain_ready ? nl->list.ptr : nullptr;
+    if (pitch)
+        *pitch = nl->pitch;
+    if (max_neighbors)
+        *max_neighbors = nl->max_neighbors;
+    return AZP_OK;
+    }
+} // namespace azp
+
 extern "C" int azp_nlist_get(const azp_nlist* nl,
                              const uint32_t** d_n_neigh,
                              const uint32_t** d_nlist,
@@ -642,15 +817,13 @@ extern "C" int azp_nlist_get(const azp_nlist* nl,
         return set_error(AZP_ERR_INVALID, "null nlist");
     if (!nl->have_list)
         return set_error(AZP_ERR_STATE, "neighbor list not built");
-    if (d_n_neigh)
-        *d_n_neigh = nl->n_neigh.ptr;
-    if (d_nlist)
-        *d_nlist = nl->list.ptr;
-    if (pitch)
-        *pitch = nl->pitch;
-    if (max_neighbors)
-        *max_neighbors = nl->max_neighbors;
-    return AZP_OK;
+    if (!nl->plain_ready)
+        {
+        // expanded on the build's stream, finished before the pointers are handed out
+        AZP_TRY(nlist_expand(nl, nl->last_stream));
+        AZP_HIP(hipStreamSynchronize(nl->last_stream));
+        }
+    return nlist_get_on_stream(nl, nl->last_stream, true, d_n_neigh, d_nlist, pitch, max_neighbors);
     }
 
 extern "C" int azp_nlist_set_tiled(azp_nlist* nl, int mode)
@@ -661,6 +834,12 @@ extern "C" int azp_nlist_set_tiled(azp_nlist* nl, int mode)
         return set_error(AZP_ERR_INVALID, "tile mode %d must be 0, 1, 256 or 512", mode);
     if ((uint32_t)mode != nl->tile_mode)
         nl->params_dirty = true; // next update rebuilds (and tiles) the list
+    if (mode == 0 && nl->have_list && !nl->plain_ready)
+        {
+        // the plain list stays usable until that rebuild
+        AZP_TRY(nlist_expand(nl, nl->last_stream));
+        AZP_HIP(hipStreamSynchronize(nl->last_stream));
+        }
     nl->tile_mode = (uint32_t)mode;
     nl->want_tiles = mode != 0;
     if (!nl->want_tiles)
@@ -691,6 +870,26 @@ extern "C" int azp_nlist_get_tiles(const azp_nlist* nl,
     return AZP_OK;
     }
 
+extern "C" int azp_nlist_get_tile_order(const azp_nlist* nl, const uint32_t** d_tile_order, const uint32_t** d_tile_nn)
+    {
+    if (!nl)
+        return set_error(AZP_ERR_INVALID, "null nlist");
+    const bool ok = nl->want_tiles && nl->tiles.valid && nl->have_list && nl->tiles.order;
+    if (d_tile_order)
+        *d_tile_order = ok ? nl->tiles.order : nullptr;
+    if (d_tile_nn)
+        *d_tile_nn = ok ? nl->tiles.nn.ptr : nullptr;
+    return AZP_OK;
+    }
+
+extern "C" int azp_nlist_walk_kind(const azp_nlist* nl, int* kind)
+    {
+    if (!nl || !kind)
+        return set_error(AZP_ERR_INVALID, "null argument");
+    *kind = nl->fused ? 1 : 0;
+    return AZP_OK;
+    }
+
 extern "C" int azp_nlist_num_builds(const azp_nlist* nl, uint64_t* n)
     {
     if (!nl || !n)
@@ -716,6 +915,7 @@ extern "C" int azp_nlist_download(const azp_nlist* nl,
         return set_error(AZP_ERR_INVALID, "host list needs %llu entries", (unsigned long long)need);
     if (N == 0)
         return AZP_OK;
+    AZP_TRY(nlist_expand(nl, nl->last_stream));
     AZP_HIP(hipDeviceSynchronize());
     AZP_HIP(hipMemcpy(h_n_neigh, nl->n_neigh.ptr, (size_t)N * sizeof(uint32_t), hipMemcpyDeviceToHost));
     AZP_HIP(hipMemcpy(h_nlist, nl->list.ptr, (size_t)need * sizeof(uint32_t), hipMemcpyDeviceToHost));

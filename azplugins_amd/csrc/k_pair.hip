@@ -49,10 +49,14 @@ struct PairKernelArgs
     double4* force;
     double* virial;
     uint32_t vpitch;
-    // tiled list (mixed form): per-tile neighbor sets and 16-bit entries
+    // tiled list (mixed form): per-tile neighbor sets and 16-bit entries;
+    // for tiles of cell-list slots, column s is particle tile_order[s] with
+    // tile_nn[s] entries (both null for tiles of consecutive particles)
     const uint32_t* tile_count;
     const uint32_t* tile_jset;
     const uint32_t* tile_list;
+    const uint32_t* tile_order;
+    const uint32_t* tile_nn;
     };
 
 // Position (double4 x, y, z, type) -> wrapped 32-bit fixed point of the box.
@@ -334,6 +338,15 @@ template<class Eval, typename R> __host__ __device__ constexpr size_t table_byte
     return (nt2 * (sizeof(typename Eval::template Prep<R>) + sizeof(TypePairExtra<R>)) + 15) & ~(size_t)15;
     }
 
+// LDS the tiled kernel may use for its type-pair tables and neighbor slots.
+constexpr size_t kTileLds = 64 * 1024;
+
+template<class Eval> static bool tile_fits(uint32_t ntypes, uint32_t slots)
+    {
+    const size_t tables = ntypes == 1 ? 0 : table_bytes<Eval, float>(ntypes * ntypes);
+    return tables + (size_t)slots * sizeof(int4) <= kTileLds;
+    }
+
 template<class Eval, bool MIXED, bool VIRIAL, bool SINGLE, bool XPLOR, bool CUBIC, bool TILED>
 __global__ void __launch_bounds__(512) k_pair(const PairKernelArgs a)
     {
@@ -384,7 +397,10 @@ __global__ void __launch_bounds__(512) k_pair(const PairKernelArgs a)
     if (lb * blockDim.x + (threadIdx.x & ~63u) >= a.N)
         return;
     const uint32_t ie = min(i, a.N - 1);
-    const uint32_t nn = a.n_neigh[ie];
+    // slot-ordered tiles: column ie belongs to particle pe
+    const bool by_slot = TILED && a.tile_order != nullptr;
+    const uint32_t pe = by_slot ? a.tile_order[ie] : ie;
+    const uint32_t nn = by_slot ? a.tile_nn[ie] : a.n_neigh[ie];
     // wave-uniform trip count, held in a scalar register
     const uint32_t P = __builtin_amdgcn_readfirstlane((wave_max(nn) + U - 1) & ~(U - 1));
     const size_t stride = a.nlist_pitch;
@@ -420,7 +436,7 @@ __global__ void __launch_bounds__(512) k_pair(const PairKernelArgs a)
         }
     else if constexpr (TILED)
         {
-        const int4 pi = fixed4(a.pos[ie], a.box);
+        const int4 pi = fixed4(a.pos[pe], a.box);
         const uint32_t ti = (uint32_t)pi.w;
         const uint32_t col_off = ie * 4u;
         Acc32 part;
@@ -449,7 +465,7 @@ __global__ void __launch_bounds__(512) k_pair(const PairKernelArgs a)
                 {
                 const uint32_t tp = SINGLE ? 0u : ti * nt + (uint32_t)pj[u].w;
                 term_mixed<Eval, VIRIAL, XPLOR, CUBIC>(a, SINGLE ? q0 : s_prep[tp], SINGLE ? x0 : s_x[tp],
-                                                       e[u] != tile_m, ie, e[u], tile_js,
+                                                       e[u] != tile_m, pe, e[u], tile_js,
                                                        fixed_delta(pi.x, pj[u].x), fixed_delta(pi.y, pj[u].y),
                                                        fixed_delta(pi.z, pj[u].z), part);
                 }
@@ -460,7 +476,7 @@ __global__ void __launch_bounds__(512) k_pair(const PairKernelArgs a)
                 }
             }
         if (i < a.N)
-            store_scaled<VIRIAL>(a, i, acc);
+            store_scaled<VIRIAL>(a, pe, acc);
         }
     else
         {
@@ -520,7 +536,7 @@ static int launch_t(const PairKernelArgs& ka, unsigned block, unsigned slots, hi
     size_t shmem = SINGLE ? 0 : table_bytes<Eval, R>((uint32_t)nt2);
     if (TILED)
         shmem += (size_t)slots * sizeof(int4);
-    if (shmem > 64 * 1024)
+    if (shmem > kTileLds)
         return set_error(AZP_ERR_UNSUPPORTED, "too many particle types (%u) for LDS staging", ka.ntypes);
     const unsigned grid = grid_for(ka.N, block);
     hipLaunchKernelGGL((k_pair<Eval, MIXED, VIRIAL, SINGLE, XPLOR, CUBIC, TILED>), dim3(grid), dim3(block), shmem,
@@ -572,10 +588,14 @@ static int launch_eval(const PairKernelArgs& ka,
     {
     if constexpr (MIXED)
         {
-        const size_t nt2 = (size_t)ka.ntypes * ka.ntypes;
-        const size_t tables = ka.ntypes == 1 ? 0 : table_bytes<Eval, float>((uint32_t)nt2);
-        if (tile_size && ka.tile_list && tables + (size_t)tile_slots * sizeof(int4) <= 64 * 1024)
+        if (tile_size && ka.tile_list && tile_fits<Eval>(ka.ntypes, tile_slots))
             return launch_virial<Eval, true, true>(ka, virial, tile_size, tile_slots, s);
+        if (!ka.nlist)
+            return set_error(AZP_ERR_INVALID,
+                             "the tiled list's %u neighbor slots and the type tables exceed %u bytes of LDS; "
+                             "pass the plain list",
+                             tile_slots,
+                             (unsigned)kTileLds);
         if (!ka.pos_fixed)
             return set_error(AZP_ERR_INVALID, "mixed precision without a tiled list needs d_pos_fixed");
         }
@@ -600,6 +620,25 @@ int pack_fixed(const double* d_pos, uint32_t N, const azp_box& box, int32_t* d_o
     return AZP_OK;
     }
 
+// Whether the mixed kernel of `evaluator` can run over a tiled list with
+// `tile_slots` neighbor slots for `ntypes` types.
+bool pair_tile_fits(int evaluator, uint32_t ntypes, uint32_t tile_slots)
+    {
+    switch (evaluator)
+        {
+    case AZP_EVAL_HERTZ:
+        return tile_fits<EvalHertz>(ntypes, tile_slots);
+    case AZP_EVAL_EXPANDED_YUKAWA:
+        return tile_fits<EvalExpandedYukawa>(ntypes, tile_slots);
+    case AZP_EVAL_PERTURBED_LJ:
+        return tile_fits<EvalPerturbedLJ>(ntypes, tile_slots);
+    case AZP_EVAL_DPD_GENERAL_WEIGHT:
+        return tile_fits<EvalDPDConservative>(ntypes, tile_slots);
+    default:
+        return false;
+        }
+    }
+
 int validate_common(const azp_pair_args* args)
     {
     if (!args)
@@ -607,8 +646,10 @@ int validate_common(const azp_pair_args* args)
     const azp_particles& p = args->particles;
     if (p.N == 0)
         return AZP_OK;
-    if (!p.d_pos || !args->out.d_force || !args->d_n_neigh || !args->d_nlist || !args->d_rcutsq)
+    if (!p.d_pos || !args->out.d_force || !args->d_n_neigh || !args->d_rcutsq)
         return set_error(AZP_ERR_INVALID, "missing device array in pair args");
+    if (!args->d_nlist && !args->d_tile_list)
+        return set_error(AZP_ERR_INVALID, "missing neighbor list in pair args");
     if (!box_valid(p.box))
         return set_error(AZP_ERR_BOX, "box lengths must be positive");
     if (args->ntypes == 0)
@@ -679,11 +720,15 @@ extern "C" int azp_gpu_compute_pair_forces(const azp_pair_args* args, int evalua
     ka.tile_count = args->d_tile_count;
     ka.tile_jset = args->d_tile_jset;
     ka.tile_list = args->d_tile_list;
+    ka.tile_order = args->d_tile_list ? args->d_tile_order : nullptr;
+    ka.tile_nn = args->d_tile_list ? args->d_tile_nn : nullptr;
     unsigned tile = 0, slots = 0;
     if (args->d_tile_list)
         {
         if (!args->d_tile_count || !args->d_tile_jset)
             return set_error(AZP_ERR_INVALID, "a tiled list needs its counts, neighbor sets and entries");
+        if ((args->d_tile_order == nullptr) != (args->d_tile_nn == nullptr))
+            return set_error(AZP_ERR_INVALID, "slot-ordered tiles need both d_tile_order and d_tile_nn");
         if (args->tile_size != 256 && args->tile_size != 512)
             return set_error(AZP_ERR_INVALID, "tile size %u must be 256 or 512", args->tile_size);
         if (args->tile_slots == 0 || args->tile_slots > kTileCap)

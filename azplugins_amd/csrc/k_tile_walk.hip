@@ -1,0 +1,575 @@
+// Fused neighbor walk and tile build.  Tiles are runs of T = 512 or 256
+// consecutive slots of the cell list, whose cells are stored in Morton order,
+// so a tile covers a compact block of space whatever order the particles
+// are in; slot s holds particle members[s].  One workgroup per tile writes
+// the tiled list (tiles.h) straight from the cell list:
+//
+//   1. union:   the distinct cells of the tile's particles and their 27
+//               neighbor cells, collected in an LDS hash and sorted by Morton
+//               rank; their member counts give each cell's offset in LDS,
+//   2. stage:   the members' fixed-point positions (the cell-ordered copy,
+//               16 bytes each, contiguous per cell) are copied into LDS, so
+//               the tile's candidates are read from global memory once,
+//   3. walk:    each lane tests its particle against the candidates of its
+//               27 cells from LDS (the same FP32 test with the FP64 decision
+//               near r_list as the plain walk, so the pair set is the same)
+//               and appends the candidate's LDS index to its own row of
+//               16-bit entries; every listed candidate is marked,
+//   4. compact: an exclusive scan of the marks gives each neighbor its rank
+//               in the tile's neighbor set (Morton cell order), whose
+//               particle indices go to jset,
+//   5. columns: each lane rereads its row and writes ranks two per word in
+//               column s of the tiled list, padded with the tile's count.
+//
+// A tile whose cells hold more candidates than LDS does, or whose neighbor
+// set exceeds kTileCap - 1, reports it; the host then uses the plain walk.
+#include "cells.h"
+#include "device_common.h"
+#include "tiles.h"
+
+namespace azp
+{
+
+template<uint32_t T> struct TileWalkShape;
+// candidates staged per tile: 96 KB (T = 512) or 64 KB (T = 256) of LDS
+template<> struct TileWalkShape<512>
+    {
+    static constexpr uint32_t cand = 6144;
+    };
+template<> struct TileWalkShape<256>
+    {
+    static constexpr uint32_t cand = 4096;
+    };
+
+constexpr uint32_t kUnionCells = 1024; // distinct cells around one tile
+constexpr uint32_t kCellHash = 2048;   // open-addressing table of those cells
+constexpr uint32_t kNoCell = 0xffffffffu;
+constexpr uint16_t kUsed = 0x8000u;    // mark bit beside a 15-bit rank
+
+__device__ __forceinline__ uint32_t cell_hash(uint32_t c)
+    {
+    return (c * 2654435761u) >> 21; // 11 bits
+    }
+
+// Inserts cell c; returns true when this call added it.  *full is set when
+// the table has no free slot left for it.
+__device__ __forceinline__ bool cell_insert(uint32_t* keys, uint32_t c, bool* full)
+    {
+    uint32_t h = cell_hash(c);
+    for (uint32_t p = 0; p < kCellHash; ++p)
+        {
+        const uint32_t prev = atomicCAS(&keys[h], kNoCell, c);
+        if (prev == kNoCell)
+            return true;
+        if (prev == c)
+            return false;
+        h = (h + 1u) & (kCellHash - 1u);
+        }
+    *full = true;
+    return false;
+    }
+
+__device__ __forceinline__ uint32_t cell_find(const uint32_t* keys, uint32_t c)
+    {
+    uint32_t h = cell_hash(c);
+    for (uint32_t p = 0; p < kCellHash; ++p)
+        {
+        const uint32_t k = keys[h];
+        if (k == c || k == kNoCell)
+            return h;
+        h = (h + 1u) & (kCellHash - 1u);
+        }
+    return 0u; // not reached: every looked-up cell was inserted
+    }
+
+// Exclusive scan of one value per thread over the T-thread workgroup.
+template<uint32_t T> __device__ __forceinline__ uint32_t tile_scan(uint32_t v, uint32_t* s_wave, uint32_t& total)
+    {
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    uint32_t x = v;
+    for (uint32_t off = 1; off < 64; off <<= 1)
+        {
+        const uint32_t y = (uint32_t)__shfl_up((int)x, off);
+        if (lane >= off)
+            x += y;
+        }
+    if (lane == 63u)
+        s_wave[w] = x;
+    __syncthreads();
+    uint32_t base = 0, tot = 0;
+    for (uint32_t k = 0; k < T / 64; ++k)
+        {
+        const uint32_t sw = s_wave[k];
+        base += k < w ? sw : 0u;
+        tot += sw;
+        }
+    __syncthreads();
+    total = tot;
+    return base + x - v;
+    }
+
+struct TileWalkLds
+    {
+    const int4* cand;
+    uint16_t* mark;
+    const uint32_t* key;
+    const uint16_t* val;
+    const uint32_t* off;
+    const uint32_t* gs; // first cell-list slot of each union cell
+    };
+
+// Lists particle i against the staged candidates; returns its neighbor
+// count.  Entries are LDS candidate indices, buffered eight at a time in the
+// lane's LDS slot and stored to its row as 16-byte chunks.
+template<bool SINGLE>
+__device__ uint32_t tile_walk_one(const TileWalkArgs& a,
+                                  const TileWalkLds& s,
+                                  uint32_t slot,
+                                  uint32_t i,
+                                  uint32_t c,
+                                  uint4* lslot)
+    {
+    const double4 pi = a.pos[i];
+    const uint32_t ti = checked_type(pi.w, a.ntypes);
+    const bool finite = isfinite(pi.x) && isfinite(pi.y) && isfinite(pi.z);
+    if (ti == a.ntypes || !finite)
+        {
+        atomicOr(a.flags + 2, finite ? 1u : 2u);
+        return 0;
+        }
+    const int32_t fxi = to_fixed(pi.x, a.box.Linv[0]);
+    const int32_t fyi = to_fixed(pi.y, a.box.Linv[1]);
+    const int32_t fzi = to_fixed(pi.z, a.box.Linv[2]);
+    const uint32_t u_own = s.val[cell_find(s.key, c)];
+    const uint32_t self = s.off[u_own] + (slot - s.gs[u_own]);
+    const int nx = a.g.n[0], ny = a.g.n[1], nz = a.g.n[2];
+    const int cx = (int)(c % (uint32_t)nx);
+    const int cy = (int)((c / (uint32_t)nx) % (uint32_t)ny);
+    const int cz = (int)(c / (uint32_t)(nx * ny));
+    // offsets -1, 0, +1 per axis; on a grid of one or two cells the offsets
+    // that repeat a cell are dropped (the order of stencil_axis)
+    const int ex = min(nx, 3) - 2, ey = min(ny, 3) - 2, ez = min(nz, 3) - 2;
+    const double rl0 = a.r_listsq[0];
+    const float lo0 = a.r_listsq_lo[0], hi0 = a.r_listsq_hi[0];
+    uint16_t* lbuf = reinterpret_cast<uint16_t*>(lslot);
+    uint16_t* row = a.rows + (size_t)slot * a.max_neighbors;
+    uint32_t count = 0;
+    for (int oz = -1; oz <= ez; ++oz)
+        for (int oy = -1; oy <= ey; ++oy)
+            for (int ox = -1; ox <= ex; ++ox)
+                {
+                const int x = (cx + ox + nx) % nx, y = (cy + oy + ny) % ny, z = (cz + oz + nz) % nz;
+                const uint32_t nb = (uint32_t)((z * ny + y) * nx + x);
+                const uint32_t u = s.val[cell_find(s.key, nb)];
+                const uint32_t b = s.off[u], e = s.off[u + 1];
+                for (uint32_t m0 = b; m0 < e; m0 += 4)
+                    {
+                    int4 pc[4];
+#pragma unroll
+                    for (uint32_t q = 0; q < 4; ++q)
+                        pc[q] = s.cand[min(m0 + q, e - 1)];
+#pragma unroll
+                    for (uint32_t q = 0; q < 4; ++q)
+                        {
+                        const uint32_t m = m0 + q;
+                        const int4 pj = pc[q];
+                        const uint32_t tj = (uint32_t)pj.w;
+                        if (m >= e || m == self || tj >= a.ntypes)
+                            continue;
+                        const uint32_t tp = SINGLE ? 0u : ti * a.ntypes + tj;
+                        const float lo = SINGLE ? lo0 : a.r_listsq_lo[tp];
+                        const float hi = SINGLE ? hi0 : a.r_listsq_hi[tp];
+                        const float dx = (float)fixed_delta(fxi, pj.x) * a.fx_scale[0];
+                        const float dy = (float)fixed_delta(fyi, pj.y) * a.fx_scale[1];
+                        const float dz = (float)fixed_delta(fzi, pj.z) * a.fx_scale[2];
+                        const float rsq = dx * dx + dy * dy + dz * dz;
+                        bool in = rsq < lo;
+                        if (!in && rsq <= hi)
+                            {
+                            const uint32_t j = a.members[s.gs[u] + (m - b)];
+                            const double4 p64 = a.pos[j];
+                            const double rl = SINGLE ? rl0 : a.r_listsq[tp];
+                            const double ex = min_image_1d(pi.x - p64.x, a.box.L[0], a.box.Linv[0]);
+                            const double ey = min_image_1d(pi.y - p64.y, a.box.L[1], a.box.Linv[1]);
+                            const double ez = min_image_1d(pi.z - p64.z, a.box.L[2], a.box.Linv[2]);
+                            in = rsq3_nofma(ex, ey, ez) <= rl;
+                            }
+                        if (in)
+                            {
+                            s.mark[m] = 1u;
+                            lbuf[count & 7u] = (uint16_t)m;
+                            if ((count & 7u) == 7u && count < a.max_neighbors)
+                                *reinterpret_cast<uint4*>(row + (count & ~7u)) = *lslot;
+                            ++count;
+                            }
+                        }
+                    }
+                }
+    if ((count & 7u) != 0u && count < a.max_neighbors)
+        *reinterpret_cast<uint4*>(row + (count & ~7u)) = *lslot;
+    return count;
+    }
+
+template<uint32_t T, bool SINGLE> __global__ void __launch_bounds__(T) k_tile_walk(const TileWalkArgs a)
+    {
+    constexpr uint32_t C = TileWalkShape<T>::cand;
+    __shared__ int4 s_cand[C];
+    __shared__ uint16_t s_mark[C];
+    __shared__ uint32_t s_key[kCellHash];
+    __shared__ uint16_t s_val[kCellHash];
+    __shared__ uint64_t s_ucell[kUnionCells];
+    __shared__ uint32_t s_off[kUnionCells + 1];
+    __shared__ uint32_t s_gs[kUnionCells];
+    __shared__ uint4 s_lbuf[T];
+    __shared__ uint32_t s_wave[T / 64];
+    __shared__ uint32_t s_n, s_bad;
+
+    const uint32_t tid = threadIdx.x, lane = tid & 63u;
+    const uint32_t t = xcd_logical_block();
+    const uint32_t slot = t * T + tid;
+    const bool active = slot < a.N;
+    const uint32_t nx = (uint32_t)a.g.n[0], ny = (uint32_t)a.g.n[1], nz = (uint32_t)a.g.n[2];
+
+    for (uint32_t k = tid; k < kCellHash; k += T)
+        s_key[k] = kNoCell;
+    if (tid == 0)
+        {
+        s_n = 0u;
+        s_bad = 0u;
+        }
+    __syncthreads();
+
+    // 1. the tile's own cells (distinct) head the union list
+    const uint32_t i = active ? a.members[slot] : 0u;
+    const uint32_t c = active ? a.cell_of[i] : 0u;
+    if (active)
+        {
+        bool full = false;
+        if (cell_insert(s_key, c, &full))
+            {
+            const uint32_t k = atomicAdd(&s_n, 1u);
+            if (k < kUnionCells)
+                s_ucell[k] = ((uint64_t)a.cell_rank[c] << 32) | c;
+            }
+        if (full)
+            s_bad = 1u;
+        }
+    __syncthreads();
+    const uint32_t D = min(s_n, kUnionCells);
+    __syncthreads();
+    // ... followed by their neighbor cells (periodic; repeats on grids
+    // narrower than 3 cells are removed by the hash)
+    for (uint32_t q = tid; q < D * 27u; q += T)
+        {
+        const uint32_t oc = (uint32_t)s_ucell[q / 27u];
+        const uint32_t sidx = q % 27u;
+        const uint32_t ox = oc % nx, oy = (oc / nx) % ny, oz = oc / (nx * ny);
+        const uint32_t x = (ox + nx + sidx % 3u - 1u) % nx;
+        const uint32_t y = (oy + ny + (sidx / 3u) % 3u - 1u) % ny;
+        const uint32_t z = (oz + nz + sidx / 9u - 1u) % nz;
+        const uint32_t nb = (z * ny + y) * nx + x;
+        bool full = false;
+        if (cell_insert(s_key, nb, &full))
+            {
+            const uint32_t k = atomicAdd(&s_n, 1u);
+            if (k < kUnionCells)
+                s_ucell[k] = ((uint64_t)a.cell_rank[nb] << 32) | nb;
+            }
+        if (full)
+            s_bad = 1u;
+        }
+    __syncthreads();
+    const uint32_t U = s_n;
+    if (s_bad != 0u || U > kUnionCells)
+        {
+        if (tid == 0)
+            atomicOr(a.flags + 3, 1u);
+        return; // the whole workgroup leaves (U and s_bad are shared)
+        }
+
+    // Morton order: bitonic sort over the next power of two
+    uint32_t P2 = 1;
+    while (P2 < U)
+        P2 <<= 1;
+    for (uint32_t k = U + tid; k < P2; k += T)
+        s_ucell[k] = ~0ull;
+    __syncthreads();
+    for (uint32_t size = 2; size <= P2; size <<= 1)
+        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1)
+            {
+            for (uint32_t e = tid; e < P2; e += T)
+                {
+                const uint32_t p = e ^ stride;
+                if (p > e)
+                    {
+                    const bool up = (e & size) == 0u;
+                    const uint64_t x = s_ucell[e], y = s_ucell[p];
+                    if ((x > y) == up)
+                        {
+                        s_ucell[e] = y;
+                        s_ucell[p] = x;
+                        }
+                    }
+                }
+            __syncthreads();
+            }
+
+    // 2. LDS offsets of the union cells' members
+    constexpr uint32_t QC = kUnionCells / T;
+    uint32_t cnt[QC];
+    uint32_t sum = 0;
+#pragma unroll
+    for (uint32_t q = 0; q < QC; ++q)
+        {
+        const uint32_t u = tid * QC + q;
+        cnt[q] = 0u;
+        if (u < U)
+            {
+            const uint32_t r = (uint32_t)(s_ucell[u] >> 32);
+            const uint32_t b = a.start[r];
+            s_gs[u] = b;
+            cnt[q] = a.start[r + 1] - b;
+            }
+        sum += cnt[q];
+        }
+    uint32_t Mc;
+    uint32_t run = tile_scan<T>(sum, s_wave, Mc);
+#pragma unroll
+    for (uint32_t q = 0; q < QC; ++q)
+        {
+        const uint32_t u = tid * QC + q;
+        if (u < U)
+            {
+            s_off[u] = run;
+            s_val[cell_find(s_key, (uint32_t)s_ucell[u])] = (uint16_t)u;
+            }
+        run += cnt[q];
+        }
+    if (tid == 0)
+        s_off[U] = Mc;
+    if (Mc > C)
+        {
+        if (tid == 0)
+            atomicOr(a.flags + 3, 2u);
+        return; // Mc is the same for every thread
+        }
+    __syncthreads();
+    // stage the candidates: one wavefront per cell, four cells' loads in
+    // flight at a time (cells past 64 members finish in a second loop)
+    constexpr uint32_t NW = T / 64;
+    auto stage_load = [&](uint32_t u) {
+        return (u < U && lane < s_off[u + 1] - s_off[u]) ? a.cell_fixed[s_gs[u] + lane] : make_int4(0, 0, 0, 0);
+    };
+    auto stage_store = [&](uint32_t u, int4 v) {
+        if (u >= U)
+            return;
+        const uint32_t o = s_off[u], n = s_off[u + 1] - o;
+        if (lane < n)
+            s_cand[o + lane] = v;
+        for (uint32_t k = lane + 64; k < n; k += 64)
+            s_cand[o + k] = a.cell_fixed[s_gs[u] + k];
+    };
+    for (uint32_t u0 = tid >> 6; u0 < U; u0 += 4 * NW)
+        {
+        const int4 v0 = stage_load(u0), v1 = stage_load(u0 + NW);
+        const int4 v2 = stage_load(u0 + 2 * NW), v3 = stage_load(u0 + 3 * NW);
+        stage_store(u0, v0);
+        stage_store(u0 + NW, v1);
+        stage_store(u0 + 2 * NW, v2);
+        stage_store(u0 + 3 * NW, v3);
+        }
+    for (uint32_t m = tid; m < Mc; m += T)
+        s_mark[m] = 0u;
+    __syncthreads();
+
+    // 3. walk
+    uint32_t count = 0;
+    if (active)
+        {
+        const TileWalkLds s = {s_cand, s_mark, s_key, s_val, s_off, s_gs};
+        count = tile_walk_one<SINGLE>(a, s, slot, i, c, &s_lbuf[tid]);
+        const uint32_t stored = min(count, a.max_neighbors);
+        a.n_neigh[i] = stored;
+        a.tile_nn[slot] = stored;
+        if (count > a.max_neighbors)
+            atomicMax(a.flags, count);
+        }
+    __syncthreads();
+
+    // 4. ranks of the listed candidates: in-place exclusive scan of the marks
+    constexpr uint32_t QM = C / T;
+    uint32_t msum = 0;
+#pragma unroll
+    for (uint32_t q = 0; q < QM; ++q)
+        {
+        const uint32_t m = tid * QM + q;
+        msum += (m < Mc && s_mark[m] != 0u) ? 1u : 0u;
+        }
+    uint32_t M;
+    uint32_t r = tile_scan<T>(msum, s_wave, M);
+#pragma unroll
+    for (uint32_t q = 0; q < QM; ++q)
+        {
+        const uint32_t m = tid * QM + q;
+        if (m < Mc)
+            {
+            const bool used = s_mark[m] != 0u;
+            s_mark[m] = (uint16_t)(used ? (kUsed | r) : r);
+            r += used ? 1u : 0u;
+            }
+        }
+    if (M > kTileCap - 1)
+        {
+        if (tid == 0)
+            atomicOr(a.flags + 3, 4u);
+        return; // M is the same for every thread
+        }
+    __syncthreads();
+    for (uint32_t m = tid; m < Mc; m += T)
+        {
+        const uint32_t v = s_mark[m];
+        if (v & kUsed)
+            {
+            // the union cell holding candidate m: largest u with s_off[u] <= m
+            uint32_t lo = 0, hi = U;
+            while (hi - lo > 1u)
+                {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (s_off[mid] <= m)
+                    lo = mid;
+                else
+                    hi = mid;
+                }
+            a.tile_jset[(size_t)t * kTileCap + (v & ~(uint32_t)kUsed)] = a.members[s_gs[lo] + (m - s_off[lo])];
+            }
+        }
+    if (tid == 0)
+        {
+        a.tile_count[t] = M;
+        atomicMax(a.flags + 4, M);
+        }
+
+    // 5. ranks two per word, padded to the wavefront's longest list rounded
+    // up to 8 with the tile's count
+    const uint32_t nn = active ? min(count, a.max_neighbors) : 0u;
+    uint32_t wmax = nn;
+    for (int off = 32; off > 0; off >>= 1)
+        wmax = max(wmax, (uint32_t)__shfl_xor((int)wmax, off));
+    const uint32_t P = (wmax + 7u) & ~7u;
+    if (!active)
+        return;
+    // the row is reread in 16-byte chunks (8 entries), four in flight
+    const uint4* row = reinterpret_cast<const uint4*>(a.rows + (size_t)slot * a.max_neighbors);
+    auto chunk_load = [&](uint32_t kc) { return kc < nn ? row[kc / 8] : make_uint4(0u, 0u, 0u, 0u); };
+    auto rank_of_entry = [&](uint32_t k, uint32_t word, uint32_t half) {
+        const uint32_t m = (word >> (16u * half)) & 0xffffu;
+        return k < nn ? (uint32_t)(s_mark[m] & ~kUsed) : M;
+    };
+    auto chunk_store = [&](uint32_t kc, uint4 ch) {
+        if (kc >= P)
+            return;
+        const uint32_t w4[4] = {ch.x, ch.y, ch.z, ch.w};
+#pragma unroll
+        for (uint32_t w = 0; w < 4; ++w)
+            a.tlist[(size_t)((kc >> 1) + w) * a.pitch + slot] =
+                rank_of_entry(kc + 2 * w, w4[w], 0u) | (rank_of_entry(kc + 2 * w + 1, w4[w], 1u) << 16);
+    };
+    for (uint32_t k0 = 0; k0 < P; k0 += 32)
+        {
+        const uint4 c0 = chunk_load(k0), c1 = chunk_load(k0 + 8);
+        const uint4 c2 = chunk_load(k0 + 16), c3 = chunk_load(k0 + 24);
+        chunk_store(k0, c0);
+        chunk_store(k0 + 8, c1);
+        chunk_store(k0 + 16, c2);
+        chunk_store(k0 + 24, c3);
+        }
+    }
+
+// Plain column list from a slot-ordered tiled list: lane i (a particle)
+// reads column rank_of[i]; entry k is jset[tile][rank], padding entries hold
+// i up to the longest list of the 64 particles rounded up to 8.
+__global__ void __launch_bounds__(64) k_tiles_to_plain(uint32_t N,
+                                                       const uint32_t* __restrict__ n_neigh,
+                                                       const uint32_t* __restrict__ rank_of,
+                                                       uint32_t tile_size,
+                                                       const uint32_t* __restrict__ tile_jset,
+                                                       const uint32_t* __restrict__ tlist,
+                                                       uint32_t pitch,
+                                                       uint32_t* __restrict__ nlist)
+    {
+    const uint32_t i = blockIdx.x * 64u + threadIdx.x;
+    const bool active = i < N;
+    const uint32_t nn = active ? n_neigh[i] : 0u;
+    uint32_t wmax = nn;
+    for (int off = 32; off > 0; off >>= 1)
+        wmax = max(wmax, (uint32_t)__shfl_xor((int)wmax, off));
+    const uint32_t P = (wmax + 7u) & ~7u;
+    if (!active)
+        return;
+    const uint32_t s = rank_of[i];
+    const uint32_t* js = tile_jset + (size_t)(s / tile_size) * kTileCap;
+    for (uint32_t k = 0; k < P; k += 2)
+        {
+        uint32_t w = 0;
+        if (k < nn)
+            w = tlist[(size_t)(k >> 1) * pitch + s];
+        nlist[(size_t)k * pitch + i] = k < nn ? js[w & 0xffffu] : i;
+        nlist[(size_t)(k + 1) * pitch + i] = k + 1 < nn ? js[w >> 16] : i;
+        }
+    }
+
+int tiles_walk(TileList& t, uint32_t tile_size, const TileWalkArgs& args, hipStream_t s)
+    {
+    if (tile_size != 256 && tile_size != 512)
+        return set_error(AZP_ERR_INVALID, "tile size %u must be 256 or 512", tile_size);
+    const uint32_t ntiles = (args.N + tile_size - 1) / tile_size;
+    const uint32_t words = (((args.max_neighbors + 7u) & ~7u) + 1u) / 2u;
+    AZP_TRY(t.count.reserve(ntiles));
+    AZP_TRY(t.jset.reserve((size_t)ntiles * kTileCap));
+    AZP_TRY(t.list.reserve((size_t)words * args.pitch));
+    AZP_TRY(t.nn.reserve(args.N));
+    TileWalkArgs a = args;
+    a.tile_nn = t.nn.ptr;
+    a.tile_count = t.count.ptr;
+    a.tile_jset = t.jset.ptr;
+    a.tlist = t.list.ptr;
+    const bool single = a.ntypes == 1;
+    if (tile_size == 512)
+        {
+        if (single)
+            hipLaunchKernelGGL((k_tile_walk<512, true>), dim3(ntiles), dim3(512), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_tile_walk<512, false>), dim3(ntiles), dim3(512), 0, s, a);
+        }
+    else
+        {
+        if (single)
+            hipLaunchKernelGGL((k_tile_walk<256, true>), dim3(ntiles), dim3(256), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_tile_walk<256, false>), dim3(ntiles), dim3(256), 0, s, a);
+        }
+    AZP_LAUNCH_CHECK();
+    t.ntiles = ntiles;
+    t.order = args.members;
+    return AZP_OK;
+    }
+
+int tiles_to_plain(const TileList& t,
+                   uint32_t N,
+                   const uint32_t* d_n_neigh,
+                   const uint32_t* d_rank_of,
+                   uint32_t pitch,
+                   uint32_t* d_nlist,
+                   hipStream_t s)
+    {
+    if (N == 0)
+        return AZP_OK;
+    if (!t.valid || !t.order)
+        return set_error(AZP_ERR_STATE, "no slot-ordered tiled list to expand");
+    hipLaunchKernelGGL(k_tiles_to_plain, dim3((N + 63) / 64), dim3(64), 0, s, N, d_n_neigh, d_rank_of, t.size,
+                       t.jset.ptr, t.list.ptr, pitch, d_nlist);
+    AZP_LAUNCH_CHECK();
+    return AZP_OK;
+    }
+
+} // namespace azp

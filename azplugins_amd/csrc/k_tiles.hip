@@ -199,6 +199,7 @@ int tiles_build(TileList& t,
     {
     t.valid = false;
     t.size = 0;
+    t.order = nullptr;
     if (tile_size != 256 && tile_size != 512)
         return set_error(AZP_ERR_INVALID, "tile size %u must be 256 or 512", tile_size);
     if (N == 0)

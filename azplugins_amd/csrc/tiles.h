@@ -16,6 +16,7 @@
 #include <cstdint>
 
 #include "azp_internal.h"
+#include "cells.h"
 
 namespace azp
 {
@@ -31,7 +32,13 @@ struct TileList
     DeviceArray<uint32_t> jset;  // [ntiles * kTileCap]
     DeviceArray<uint32_t> list;  // [words * pitch]
     DeviceArray<uint32_t> flag;  // [0] largest distinct count above capacity, [1] largest count
-    uint32_t size = 0;           // particles per tile (0: no tiles)
+    DeviceArray<uint32_t> nn;    // slot-ordered tiles: list length of column s
+    // Slot-ordered tiles (the fused walk): column s of the list belongs to
+    // particle order[s] (the cell list's members) and has nn[s] entries.
+    // Null for particle-ordered tiles (column i is particle i, counts in the
+    // list's n_neigh).
+    const uint32_t* order = nullptr;
+    uint32_t size = 0;           // columns per tile (0: no tiles)
     uint32_t slots = 0;          // LDS slots the force kernel needs: largest count + 1
     uint32_t ntiles = 0;
     bool valid = false;
@@ -53,5 +60,56 @@ int tiles_build(TileList& t,
                 uint32_t max_neighbors,
                 uint32_t* h_flag,
                 hipStream_t stream);
+
+// Inputs and outputs of the fused walk (k_tile_walk.hip): the cell list
+// (cell_of, cell_rank, start, members, the cell-ordered fixed-point copy and
+// rank_of) and the list radii as in the plain walk; per-particle counts;
+// 16-bit slot-major rows (entry k of slot s at rows[s * max_neighbors + k],
+// LDS candidate indices) as scratch; the slot-ordered tiled list as output.
+struct TileWalkArgs
+    {
+    const double4* pos;
+    uint32_t N;
+    DevBox box;
+    float fx_scale[3]; // L / 2^32 per axis
+    CellGrid g;
+    const uint32_t* cell_of;   // geometric cell of each particle
+    const uint32_t* cell_rank; // Morton rank of each geometric cell
+    const uint32_t* start;     // slots of rank r: start[r] .. start[r + 1]
+    const uint32_t* members;
+    const int4* cell_fixed;
+    const uint32_t* rank_of;
+    const double* r_listsq;
+    const float* r_listsq_lo;
+    const float* r_listsq_hi;
+    uint32_t ntypes;
+    uint32_t* n_neigh;  // by particle
+    uint32_t* tile_nn;  // by slot (set by tiles_walk)
+    uint16_t* rows;     // by slot
+    uint32_t max_neighbors; // a multiple of 8
+    // [0] largest count above max_neighbors, [2] bad type (1) / position (2),
+    // [3] a tile did not fit (1: cells, 2: candidates, 4: neighbor set),
+    // [4] largest neighbor-set size
+    uint32_t* flags;
+    uint32_t* tile_count;
+    uint32_t* tile_jset;
+    uint32_t* tlist;
+    uint32_t pitch;
+    };
+
+// Enqueues the fused walk with tiles of `tile_size` (256 or 512) particles
+// into t's arrays (sized here); the caller reads args.flags afterwards and
+// sets t.size / t.slots / t.valid when flags[3] is clear.
+int tiles_walk(TileList& t, uint32_t tile_size, const TileWalkArgs& args, hipStream_t stream);
+
+// Enqueues the expansion of a valid slot-ordered tiled list into the plain
+// column list (d_nlist[k * pitch + i], padding entries hold i).
+int tiles_to_plain(const TileList& t,
+                   uint32_t N,
+                   const uint32_t* d_n_neigh,
+                   const uint32_t* d_rank_of,
+                   uint32_t pitch,
+                   uint32_t* d_nlist,
+                   hipStream_t stream);
 
 } // namespace azp

@@ -105,7 +105,9 @@ def main():
     torch.cuda.synchronize()
     t_nlist_tiled = time.perf_counter() - t0
     tiles = nl.get_tiles()
-    log(f"bench: tile size {tiles[3]} (0 = plain list), LDS slots {tiles[4]}")
+    tile_order = nl.get_tile_order()
+    log(f"bench: tile size {tiles[3]} (0 = plain list), LDS slots {tiles[4]}, "
+        f"{'per-tile walk' if nl.walk_kind() else 'plain walk'}")
     d_nn, d_list, pitch, mx = nl.get()
     nn_host, _ = nl.download(N)
     nlist_entries = int(nn_host.sum())
@@ -143,6 +145,7 @@ def main():
     args_mixed = make_args(1)
     args_mixed.d_tile_count, args_mixed.d_tile_jset, args_mixed.d_tile_list, args_mixed.tile_size, \
         args_mixed.tile_slots = tiles
+    args_mixed.d_tile_order, args_mixed.d_tile_nn = tile_order
     args_fp64 = make_args(0)
 
     def step_mixed():

@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define AZP_ABI_VERSION 1
+#define AZP_ABI_VERSION 2
 
 enum azp_status
     {
@@ -186,12 +186,17 @@ typedef struct
     void* stream;
     /* optional tiled list for the mixed form (azp_nlist_get_tiles); NULL
      * d_tile_list = use d_nlist (and d_pos_fixed) only.  The tiled kernel
-     * converts positions to fixed point itself (d_pos_fixed is not read). */
+     * converts positions to fixed point itself (d_pos_fixed is not read).
+     * d_tile_order / d_tile_nn (azp_nlist_get_tile_order) for tiles of
+     * cell-list slots, NULL for tiles of consecutive particles; d_nlist may
+     * be NULL when the tiled list is given. */
     const uint32_t* d_tile_count;
     const uint32_t* d_tile_jset;
     const uint32_t* d_tile_list;
     uint32_t tile_size;
     uint32_t tile_slots;
+    const uint32_t* d_tile_order;
+    const uint32_t* d_tile_nn;
     } azp_pair_args;
 
 /* ---- library ---- */
@@ -330,11 +335,15 @@ int azp_nlist_num_builds(const azp_nlist* nl, uint64_t* n);
 /* Tiled form of the list for the mixed-precision pair kernel (this
  * library's own addition; no reference counterpart): the full list of each
  * block of 512 (or 256) consecutive particles rewritten as 16-bit indices
- * into the block's sorted set of distinct neighbors.  Built with every
- * rebuild once enabled.  mode: 0 off, 1 blocks of 512 else 256, 256 or 512
- * that size only.  tile_size = 0 when some block has more than 3967
- * distinct neighbors (the plain list is then used); tile_slots = LDS slots
- * the pair kernel stages (largest distinct count + 1). */
+ * into the block's set of distinct neighbors.  Built with every rebuild
+ * once enabled.  mode: 0 off, 1 blocks of 512 else 256, 256 or 512 that
+ * size only.  tile_size = 0 when some block has more than 3967 distinct
+ * neighbors (the plain list is then used); tile_slots = LDS slots the pair
+ * kernel stages (largest distinct count + 1).  While tiles are enabled the
+ * build walks the cell list once per block (the block's candidates staged
+ * in LDS) and writes only the tiled list; azp_nlist_get / _download then
+ * expand the plain list from it on first use.  Blocks whose surrounding
+ * cells do not fit in LDS make that build take the plain walk instead. */
 int azp_nlist_set_tiled(azp_nlist* nl, int mode);
 int azp_nlist_get_tiles(const azp_nlist* nl,
                         const uint32_t** d_tile_count,
@@ -342,6 +351,16 @@ int azp_nlist_get_tiles(const azp_nlist* nl,
                         const uint32_t** d_tile_list,
                         uint32_t* tile_size,
                         uint32_t* tile_slots);
+/* Column order of the tiled list.  Tiles from the per-block walk are runs
+ * of consecutive cell-list slots (cells in Morton order, so each tile is a
+ * compact block of space for any particle order): column s belongs to
+ * particle d_tile_order[s] and has d_tile_nn[s] entries.  Both are NULL for
+ * tiles built from the plain list, whose column i is particle i with
+ * n_neigh[i] entries. */
+int azp_nlist_get_tile_order(const azp_nlist* nl, const uint32_t** d_tile_order, const uint32_t** d_tile_nn);
+/* How the last build made the list: 1 the per-block walk that writes the
+ * tiled list directly, 0 the plain walk. */
+int azp_nlist_walk_kind(const azp_nlist* nl, int* kind);
 /* Copy n_neigh[N] and the list (max_neighbors * pitch entries) to host memory
  * (synchronizes the device). */
 int azp_nlist_download(const azp_nlist* nl,

@@ -13,7 +13,7 @@ import pytest
 
 import oracle
 from azplugins_amd import _azp
-from gpu_util import GpuSystem, gpu_pair, jittered_lattice, rel_err, say
+from gpu_util import GpuSystem, gpu_pair, jittered_lattice, nlist_pairs, rel_err, say
 
 torch = _azp.torch
 pytestmark = pytest.mark.gpu
@@ -89,7 +89,7 @@ def test_tiles_fall_back_when_a_block_has_too_many_neighbors():
     Lv = np.array([L, L, L])
     sysm = GpuSystem(pos4, Lv)
     plain, tiled = _both("PotentialPairPerturbedLennardJones", sysm, 1, PLJ, 7.0)
-    assert tiled.nl.get_tiles()[3] == 0
+    assert tiled.nl.get_tiles()[3] == 0 and tiled.nl.walk_kind() == 0
     assert np.array_equal(plain.force, tiled.force)
 
 
@@ -117,6 +117,7 @@ def test_tiled_launcher_entry_point():
     a = raw_args(sysm, ref.nl, rcsq, None, 1, force, vir, pitch, precision=1)
     a.particles.d_pos_fixed = pf.data_ptr()
     a.d_tile_count, a.d_tile_jset, a.d_tile_list, a.tile_size, a.tile_slots = ref.nl.get_tiles()
+    a.d_tile_order, a.d_tile_nn = ref.nl.get_tile_order()
     a.particles.d_pos_fixed = None  # the tiled kernel converts positions itself
     _azp.call("azp_gpu_compute_pair_forces", ctypes.byref(a), 2, _azp.dptr(d_params))
     torch.cuda.synchronize()
@@ -128,3 +129,122 @@ def test_tiled_launcher_entry_point():
     a.tile_slots = 5000
     with pytest.raises(_azp.AzpError):
         _azp.call("azp_gpu_compute_pair_forces", ctypes.byref(a), 2, _azp.dptr(d_params))
+
+
+# ---------------------------------------------------------------------------
+# Fused per-tile walk (k_tile_walk.hip): with tiles enabled the build stages
+# each tile's surrounding cells in LDS and writes the tiled list directly.
+# The plain list expanded from it must equal the plain walk's list entry for
+# entry (same order), and the pair set must be the oracle's.
+# ---------------------------------------------------------------------------
+
+def _morton(pos4, L, cell=3.4):
+    from azplugins_amd import configs
+
+    return pos4[configs.morton_order(pos4, np.array([L, L, L]), cell)].copy()
+
+
+def _lists(sysm, nt, rc_matrix, tiled, r_buff=0.4):
+    from azplugins_amd import _azplugins
+
+    nl = _azplugins.NeighborListGPU(r_buff)
+    nl.set_r_cut(nt, rc_matrix)
+    if tiled:
+        nl.set_tiled(tiled)
+    nl.update(sysm.particles())
+    kind = nl.walk_kind()
+    nn, lst = nl.download(sysm.N)
+    return nl, kind, nn, lst
+
+
+def _same_lists(nn_a, lst_a, nn_b, lst_b):
+    assert np.array_equal(nn_a, nn_b)
+    for i in range(nn_a.shape[0]):
+        k = int(nn_a[i])
+        if not np.array_equal(lst_a[:k, i], lst_b[:k, i]):
+            return False
+    return True
+
+
+@pytest.mark.parametrize("n,L,mode", [(24, 25.3, 1), (32, 33.7, 1), (24, 25.3, 256), (18, 18.9, 512),
+                                      (7, 7.3, 1)])
+def test_fused_walk_equals_plain_walk(n, L, mode):
+    """Morton-ordered lattices (the order a space-filling-curve sort leaves):
+    N = 343 (two cells per axis: repeated stencil cells) .. 32768."""
+    pos4, _ = jittered_lattice(n, L, 0.1, 1, seed=n + 5)
+    pos4 = _morton(pos4, L)
+    sysm = GpuSystem(pos4, np.array([L, L, L]))
+    rc = [3.0]
+    nl_t, kind, nn_t, lst_t = _lists(sysm, 1, rc, mode)
+    _, kind_p, nn_p, lst_p = _lists(sysm, 1, rc, 0)
+    size, slots = nl_t.get_tiles()[3:]
+    say(f"[fused] N={sysm.N} L={L} mode {mode}: walk kind {kind}, tile size {size}, slots {slots}")
+    assert kind == 1 and kind_p == 0
+    assert size == (512 if mode in (1, 512) else 256)
+    assert _same_lists(nn_t, lst_t, nn_p, lst_p)
+    pairs = oracle.nlist(pos4, np.array([L, L, L]), 1, oracle.r_listsq_matrix([3.0], 0.4), "cell")
+    got = nlist_pairs(nl_t, sysm.N)
+    assert got == set(zip(pairs[0].tolist(), pairs[1].tolist()))
+
+
+def test_fused_walk_two_types_excluded_pair():
+    """Two types with the (1, 1) pair excluded (r_cut 0) and unequal cutoffs."""
+    L = 22.0
+    pos4, _ = jittered_lattice(22, L, 0.15, 2, seed=77)
+    pos4 = _morton(pos4, L)
+    sysm = GpuSystem(pos4, np.array([L, L, L]))
+    rc = [2.5, 3.0, 3.0, 0.0]
+    _, kind, nn_t, lst_t = _lists(sysm, 2, rc, 1)
+    _, _, nn_p, lst_p = _lists(sysm, 2, rc, 0)
+    say(f"[fused] two types, excluded (1,1): walk kind {kind}")
+    assert kind == 1
+    assert _same_lists(nn_t, lst_t, nn_p, lst_p)
+    types = pos4[:, 3].astype(int)
+    for i in range(0, sysm.N, 97):
+        js = lst_t[: nn_t[i], i]
+        assert not (types[i] == 1 and np.any(types[js] == 1))
+
+
+def test_fused_walk_any_particle_order():
+    """Tiles are runs of cell-list slots (cells in Morton order), so a random
+    particle order still gives compact tiles: the per-tile walk runs and the
+    lists and forces equal the plain ones."""
+    L = 25.3
+    pos4, _ = jittered_lattice(24, L, 0.1, 1, seed=9)
+    pos4 = pos4[np.random.default_rng(1).permutation(pos4.shape[0])].copy()
+    sysm = GpuSystem(pos4, np.array([L, L, L]))
+    plain, tiled = _both("PotentialPairPerturbedLennardJones", sysm, 1, PLJ, 3.0)
+    kind = tiled.nl.walk_kind()
+    size = tiled.nl.get_tiles()[3]
+    say(f"[fused] random order: walk kind {kind}, tile size {size}")
+    assert kind == 1 and size == 512
+    assert np.array_equal(plain.force, tiled.force)
+    assert np.array_equal(plain.virial, tiled.virial)
+    nn_t, lst_t = tiled.nl.download(sysm.N)
+    nn_p, lst_p = plain.nl.download(sysm.N)
+    assert _same_lists(nn_t, lst_t, nn_p, lst_p)
+
+
+def test_fused_walk_forces_and_rebuild():
+    """Mixed forces over the fused tiled list equal those over the plain list;
+    a second build after moving particles is fused again and still equal."""
+    L = 25.3
+    pos4, _ = jittered_lattice(24, L, 0.1, 1, seed=21)
+    pos4 = _morton(pos4, L)
+    Lv = np.array([L, L, L])
+    sysm = GpuSystem(pos4, Lv)
+    plain, tiled = _both("PotentialPairPerturbedLennardJones", sysm, 1, PLJ, 3.0)
+    assert tiled.nl.walk_kind() == 1
+    assert np.array_equal(plain.force, tiled.force)
+    assert np.array_equal(plain.virial, tiled.virial)
+    # move every particle by up to 0.3 (more than r_buff / 2): the list rebuilds
+    rng = np.random.default_rng(5)
+    moved = pos4.copy()
+    moved[:, :3] += rng.uniform(-0.3, 0.3, (pos4.shape[0], 3))
+    moved[:, :3] -= L * np.round(moved[:, :3] / L)
+    sysm.pos.copy_(torch.tensor(moved, device=sysm.pos.device))
+    parts = sysm.particles()
+    tiled.nl.update(parts)
+    assert tiled.nl.walk_kind() == 1 and tiled.nl.num_builds() == 2
+    pairs = oracle.nlist(moved, Lv, 1, oracle.r_listsq_matrix([3.0], 0.4), "cell")
+    assert nlist_pairs(tiled.nl, sysm.N) == set(zip(pairs[0].tolist(), pairs[1].tolist()))
