@@ -379,6 +379,7 @@ struct azp_nlist
     DeviceArray<double4> last_pos;
     DeviceArray<uint32_t> cell_of, slot_of, count, start, members, scan_sums;
     DeviceArray<uint32_t> cell_rank; // Morton rank of each cell of the grid below
+    DeviceArray<uint32_t> rank_cell; // its inverse
     int rank_grid[3] = {0, 0, 0};
     DeviceArray<uint32_t> flags; // [0] overflow, [1] rebuild, [2] bad type, [3] tile misfit, [4] largest tile set
     uint32_t* h_flags = nullptr;  // pinned, 8 words ([4], [5] tile build flags)
@@ -439,11 +440,15 @@ static int nlist_build(azp_nlist* nl, const azp_particles* p, hipStream_t s)
 
     if (g.n[0] != nl->rank_grid[0] || g.n[1] != nl->rank_grid[1] || g.n[2] != nl->rank_grid[2])
         {
-        std::vector<uint32_t> rank;
+        std::vector<uint32_t> rank, inv(ncell);
         morton_cell_rank(g, rank);
+        for (size_t k = 0; k < ncell; ++k)
+            inv[rank[k]] = (uint32_t)k;
         AZP_TRY(nl->cell_rank.reserve(ncell));
+        AZP_TRY(nl->rank_cell.reserve(ncell));
         AZP_HIP(hipMemcpyAsync(nl->cell_rank.ptr, rank.data(), ncell * sizeof(uint32_t), hipMemcpyHostToDevice, s));
-        AZP_HIP(hipStreamSynchronize(s)); // the host vector goes out of scope
+        AZP_HIP(hipMemcpyAsync(nl->rank_cell.ptr, inv.data(), ncell * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+        AZP_HIP(hipStreamSynchronize(s)); // the host vectors go out of scope
         for (int k = 0; k < 3; ++k)
             nl->rank_grid[k] = g.n[k];
         }
@@ -481,7 +486,7 @@ static int nlist_build(azp_nlist* nl, const azp_particles* p, hipStream_t s)
         nl->tiles.valid = false;
         nl->plain_ready = true;
         nl->fused = false;
-        if (nl->want_tiles)
+        if (nl->want_tiles && nl->ntypes <= kTileWalkMaxTypes)
             {
             // fused walk: the tiled list straight from the cell list, 512-
             // particle tiles, else 256
@@ -506,6 +511,7 @@ static int nlist_build(azp_nlist* nl, const azp_particles* p, hipStream_t s)
                 ta.g = g;
                 ta.cell_of = nl->cell_of.ptr;
                 ta.cell_rank = nl->cell_rank.ptr;
+                ta.rank_cell = nl->rank_cell.ptr;
                 ta.start = nl->start.ptr;
                 ta.members = nl->members.ptr;
                 ta.cell_fixed = nl->cell_fixed.ptr;

@@ -41,27 +41,38 @@ template<> struct TileWalkShape<256>
     static constexpr uint32_t cand = 4096;
     };
 
-constexpr uint32_t kUnionCells = 1024; // distinct cells around one tile
-constexpr uint32_t kCellHash = 2048;   // open-addressing table of those cells
-constexpr uint32_t kNoCell = 0xffffffffu;
+constexpr uint32_t kUnionCells = 512; // distinct cells around one tile
+constexpr uint32_t kCellHash = 1024;  // open-addressing table of those cells
+// Per-lane ring of 16 list entries (32 bytes) padded to 40 bytes, so that the
+// lanes' 16-bit appends fall in different LDS banks.
+constexpr uint32_t kRingWords = 10;
+// The tile's own cells are the cells of Morton rank first .. last of its
+// slots; each gets a 27-entry table of its neighbor cells' candidate ranges.
+constexpr uint32_t kOwnCells = 128;
+constexpr uint32_t kNoRange = 0xffffffffu; // a repeated stencil cell (grids narrower than 3)
 constexpr uint16_t kUsed = 0x8000u;    // mark bit beside a 15-bit rank
 
 __device__ __forceinline__ uint32_t cell_hash(uint32_t c)
     {
-    return (c * 2654435761u) >> 21; // 11 bits
+    return (c * 2654435761u) >> 22; // 10 bits
     }
+
+// Hash entries: cell index in the high word; once the offsets are known the
+// low word holds the cell's candidate range in LDS (first | end << 16).
+constexpr uint64_t kNoEntry = ~0ull;
 
 // Inserts cell c; returns true when this call added it.  *full is set when
 // the table has no free slot left for it.
-__device__ __forceinline__ bool cell_insert(uint32_t* keys, uint32_t c, bool* full)
+__device__ __forceinline__ bool cell_insert(uint64_t* tab, uint32_t c, bool* full)
     {
+    const uint64_t key = (uint64_t)c << 32;
     uint32_t h = cell_hash(c);
     for (uint32_t p = 0; p < kCellHash; ++p)
         {
-        const uint32_t prev = atomicCAS(&keys[h], kNoCell, c);
-        if (prev == kNoCell)
+        const uint64_t prev = atomicCAS(reinterpret_cast<unsigned long long*>(&tab[h]), kNoEntry, key);
+        if (prev == kNoEntry)
             return true;
-        if (prev == c)
+        if ((uint32_t)(prev >> 32) == c)
             return false;
         h = (h + 1u) & (kCellHash - 1u);
         }
@@ -69,17 +80,33 @@ __device__ __forceinline__ bool cell_insert(uint32_t* keys, uint32_t c, bool* fu
     return false;
     }
 
-__device__ __forceinline__ uint32_t cell_find(const uint32_t* keys, uint32_t c)
+// Slot of cell c (present in the table).
+__device__ __forceinline__ uint32_t cell_slot(const uint64_t* tab, uint32_t c)
     {
     uint32_t h = cell_hash(c);
     for (uint32_t p = 0; p < kCellHash; ++p)
         {
-        const uint32_t k = keys[h];
-        if (k == c || k == kNoCell)
+        if ((uint32_t)(tab[h] >> 32) == c)
             return h;
         h = (h + 1u) & (kCellHash - 1u);
         }
     return 0u; // not reached: every looked-up cell was inserted
+    }
+
+// Index of the union cell whose candidate range holds m (ranges ascending,
+// the cell's own range non-empty): the largest u < U with off[u] <= m.
+__device__ __forceinline__ uint32_t union_cell_of(const uint32_t* off, uint32_t U, uint32_t m)
+    {
+    uint32_t lo = 0, hi = U;
+    while (hi - lo > 1u)
+        {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (off[mid] <= m)
+            lo = mid;
+        else
+            hi = mid;
+        }
+    return lo;
     }
 
 // Exclusive scan of one value per thread over the T-thread workgroup.
@@ -112,22 +139,31 @@ struct TileWalkLds
     {
     const int4* cand;
     uint16_t* mark;
-    const uint32_t* key;
-    const uint16_t* val;
+    const uint32_t* stencil; // 27 candidate ranges (first | end << 16) per own cell
     const uint32_t* off;
     const uint32_t* gs; // first cell-list slot of each union cell
+    const float2* lohi; // r_list^2 (1 -+ 1e-5) per type pair (several types)
+    uint32_t U;
     };
 
+// the fused walk's type-pair table lives in LDS
+constexpr uint32_t kTileWalkTypes = kTileWalkMaxTypes;
+
 // Lists particle i against the staged candidates; returns its neighbor
-// count.  Entries are LDS candidate indices, buffered eight at a time in the
-// lane's LDS slot and stored to its row as 16-byte chunks.
-template<bool SINGLE>
+// count.  Entries are LDS candidate indices, appended to the lane's 16-entry
+// LDS ring and stored to its row eight at a time (16 bytes).  Candidates are
+// read four at a time (the staging array has room for the three past a
+// cell's end); the per-candidate tests are branch-free, and only the rare
+// candidates near r_list take the FP64 branch.  With CUBIC the squared
+// distance is formed in fixed-point units (L / 2^32 on every axis) and
+// compared with r_list^2 in those units.
+template<bool SINGLE, bool CUBIC>
 __device__ uint32_t tile_walk_one(const TileWalkArgs& a,
                                   const TileWalkLds& s,
                                   uint32_t slot,
                                   uint32_t i,
-                                  uint32_t c,
-                                  uint4* lslot)
+                                  uint32_t own,
+                                  uint32_t* ring)
     {
     const double4 pi = a.pos[i];
     const uint32_t ti = checked_type(pi.w, a.ntypes);
@@ -140,89 +176,118 @@ __device__ uint32_t tile_walk_one(const TileWalkArgs& a,
     const int32_t fxi = to_fixed(pi.x, a.box.Linv[0]);
     const int32_t fyi = to_fixed(pi.y, a.box.Linv[1]);
     const int32_t fzi = to_fixed(pi.z, a.box.Linv[2]);
-    const uint32_t u_own = s.val[cell_find(s.key, c)];
-    const uint32_t self = s.off[u_own] + (slot - s.gs[u_own]);
-    const int nx = a.g.n[0], ny = a.g.n[1], nz = a.g.n[2];
-    const int cx = (int)(c % (uint32_t)nx);
-    const int cy = (int)((c / (uint32_t)nx) % (uint32_t)ny);
-    const int cz = (int)(c / (uint32_t)(nx * ny));
-    // offsets -1, 0, +1 per axis; on a grid of one or two cells the offsets
-    // that repeat a cell are dropped (the order of stencil_axis)
-    const int ex = min(nx, 3) - 2, ey = min(ny, 3) - 2, ez = min(nz, 3) - 2;
+    const float sx = a.fx_scale[0], sy = a.fx_scale[1], sz = a.fx_scale[2];
+    const float unit2 = CUBIC ? 1.0f / (sx * sx) : 1.0f; // r^2 -> fixed-point units^2
+    // the lane's 27 neighbor cells (own cell at 13), in stencil_axis order
+    const uint32_t* stencil = s.stencil + own * 27u;
+    const uint32_t own_first = stencil[13] & 0xffffu;
+    const uint32_t self = own_first + (slot - s.gs[union_cell_of(s.off, s.U, own_first)]);
     const double rl0 = a.r_listsq[0];
-    const float lo0 = a.r_listsq_lo[0], hi0 = a.r_listsq_hi[0];
-    uint16_t* lbuf = reinterpret_cast<uint16_t*>(lslot);
+    // wave-uniform, held in scalar registers: the candidate loop then never
+    // waits on the vector memory counter (which the row stores also use)
+    const float lo0 = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, a.r_listsq_lo[0])))
+                      * unit2;
+    const float hi0 = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, a.r_listsq_hi[0])))
+                      * unit2;
+    uint16_t* ring16 = reinterpret_cast<uint16_t*>(ring);
     uint16_t* row = a.rows + (size_t)slot * a.max_neighbors;
     uint32_t count = 0;
-    for (int oz = -1; oz <= ez; ++oz)
-        for (int oy = -1; oy <= ey; ++oy)
-            for (int ox = -1; ox <= ex; ++ox)
-                {
-                const int x = (cx + ox + nx) % nx, y = (cy + oy + ny) % ny, z = (cz + oz + nz) % nz;
-                const uint32_t nb = (uint32_t)((z * ny + y) * nx + x);
-                const uint32_t u = s.val[cell_find(s.key, nb)];
-                const uint32_t b = s.off[u], e = s.off[u + 1];
-                for (uint32_t m0 = b; m0 < e; m0 += 4)
+    // the next cell's range and first candidates are read one step ahead
+    uint32_t range_next = stencil[0];
+    for (uint32_t k = 0; k < 27u; ++k)
+        {
+        const uint32_t range = range_next;
+        range_next = stencil[min(k + 1u, 26u)];
+        const uint32_t b = range & 0xffffu, e = range >> 16;
+        int4 ahead[4] = {s.cand[b], s.cand[b + 1], s.cand[b + 2], s.cand[b + 3]};
+        for (uint32_t m0 = b; m0 < e; m0 += 4)
                     {
-                    int4 pc[4];
+                    const int4 pc[4] = {ahead[0], ahead[1], ahead[2], ahead[3]};
 #pragma unroll
                     for (uint32_t q = 0; q < 4; ++q)
-                        pc[q] = s.cand[min(m0 + q, e - 1)];
+                        ahead[q] = s.cand[m0 + 4 + q];
+                    bool hit[4], band[4];
 #pragma unroll
                     for (uint32_t q = 0; q < 4; ++q)
                         {
                         const uint32_t m = m0 + q;
-                        const int4 pj = pc[q];
-                        const uint32_t tj = (uint32_t)pj.w;
-                        if (m >= e || m == self || tj >= a.ntypes)
-                            continue;
-                        const uint32_t tp = SINGLE ? 0u : ti * a.ntypes + tj;
-                        const float lo = SINGLE ? lo0 : a.r_listsq_lo[tp];
-                        const float hi = SINGLE ? hi0 : a.r_listsq_hi[tp];
-                        const float dx = (float)fixed_delta(fxi, pj.x) * a.fx_scale[0];
-                        const float dy = (float)fixed_delta(fyi, pj.y) * a.fx_scale[1];
-                        const float dz = (float)fixed_delta(fzi, pj.z) * a.fx_scale[2];
-                        const float rsq = dx * dx + dy * dy + dz * dz;
-                        bool in = rsq < lo;
-                        if (!in && rsq <= hi)
-                            {
-                            const uint32_t j = a.members[s.gs[u] + (m - b)];
-                            const double4 p64 = a.pos[j];
-                            const double rl = SINGLE ? rl0 : a.r_listsq[tp];
-                            const double ex = min_image_1d(pi.x - p64.x, a.box.L[0], a.box.Linv[0]);
-                            const double ey = min_image_1d(pi.y - p64.y, a.box.L[1], a.box.Linv[1]);
-                            const double ez = min_image_1d(pi.z - p64.z, a.box.L[2], a.box.Linv[2]);
-                            in = rsq3_nofma(ex, ey, ez) <= rl;
-                            }
-                        if (in)
-                            {
-                            s.mark[m] = 1u;
-                            lbuf[count & 7u] = (uint16_t)m;
-                            if ((count & 7u) == 7u && count < a.max_neighbors)
-                                *reinterpret_cast<uint4*>(row + (count & ~7u)) = *lslot;
-                            ++count;
-                            }
+                        const uint32_t tj = (uint32_t)pc[q].w;
+                        const bool valid = (m < e) & (m != self) & (tj < a.ntypes);
+                        const uint32_t tp = SINGLE ? 0u : ti * a.ntypes + min(tj, a.ntypes - 1);
+                        const float lo = SINGLE ? lo0 : s.lohi[tp].x * unit2;
+                        const float hi = SINGLE ? hi0 : s.lohi[tp].y * unit2;
+                        const float dx = (float)fixed_delta(fxi, pc[q].x) * (CUBIC ? 1.0f : sx);
+                        const float dy = (float)fixed_delta(fyi, pc[q].y) * (CUBIC ? 1.0f : sy);
+                        const float dz = (float)fixed_delta(fzi, pc[q].z) * (CUBIC ? 1.0f : sz);
+                        // FP32 rounding here is far inside the 1e-5 band
+                        const float rsq = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+                        hit[q] = valid & (rsq < lo);
+                        band[q] = valid & (rsq >= lo) & (rsq <= hi);
                         }
+                    if (band[0] | band[1] | band[2] | band[3])
+                        {
+                        // FP32 cannot decide: the FP64 predicate of the host oracle
+#pragma unroll
+                        for (uint32_t q = 0; q < 4; ++q)
+                            if (band[q])
+                                {
+                                const uint32_t m = m0 + q;
+                                const uint32_t u = union_cell_of(s.off, s.U, m);
+                                const uint32_t j = a.members[s.gs[u] + (m - s.off[u])];
+                                const double4 p64 = a.pos[j];
+                                const uint32_t tp = SINGLE ? 0u : ti * a.ntypes + (uint32_t)pc[q].w;
+                                const double rl = SINGLE ? rl0 : a.r_listsq[tp];
+                                const double qx = min_image_1d(pi.x - p64.x, a.box.L[0], a.box.Linv[0]);
+                                const double qy = min_image_1d(pi.y - p64.y, a.box.L[1], a.box.Linv[1]);
+                                const double qz = min_image_1d(pi.z - p64.z, a.box.L[2], a.box.Linv[2]);
+                                hit[q] = rsq3_nofma(qx, qy, qz) <= rl;
+                                }
+                        }
+                    // append the hits in candidate order
+                    uint32_t pos = count;
+#pragma unroll
+                    for (uint32_t q = 0; q < 4; ++q)
+                        {
+                        if (hit[q])
+                            {
+                            s.mark[m0 + q] = 1u;
+                            ring16[pos & 15u] = (uint16_t)(m0 + q);
+                            }
+                        pos += hit[q] ? 1u : 0u;
+                        }
+                    // a completed group of eight leaves as one 16-byte store
+                    if (((pos ^ count) & ~7u) != 0u && count < a.max_neighbors)
+                        {
+                        const uint2* h = reinterpret_cast<const uint2*>(ring + ((count >> 1) & 4u));
+                        const uint2 w0 = h[0], w1 = h[1];
+                        *reinterpret_cast<uint4*>(row + (count & ~7u)) = make_uint4(w0.x, w0.y, w1.x, w1.y);
+                        }
+                    count = pos;
                     }
-                }
+        }
     if ((count & 7u) != 0u && count < a.max_neighbors)
-        *reinterpret_cast<uint4*>(row + (count & ~7u)) = *lslot;
+        {
+        const uint2* h = reinterpret_cast<const uint2*>(ring + ((count >> 1) & 4u));
+        const uint2 w0 = h[0], w1 = h[1];
+        *reinterpret_cast<uint4*>(row + (count & ~7u)) = make_uint4(w0.x, w0.y, w1.x, w1.y);
+        }
     return count;
     }
 
 template<uint32_t T, bool SINGLE> __global__ void __launch_bounds__(T) k_tile_walk(const TileWalkArgs a)
     {
     constexpr uint32_t C = TileWalkShape<T>::cand;
-    __shared__ int4 s_cand[C];
+    __shared__ int4 s_cand[C + 8];
     __shared__ uint16_t s_mark[C];
-    __shared__ uint32_t s_key[kCellHash];
-    __shared__ uint16_t s_val[kCellHash];
+    __shared__ uint64_t s_tab[kCellHash];
     __shared__ uint64_t s_ucell[kUnionCells];
     __shared__ uint32_t s_off[kUnionCells + 1];
     __shared__ uint32_t s_gs[kUnionCells];
-    __shared__ uint4 s_lbuf[T];
+    __shared__ uint32_t s_ring[T * kRingWords];
+    __shared__ uint32_t s_stencil[kOwnCells * 27];
     __shared__ uint32_t s_wave[T / 64];
-    __shared__ uint32_t s_n, s_bad;
+    __shared__ float2 s_lohi[SINGLE ? 1 : kTileWalkTypes * kTileWalkTypes];
+    __shared__ uint32_t s_n, s_bad, s_rfirst, s_rlast;
 
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
     const uint32_t t = xcd_logical_block();
@@ -231,7 +296,10 @@ template<uint32_t T, bool SINGLE> __global__ void __launch_bounds__(T) k_tile_wa
     const uint32_t nx = (uint32_t)a.g.n[0], ny = (uint32_t)a.g.n[1], nz = (uint32_t)a.g.n[2];
 
     for (uint32_t k = tid; k < kCellHash; k += T)
-        s_key[k] = kNoCell;
+        s_tab[k] = kNoEntry;
+    if (!SINGLE)
+        for (uint32_t k = tid; k < a.ntypes * a.ntypes; k += T)
+            s_lohi[k] = make_float2(a.r_listsq_lo[k], a.r_listsq_hi[k]);
     if (tid == 0)
         {
         s_n = 0u;
@@ -239,37 +307,44 @@ template<uint32_t T, bool SINGLE> __global__ void __launch_bounds__(T) k_tile_wa
         }
     __syncthreads();
 
-    // 1. the tile's own cells (distinct) head the union list
+    // 1. the tile's own cells: Morton ranks first .. last of its slots
     const uint32_t i = active ? a.members[slot] : 0u;
     const uint32_t c = active ? a.cell_of[i] : 0u;
-    if (active)
+    const uint32_t crank = active ? a.cell_rank[c] : 0u;
+    if (tid == 0)
+        s_rfirst = crank;
+    if (active && (tid == T - 1 || slot == a.N - 1))
+        s_rlast = crank;
+    __syncthreads();
+    const uint32_t r_first = s_rfirst;
+    const uint32_t D = s_rlast - r_first + 1u;
+    if (D > kOwnCells)
         {
-        bool full = false;
-        if (cell_insert(s_key, c, &full))
-            {
-            const uint32_t k = atomicAdd(&s_n, 1u);
-            if (k < kUnionCells)
-                s_ucell[k] = ((uint64_t)a.cell_rank[c] << 32) | c;
-            }
-        if (full)
-            s_bad = 1u;
+        if (tid == 0)
+            atomicOr(a.flags + 3, 8u);
+        return; // D is the same for every thread
         }
-    __syncthreads();
-    const uint32_t D = min(s_n, kUnionCells);
-    __syncthreads();
-    // ... followed by their neighbor cells (periodic; repeats on grids
-    // narrower than 3 cells are removed by the hash)
+    // 2. their neighbor cells (periodic) make the union; the stencil table
+    // keeps each neighbor's cell index until the ranges are known, and marks
+    // the offsets that repeat a cell on grids narrower than 3
     for (uint32_t q = tid; q < D * 27u; q += T)
         {
-        const uint32_t oc = (uint32_t)s_ucell[q / 27u];
+        const uint32_t oc = a.rank_cell[r_first + q / 27u];
         const uint32_t sidx = q % 27u;
+        const uint32_t dx = sidx % 3u, dy = (sidx / 3u) % 3u, dz = sidx / 9u; // offset + 1
+        if (dx > min(nx, 3u) - 1u || dy > min(ny, 3u) - 1u || dz > min(nz, 3u) - 1u)
+            {
+            s_stencil[q] = kNoRange;
+            continue;
+            }
         const uint32_t ox = oc % nx, oy = (oc / nx) % ny, oz = oc / (nx * ny);
-        const uint32_t x = (ox + nx + sidx % 3u - 1u) % nx;
-        const uint32_t y = (oy + ny + (sidx / 3u) % 3u - 1u) % ny;
-        const uint32_t z = (oz + nz + sidx / 9u - 1u) % nz;
+        const uint32_t x = (ox + nx + dx - 1u) % nx;
+        const uint32_t y = (oy + ny + dy - 1u) % ny;
+        const uint32_t z = (oz + nz + dz - 1u) % nz;
         const uint32_t nb = (z * ny + y) * nx + x;
+        s_stencil[q] = nb;
         bool full = false;
-        if (cell_insert(s_key, nb, &full))
+        if (cell_insert(s_tab, nb, &full))
             {
             const uint32_t k = atomicAdd(&s_n, 1u);
             if (k < kUnionCells)
@@ -340,8 +415,11 @@ template<uint32_t T, bool SINGLE> __global__ void __launch_bounds__(T) k_tile_wa
         const uint32_t u = tid * QC + q;
         if (u < U)
             {
+            const uint32_t cu = (uint32_t)s_ucell[u];
             s_off[u] = run;
-            s_val[cell_find(s_key, (uint32_t)s_ucell[u])] = (uint16_t)u;
+            // candidate range in the hash entry (Mc <= C < 2^16 is checked below
+            // before any lookup)
+            s_tab[cell_slot(s_tab, cu)] = ((uint64_t)cu << 32) | (run & 0xffffu) | ((run + cnt[q]) << 16);
             }
         run += cnt[q];
         }
@@ -380,14 +458,23 @@ template<uint32_t T, bool SINGLE> __global__ void __launch_bounds__(T) k_tile_wa
         }
     for (uint32_t m = tid; m < Mc; m += T)
         s_mark[m] = 0u;
+    for (uint32_t q = tid; q < D * 27u; q += T)
+        {
+        const uint32_t nb = s_stencil[q];
+        s_stencil[q] = nb == kNoRange ? 0u : (uint32_t)s_tab[cell_slot(s_tab, nb)];
+        }
     __syncthreads();
 
     // 3. walk
     uint32_t count = 0;
     if (active)
         {
-        const TileWalkLds s = {s_cand, s_mark, s_key, s_val, s_off, s_gs};
-        count = tile_walk_one<SINGLE>(a, s, slot, i, c, &s_lbuf[tid]);
+        const TileWalkLds s = {s_cand, s_mark, s_stencil, s_off, s_gs, s_lohi, U};
+        uint32_t* ring = s_ring + tid * kRingWords;
+        const bool cubic = a.fx_scale[0] == a.fx_scale[1] && a.fx_scale[0] == a.fx_scale[2];
+        const uint32_t own = crank - r_first;
+        count = cubic ? tile_walk_one<SINGLE, true>(a, s, slot, i, own, ring)
+                      : tile_walk_one<SINGLE, false>(a, s, slot, i, own, ring);
         const uint32_t stored = min(count, a.max_neighbors);
         a.n_neigh[i] = stored;
         a.tile_nn[slot] = stored;
@@ -430,16 +517,7 @@ template<uint32_t T, bool SINGLE> __global__ void __launch_bounds__(T) k_tile_wa
         const uint32_t v = s_mark[m];
         if (v & kUsed)
             {
-            // the union cell holding candidate m: largest u with s_off[u] <= m
-            uint32_t lo = 0, hi = U;
-            while (hi - lo > 1u)
-                {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (s_off[mid] <= m)
-                    lo = mid;
-                else
-                    hi = mid;
-                }
+            const uint32_t lo = union_cell_of(s_off, U, m);
             a.tile_jset[(size_t)t * kTileCap + (v & ~(uint32_t)kUsed)] = a.members[s_gs[lo] + (m - s_off[lo])];
             }
         }
@@ -522,6 +600,8 @@ int tiles_walk(TileList& t, uint32_t tile_size, const TileWalkArgs& args, hipStr
     {
     if (tile_size != 256 && tile_size != 512)
         return set_error(AZP_ERR_INVALID, "tile size %u must be 256 or 512", tile_size);
+    if (args.ntypes == 0 || args.ntypes > kTileWalkTypes)
+        return set_error(AZP_ERR_UNSUPPORTED, "the per-tile walk serves 1 to %u particle types", kTileWalkTypes);
     const uint32_t ntiles = (args.N + tile_size - 1) / tile_size;
     const uint32_t words = (((args.max_neighbors + 7u) & ~7u) + 1u) / 2u;
     AZP_TRY(t.count.reserve(ntiles));

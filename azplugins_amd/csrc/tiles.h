@@ -75,6 +75,7 @@ struct TileWalkArgs
     CellGrid g;
     const uint32_t* cell_of;   // geometric cell of each particle
     const uint32_t* cell_rank; // Morton rank of each geometric cell
+    const uint32_t* rank_cell; // geometric cell of each Morton rank
     const uint32_t* start;     // slots of rank r: start[r] .. start[r + 1]
     const uint32_t* members;
     const int4* cell_fixed;
@@ -101,6 +102,9 @@ struct TileWalkArgs
 // into t's arrays (sized here); the caller reads args.flags afterwards and
 // sets t.size / t.slots / t.valid when flags[3] is clear.
 int tiles_walk(TileList& t, uint32_t tile_size, const TileWalkArgs& args, hipStream_t stream);
+
+// Most particle types tiles_walk serves.
+constexpr uint32_t kTileWalkMaxTypes = 8;
 
 // Enqueues the expansion of a valid slot-ordered tiled list into the plain
 // column list (d_nlist[k * pitch + i], padding entries hold i).
