@@ -45,6 +45,8 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n)
 
 // cell_of[i] is the geometric cell index; the cell's members occupy slots
 // start[cell_rank[c]] .. start[cell_rank[c] + 1] (Morton order of cells).
+// The count is aggregated per wavefront: one atomic per distinct cell among
+// its 64 particles (neighbors in memory usually share a cell).
 __global__ void k_bin(const double4* __restrict__ pos,
                       uint32_t N,
                       DevBox box,
@@ -55,15 +57,33 @@ __global__ void k_bin(const double4* __restrict__ pos,
                       uint32_t* __restrict__ count)
     {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N)
-        return;
-    const double4 p = pos[i];
-    const int cx = cell_coord(p.x, box.L[0], box.Linv[0], g.n[0]);
-    const int cy = cell_coord(p.y, box.L[1], box.Linv[1], g.n[1]);
-    const int cz = cell_coord(p.z, box.L[2], box.Linv[2], g.n[2]);
-    const uint32_t c = (uint32_t)((cz * g.n[1] + cy) * g.n[0] + cx);
-    cell_of[i] = c;
-    slot_of[i] = atomicAdd(&count[cell_rank[c]], 1u);
+    const uint32_t lane = threadIdx.x & 63u;
+    const bool active = i < N;
+    uint32_t r = 0xffffffffu;
+    if (active)
+        {
+        const double4 p = pos[i];
+        const int cx = cell_coord(p.x, box.L[0], box.Linv[0], g.n[0]);
+        const int cy = cell_coord(p.y, box.L[1], box.Linv[1], g.n[1]);
+        const int cz = cell_coord(p.z, box.L[2], box.Linv[2], g.n[2]);
+        const uint32_t c = (uint32_t)((cz * g.n[1] + cy) * g.n[0] + cx);
+        cell_of[i] = c;
+        r = cell_rank[c];
+        }
+    uint64_t pending = __ballot(active);
+    while (pending != 0ull)
+        {
+        const uint32_t leader = (uint32_t)__ffsll((unsigned long long)pending) - 1u;
+        const uint32_t lr = (uint32_t)__shfl((int)r, (int)leader);
+        const uint64_t same = __ballot(active && r == lr);
+        uint32_t base = 0;
+        if (lane == leader)
+            base = atomicAdd(&count[lr], (uint32_t)__popcll(same));
+        base = (uint32_t)__shfl((int)base, (int)leader);
+        if (active && r == lr)
+            slot_of[i] = base + (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
+        pending &= ~same;
+        }
     }
 
 __global__ void k_scatter(uint32_t N,

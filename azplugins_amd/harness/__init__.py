@@ -83,6 +83,15 @@ class Simulation:
         self.state = State(self, snapshot)
         return self.state
 
+    def _needs_virial(self):
+        """Whether force computes fill the virial: a thermodynamic compute is
+        attached or a force's virials were read (HOOMD requests the pressure
+        tensor the same way)."""
+        from .md.compute import ThermodynamicQuantities
+
+        return getattr(self, "_virial_requested", False) or any(
+            isinstance(c, ThermodynamicQuantities) for c in self.operations.computes)
+
     def _attach(self):
         if self.state is None:
             raise RuntimeError("create the state before running")

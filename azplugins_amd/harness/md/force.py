@@ -53,6 +53,11 @@ class Force:
     @property
     def virials(self):
         self._require()
+        if not getattr(self, "_virial_current", True):
+            # computed without the virial (nothing asked for it): from now on
+            # every step computes it, and this step's is computed here
+            self._sim._virial_requested = True
+            self._compute(self._sim.timestep, virial=True)
         N = self._sim.state.N
         return self._virial[:, :N].T.cpu().numpy()
 

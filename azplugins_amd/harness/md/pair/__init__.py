@@ -127,12 +127,15 @@ class Pair(Force):
     def _thermostat(self, timestep):
         """Hook for pair forces with a thermostat (DPD)."""
 
-    def _compute(self, timestep):
+    def _compute(self, timestep, virial=None):
         self.nlist._update(timestep)
         self._thermostat(timestep)
+        # the virial only when something reads it (HOOMD's pressure_tensor flag)
+        virial = self._sim._needs_virial() if virial is None else virial
         self._cpp.compute(
-            self.nlist.handle, self._sim.state.particles(), self._outputs(), timestep, compute_virial=True
+            self.nlist.handle, self._sim.state.particles(), self._outputs(), timestep, compute_virial=virial
         )
+        self._virial_current = bool(virial)
 
 
 from . import aniso  # noqa: E402,F401

@@ -1,0 +1,51 @@
+"""Drop-in harness behaviour around the pair forces.
+
+Force computes fill the per-particle virial only when something reads it (a
+ThermodynamicQuantities compute, or a force's ``virials``), as HOOMD computes
+it only under its pressure-tensor flag; reading ``virials`` of a force that was
+computed without it computes it for the current state.
+"""
+
+import numpy as np
+import pytest
+
+from gpu_util import GpuSystem, gpu_pair, jittered_lattice, say
+
+pytestmark = pytest.mark.gpu
+
+
+def _sim(pos4, L, precision):
+    import azplugins_amd.harness as hoomd
+
+    snap = hoomd.Snapshot()
+    snap.configuration.box = [L, L, L, 0, 0, 0]
+    snap.particles.N = pos4.shape[0]
+    snap.particles.position[:] = pos4[:, :3]
+    sim = hoomd.Simulation(seed=3)
+    sim.create_state_from_snapshot(snap)
+    integ = hoomd.md.Integrator(dt=0.005)
+    nl = hoomd.md.nlist.Cell(buffer=0.4)
+    plj = hoomd.azplugins.pair.PerturbedLennardJones(nlist=nl, default_r_cut=3.0)
+    plj.params[("A", "A")] = dict(epsilon=1.0, sigma=1.0, attraction_scale_factor=0.5)
+    plj.precision = precision
+    integ.forces = [plj]
+    integ.methods = [hoomd.md.methods.ConstantVolume(hoomd.filter.All())]
+    sim.operations.integrator = integ
+    return sim, plj
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_virials_on_demand(precision):
+    L = 12.0
+    pos4, _ = jittered_lattice(12, L, 0.1, 1, seed=17)
+    sim, plj = _sim(pos4, L, precision)
+    sim.run(0)
+    assert not sim._needs_virial()
+    got = plj.virials
+    assert sim._needs_virial()
+    ref = gpu_pair("PotentialPairPerturbedLennardJones", GpuSystem(pos4, np.array([L, L, L])), 1,
+                   {(0, 0): {"epsilon": 1.0, "sigma": 1.0, "attraction_scale_factor": 0.5}}, 3.0,
+                   precision=precision, tiled=(precision == "mixed"))
+    say(f"[harness] {precision}: on-demand virial max |v| {np.abs(got).max():.3e}")
+    assert np.array_equal(got, ref.virial.T)
+    assert np.array_equal(plj.forces, ref.force[:, :3])
