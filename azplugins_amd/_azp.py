@@ -185,6 +185,7 @@ SIGNATURES = [
                                  ctypes.POINTER(_u32), ctypes.POINTER(_u32)]),
     ("azp_nlist_get_tile_order", _i, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
     ("azp_nlist_walk_kind", _i, [_vp, ctypes.POINTER(_i)]),
+    ("azp_nlist_tile_stats", _i, [_vp, ctypes.POINTER(_u32), ctypes.POINTER(_u32)]),
     ("azp_pair_create", _i, [ctypes.POINTER(_vp), ctypes.c_char_p, _u32]),
     ("azp_pair_destroy", _i, [_vp]),
     ("azp_pair_set_params", _i, [_vp, _u32, _u32, _pd]),

@@ -182,6 +182,13 @@ class NeighborListGPU:
         _azp.call("azp_nlist_walk_kind", self._h, ctypes.byref(k))
         return k.value
 
+    def tile_stats(self):
+        """(blocks, blocks walked in FP32 offsets) of the last per-block walk."""
+        n = ctypes.c_uint32()
+        r = ctypes.c_uint32()
+        _azp.call("azp_nlist_tile_stats", self._h, ctypes.byref(n), ctypes.byref(r))
+        return n.value, r.value
+
     def download(self, N):
         """(n_neigh[N], list[max_neighbors, pitch]) as numpy arrays."""
         import numpy as np

@@ -401,7 +401,8 @@ struct azp_nlist
     DeviceArray<uint32_t> cell_rank; // Morton rank of each cell of the grid below
     DeviceArray<uint32_t> rank_cell; // its inverse
     int rank_grid[3] = {0, 0, 0};
-    DeviceArray<uint32_t> flags; // [0] overflow, [1] rebuild, [2] bad type, [3] tile misfit, [4] largest tile set
+    DeviceArray<uint32_t> flags; // [0] overflow, [1] rebuild, [2] bad type, [3] tile misfit, [4] largest tile set,
+                                 // [5] tiles walked in FP32 offsets
     uint32_t* h_flags = nullptr;  // pinned, 8 words ([4], [5] tile build flags)
 
     bool want_tiles = false;
@@ -411,6 +412,7 @@ struct azp_nlist
     // from it when a consumer asks for it (plain_ready false until then).
     bool plain_ready = true;
     bool fused = false; // the last build used the fused walk
+    uint32_t relative_tiles = 0; // of which tiles were walked in FP32 offsets
     hipStream_t last_stream = nullptr;
     uint32_t walk_skip[2] = {0, 0}; // builds to skip the fused walk of 512 / 256 after a tile did not fit
 
@@ -456,7 +458,7 @@ static int nlist_build(azp_nlist* nl, const azp_particles* p, hipStream_t s)
     AZP_TRY(nl->rank_of.reserve(N));
     AZP_TRY(nl->n_neigh.reserve(N));
     AZP_TRY(nl->last_pos.reserve(N));
-    AZP_TRY(nl->flags.reserve(5));
+    AZP_TRY(nl->flags.reserve(6));
 
     if (g.n[0] != nl->rank_grid[0] || g.n[1] != nl->rank_grid[1] || g.n[2] != nl->rank_grid[2])
         {
@@ -521,7 +523,7 @@ static int nlist_build(azp_nlist* nl, const azp_particles* p, hipStream_t s)
                     --skip;
                     continue;
                     }
-                hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, nl->flags.ptr, 0u, (size_t)5);
+                hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, nl->flags.ptr, 0u, (size_t)6);
                 TileWalkArgs ta;
                 ta.pos = pos;
                 ta.N = N;
@@ -539,6 +541,7 @@ static int nlist_build(azp_nlist* nl, const azp_particles* p, hipStream_t s)
                 ta.r_listsq = nl->r_listsq.ptr;
                 ta.r_listsq_lo = nl->r_listsq_lo.ptr;
                 ta.r_listsq_hi = nl->r_listsq_hi.ptr;
+                ta.r_list_max = rmax;
                 ta.ntypes = nl->ntypes;
                 ta.n_neigh = nl->n_neigh.ptr;
                 ta.rows = reinterpret_cast<uint16_t*>(nl->rows.ptr);
@@ -546,7 +549,7 @@ static int nlist_build(azp_nlist* nl, const azp_particles* p, hipStream_t s)
                 ta.flags = nl->flags.ptr;
                 ta.pitch = nl->pitch;
                 AZP_TRY(tiles_walk(nl->tiles, ts, ta, s));
-                AZP_HIP(hipMemcpyAsync(nl->h_flags, nl->flags.ptr, 5 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+                AZP_HIP(hipMemcpyAsync(nl->h_flags, nl->flags.ptr, 6 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
                 AZP_HIP(hipStreamSynchronize(s));
                 if (nl->h_flags[2] & 2u)
                     return set_error(AZP_ERR_INVALID, "a particle position is not finite (NaN or inf)");
@@ -566,6 +569,7 @@ static int nlist_build(azp_nlist* nl, const azp_particles* p, hipStream_t s)
                     nl->tiles.valid = true;
                     nl->plain_ready = false;
                     nl->fused = true;
+                    nl->relative_tiles = nl->h_flags[5];
                     done = true;
                     break;
                     }
@@ -913,6 +917,16 @@ extern "C" int azp_nlist_walk_kind(const azp_nlist* nl, int* kind)
     if (!nl || !kind)
         return set_error(AZP_ERR_INVALID, "null argument");
     *kind = nl->fused ? 1 : 0;
+    return AZP_OK;
+    }
+
+extern "C" int azp_nlist_tile_stats(const azp_nlist* nl, uint32_t* ntiles, uint32_t* relative_tiles)
+    {
+    if (!nl || !ntiles || !relative_tiles)
+        return set_error(AZP_ERR_INVALID, "null argument");
+    const bool ok = nl->fused && nl->tiles.valid;
+    *ntiles = ok ? nl->tiles.ntiles : 0u;
+    *relative_tiles = ok ? nl->relative_tiles : 0u;
     return AZP_OK;
     }
 

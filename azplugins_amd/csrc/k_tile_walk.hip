@@ -149,15 +149,29 @@ struct TileWalkLds
 // the fused walk's type-pair table lives in LDS
 constexpr uint32_t kTileWalkTypes = kTileWalkMaxTypes;
 
+// How a tile's candidates are staged and tested.
+enum WalkMode
+    {
+    kWalkFixed = 0,      // wrapped fixed-point positions, r^2 in physical units
+    kWalkFixedCubic = 1, // the same, r^2 in fixed-point units (cubic box)
+    kWalkRelative = 2,   // FP32 offsets from a tile anchor in physical units
+    };
+
 // Lists particle i against the staged candidates; returns its neighbor
 // count.  Entries are LDS candidate indices, appended to the lane's 16-entry
 // LDS ring and stored to its row eight at a time (16 bytes).  Candidates are
 // read four at a time (the staging array has room for the three past a
 // cell's end); the per-candidate tests are branch-free, and only the rare
-// candidates near r_list take the FP64 branch.  With CUBIC the squared
-// distance is formed in fixed-point units (L / 2^32 on every axis) and
-// compared with r_list^2 in those units.
-template<bool SINGLE, bool CUBIC>
+// candidates near r_list take the FP64 branch.
+//
+// kWalkRelative: every staged position is its minimum-image offset from the
+// tile's anchor in FP32 (physical units), and the lane's own position is its
+// own staged entry, so a displacement is one FP32 subtraction per axis.  The
+// tile uses it only when all its candidate cells lie within L/4 of the
+// anchor on every axis, where the difference of two such offsets is the
+// minimum image, and within 8 r_list, where the FP32 error of r^2 stays
+// inside the 1e-5 band.
+template<bool SINGLE, int MODE>
 __device__ uint32_t tile_walk_one(const TileWalkArgs& a,
                                   const TileWalkLds& s,
                                   uint32_t slot,
@@ -173,6 +187,8 @@ __device__ uint32_t tile_walk_one(const TileWalkArgs& a,
         atomicOr(a.flags + 2, finite ? 1u : 2u);
         return 0;
         }
+    constexpr bool CUBIC = MODE == kWalkFixedCubic;
+    constexpr bool REL = MODE == kWalkRelative;
     const int32_t fxi = to_fixed(pi.x, a.box.Linv[0]);
     const int32_t fyi = to_fixed(pi.y, a.box.Linv[1]);
     const int32_t fzi = to_fixed(pi.z, a.box.Linv[2]);
@@ -182,6 +198,8 @@ __device__ uint32_t tile_walk_one(const TileWalkArgs& a,
     const uint32_t* stencil = s.stencil + own * 27u;
     const uint32_t own_first = stencil[13] & 0xffffu;
     const uint32_t self = own_first + (slot - s.gs[union_cell_of(s.off, s.U, own_first)]);
+    const int4 me = s.cand[self];
+    const float rxi = __int_as_float(me.x), ryi = __int_as_float(me.y), rzi = __int_as_float(me.z);
     const double rl0 = a.r_listsq[0];
     // wave-uniform, held in scalar registers: the candidate loop then never
     // waits on the vector memory counter (which the row stores also use)
@@ -193,77 +211,109 @@ __device__ uint32_t tile_walk_one(const TileWalkArgs& a,
     uint16_t* row = a.rows + (size_t)slot * a.max_neighbors;
     uint32_t count = 0;
     // the next cell's range and first candidates are read one step ahead
+    // one batch: four candidates from m0 (entries past the cell's end masked)
+    auto batch = [&](const int4 (&pc)[4], uint32_t m0, uint32_t e) {
+        bool hit[4], band[4];
+#pragma unroll
+        for (uint32_t q = 0; q < 4; ++q)
+            {
+            const uint32_t m = m0 + q;
+            const uint32_t tj = (uint32_t)pc[q].w;
+            // with one type, a candidate of an invalid type can only exist
+            // when its own lane reports it, which fails the build
+            const bool valid = (m < e) & (m != self) & (SINGLE || tj < a.ntypes);
+            const uint32_t tp = SINGLE ? 0u : ti * a.ntypes + min(tj, a.ntypes - 1);
+            const float lo = SINGLE ? lo0 : s.lohi[tp].x * unit2;
+            const float hi = SINGLE ? hi0 : s.lohi[tp].y * unit2;
+            float dx, dy, dz;
+            if constexpr (REL)
+                {
+                dx = __int_as_float(pc[q].x) - rxi;
+                dy = __int_as_float(pc[q].y) - ryi;
+                dz = __int_as_float(pc[q].z) - rzi;
+                }
+            else
+                {
+                dx = (float)fixed_delta(fxi, pc[q].x) * (CUBIC ? 1.0f : sx);
+                dy = (float)fixed_delta(fyi, pc[q].y) * (CUBIC ? 1.0f : sy);
+                dz = (float)fixed_delta(fzi, pc[q].z) * (CUBIC ? 1.0f : sz);
+                }
+            // FP32 rounding here is far inside the 1e-5 band
+            const float rsq = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+            const bool below = rsq < lo;
+            hit[q] = valid && below;
+            band[q] = valid && !below && rsq <= hi;
+            }
+        if (band[0] || band[1] || band[2] || band[3])
+            {
+            // FP32 cannot decide: the FP64 predicate of the host oracle
+#pragma unroll
+            for (uint32_t q = 0; q < 4; ++q)
+                if (band[q])
+                    {
+                    const uint32_t m = m0 + q;
+                    const uint32_t u = union_cell_of(s.off, s.U, m);
+                    const uint32_t j = a.members[s.gs[u] + (m - s.off[u])];
+                    const double4 p64 = a.pos[j];
+                    const uint32_t tp = SINGLE ? 0u : ti * a.ntypes + (uint32_t)pc[q].w;
+                    const double rl = SINGLE ? rl0 : a.r_listsq[tp];
+                    const double qx = min_image_1d(pi.x - p64.x, a.box.L[0], a.box.Linv[0]);
+                    const double qy = min_image_1d(pi.y - p64.y, a.box.L[1], a.box.Linv[1]);
+                    const double qz = min_image_1d(pi.z - p64.z, a.box.L[2], a.box.Linv[2]);
+                    hit[q] = rsq3_nofma(qx, qy, qz) <= rl;
+                    }
+            }
+        // append the hits in candidate order: positions are a running sum
+        // of the hit flags; the marks share one base address
+        uint16_t* mark = s.mark + m0;
+        uint32_t pos = count;
+#pragma unroll
+        for (uint32_t q = 0; q < 4; ++q)
+            {
+            if (hit[q])
+                {
+                mark[q] = 1u;
+                ring16[pos & 15u] = (uint16_t)(m0 + q);
+                }
+            pos += (uint32_t)hit[q];
+            }
+        // a completed group of eight leaves as one 16-byte store
+        if (((pos ^ count) & ~7u) != 0u && count < a.max_neighbors)
+            {
+            const uint2* h = reinterpret_cast<const uint2*>(ring + ((count >> 1) & 4u));
+            const uint2 w0 = h[0], w1 = h[1];
+            *reinterpret_cast<uint4*>(row + (count & ~7u)) = make_uint4(w0.x, w0.y, w1.x, w1.y);
+            }
+        count = pos;
+    };
+    auto load4 = [&](int4 (&pc)[4], uint32_t m0) {
+#pragma unroll
+        for (uint32_t q = 0; q < 4; ++q)
+            pc[q] = s.cand[m0 + q];
+    };
+    // Two batches per iteration in two register sets: the reads of the next
+    // batch are issued before the current one is tested (the scheduling
+    // barriers keep them there), so LDS latency overlaps the tests.
     uint32_t range_next = stencil[0];
     for (uint32_t k = 0; k < 27u; ++k)
         {
         const uint32_t range = range_next;
         range_next = stencil[min(k + 1u, 26u)];
         const uint32_t b = range & 0xffffu, e = range >> 16;
-        int4 ahead[4] = {s.cand[b], s.cand[b + 1], s.cand[b + 2], s.cand[b + 3]};
-        for (uint32_t m0 = b; m0 < e; m0 += 4)
-                    {
-                    const int4 pc[4] = {ahead[0], ahead[1], ahead[2], ahead[3]};
-#pragma unroll
-                    for (uint32_t q = 0; q < 4; ++q)
-                        ahead[q] = s.cand[m0 + 4 + q];
-                    bool hit[4], band[4];
-#pragma unroll
-                    for (uint32_t q = 0; q < 4; ++q)
-                        {
-                        const uint32_t m = m0 + q;
-                        const uint32_t tj = (uint32_t)pc[q].w;
-                        const bool valid = (m < e) & (m != self) & (tj < a.ntypes);
-                        const uint32_t tp = SINGLE ? 0u : ti * a.ntypes + min(tj, a.ntypes - 1);
-                        const float lo = SINGLE ? lo0 : s.lohi[tp].x * unit2;
-                        const float hi = SINGLE ? hi0 : s.lohi[tp].y * unit2;
-                        const float dx = (float)fixed_delta(fxi, pc[q].x) * (CUBIC ? 1.0f : sx);
-                        const float dy = (float)fixed_delta(fyi, pc[q].y) * (CUBIC ? 1.0f : sy);
-                        const float dz = (float)fixed_delta(fzi, pc[q].z) * (CUBIC ? 1.0f : sz);
-                        // FP32 rounding here is far inside the 1e-5 band
-                        const float rsq = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-                        hit[q] = valid & (rsq < lo);
-                        band[q] = valid & (rsq >= lo) & (rsq <= hi);
-                        }
-                    if (band[0] | band[1] | band[2] | band[3])
-                        {
-                        // FP32 cannot decide: the FP64 predicate of the host oracle
-#pragma unroll
-                        for (uint32_t q = 0; q < 4; ++q)
-                            if (band[q])
-                                {
-                                const uint32_t m = m0 + q;
-                                const uint32_t u = union_cell_of(s.off, s.U, m);
-                                const uint32_t j = a.members[s.gs[u] + (m - s.off[u])];
-                                const double4 p64 = a.pos[j];
-                                const uint32_t tp = SINGLE ? 0u : ti * a.ntypes + (uint32_t)pc[q].w;
-                                const double rl = SINGLE ? rl0 : a.r_listsq[tp];
-                                const double qx = min_image_1d(pi.x - p64.x, a.box.L[0], a.box.Linv[0]);
-                                const double qy = min_image_1d(pi.y - p64.y, a.box.L[1], a.box.Linv[1]);
-                                const double qz = min_image_1d(pi.z - p64.z, a.box.L[2], a.box.Linv[2]);
-                                hit[q] = rsq3_nofma(qx, qy, qz) <= rl;
-                                }
-                        }
-                    // append the hits in candidate order
-                    uint32_t pos = count;
-#pragma unroll
-                    for (uint32_t q = 0; q < 4; ++q)
-                        {
-                        if (hit[q])
-                            {
-                            s.mark[m0 + q] = 1u;
-                            ring16[pos & 15u] = (uint16_t)(m0 + q);
-                            }
-                        pos += hit[q] ? 1u : 0u;
-                        }
-                    // a completed group of eight leaves as one 16-byte store
-                    if (((pos ^ count) & ~7u) != 0u && count < a.max_neighbors)
-                        {
-                        const uint2* h = reinterpret_cast<const uint2*>(ring + ((count >> 1) & 4u));
-                        const uint2 w0 = h[0], w1 = h[1];
-                        *reinterpret_cast<uint4*>(row + (count & ~7u)) = make_uint4(w0.x, w0.y, w1.x, w1.y);
-                        }
-                    count = pos;
-                    }
+        int4 pa[4], pb[4];
+        load4(pa, b);
+        for (uint32_t m0 = b; m0 < e; m0 += 8)
+            {
+            load4(pb, m0 + 4);
+            __builtin_amdgcn_sched_barrier(0);
+            batch(pa, m0, e);
+            if (m0 + 4 < e)
+                {
+                load4(pa, m0 + 8);
+                __builtin_amdgcn_sched_barrier(0);
+                batch(pb, m0 + 4, e);
+                }
+            }
         }
     if ((count & 7u) != 0u && count < a.max_neighbors)
         {
@@ -277,7 +327,7 @@ __device__ uint32_t tile_walk_one(const TileWalkArgs& a,
 template<uint32_t T, bool SINGLE> __global__ void __launch_bounds__(T) k_tile_walk(const TileWalkArgs a)
     {
     constexpr uint32_t C = TileWalkShape<T>::cand;
-    __shared__ int4 s_cand[C + 8];
+    __shared__ int4 s_cand[C + 12];
     __shared__ uint16_t s_mark[C];
     __shared__ uint64_t s_tab[kCellHash];
     __shared__ uint64_t s_ucell[kUnionCells];
@@ -287,7 +337,7 @@ template<uint32_t T, bool SINGLE> __global__ void __launch_bounds__(T) k_tile_wa
     __shared__ uint32_t s_stencil[kOwnCells * 27];
     __shared__ uint32_t s_wave[T / 64];
     __shared__ float2 s_lohi[SINGLE ? 1 : kTileWalkTypes * kTileWalkTypes];
-    __shared__ uint32_t s_n, s_bad, s_rfirst, s_rlast;
+    __shared__ uint32_t s_n, s_bad, s_rfirst, s_rlast, s_ext[3];
 
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
     const uint32_t t = xcd_logical_block();
@@ -304,6 +354,7 @@ template<uint32_t T, bool SINGLE> __global__ void __launch_bounds__(T) k_tile_wa
         {
         s_n = 0u;
         s_bad = 0u;
+        s_ext[0] = s_ext[1] = s_ext[2] = 0u;
         }
     __syncthreads();
 
@@ -389,7 +440,10 @@ template<uint32_t T, bool SINGLE> __global__ void __launch_bounds__(T) k_tile_wa
             __syncthreads();
             }
 
-    // 2. LDS offsets of the union cells' members
+    // 2. LDS offsets of the union cells' members; the anchor of the FP32
+    // offsets is the tile's middle particle
+    const uint32_t mid = t * T + (min(T, a.N - t * T) >> 1);
+    const uint32_t anchor_cell = a.cell_of[a.members[mid]];
     constexpr uint32_t QC = kUnionCells / T;
     uint32_t cnt[QC];
     uint32_t sum = 0;
@@ -404,6 +458,16 @@ template<uint32_t T, bool SINGLE> __global__ void __launch_bounds__(T) k_tile_wa
             const uint32_t b = a.start[r];
             s_gs[u] = b;
             cnt[q] = a.start[r + 1] - b;
+            // periodic distance in cells from the anchor's cell, per axis
+            const uint32_t cu = (uint32_t)s_ucell[u], ca = anchor_cell;
+            const uint32_t d[3] = {cu % nx > ca % nx ? cu % nx - ca % nx : ca % nx - cu % nx,
+                                   (cu / nx) % ny > (ca / nx) % ny ? (cu / nx) % ny - (ca / nx) % ny
+                                                                   : (ca / nx) % ny - (cu / nx) % ny,
+                                   cu / (nx * ny) > ca / (nx * ny) ? cu / (nx * ny) - ca / (nx * ny)
+                                                                   : ca / (nx * ny) - cu / (nx * ny)};
+            atomicMax(&s_ext[0], min(d[0], nx - d[0]));
+            atomicMax(&s_ext[1], min(d[1], ny - d[1]));
+            atomicMax(&s_ext[2], min(d[2], nz - d[2]));
             }
         sum += cnt[q];
         }
@@ -432,6 +496,23 @@ template<uint32_t T, bool SINGLE> __global__ void __launch_bounds__(T) k_tile_wa
         return; // Mc is the same for every thread
         }
     __syncthreads();
+    // FP32 offsets from the anchor (the tile's middle particle) when every
+    // candidate cell is within L/4 of the anchor's cell on every axis (so
+    // differences of offsets are minimum images) and within 8 r_list (so the
+    // FP32 error of r^2, below 6e-8 (13.9 R/r + 6.5) r^2 for offsets up to R,
+    // stays under the 1e-5 band)
+    const double reach = 8.0 * a.r_list_max;
+    const bool rel = (s_ext[0] + 1u) * 4u < nx && (s_ext[1] + 1u) * 4u < ny && (s_ext[2] + 1u) * 4u < nz
+                     && (s_ext[0] + 1u) * a.box.L[0] <= reach * nx && (s_ext[1] + 1u) * a.box.L[1] <= reach * ny
+                     && (s_ext[2] + 1u) * a.box.L[2] <= reach * nz;
+    const int4 anchor = a.cell_fixed[mid];
+    const float sx = a.fx_scale[0], sy = a.fx_scale[1], sz = a.fx_scale[2];
+    auto staged = [&](int4 v) {
+        return rel ? make_int4(__float_as_int((float)fixed_delta(v.x, anchor.x) * sx),
+                               __float_as_int((float)fixed_delta(v.y, anchor.y) * sy),
+                               __float_as_int((float)fixed_delta(v.z, anchor.z) * sz), v.w)
+                   : v;
+    };
     // stage the candidates: one wavefront per cell, four cells' loads in
     // flight at a time (cells past 64 members finish in a second loop)
     constexpr uint32_t NW = T / 64;
@@ -443,9 +524,9 @@ template<uint32_t T, bool SINGLE> __global__ void __launch_bounds__(T) k_tile_wa
             return;
         const uint32_t o = s_off[u], n = s_off[u + 1] - o;
         if (lane < n)
-            s_cand[o + lane] = v;
+            s_cand[o + lane] = staged(v);
         for (uint32_t k = lane + 64; k < n; k += 64)
-            s_cand[o + k] = a.cell_fixed[s_gs[u] + k];
+            s_cand[o + k] = staged(a.cell_fixed[s_gs[u] + k]);
     };
     for (uint32_t u0 = tid >> 6; u0 < U; u0 += 4 * NW)
         {
@@ -473,8 +554,9 @@ template<uint32_t T, bool SINGLE> __global__ void __launch_bounds__(T) k_tile_wa
         uint32_t* ring = s_ring + tid * kRingWords;
         const bool cubic = a.fx_scale[0] == a.fx_scale[1] && a.fx_scale[0] == a.fx_scale[2];
         const uint32_t own = crank - r_first;
-        count = cubic ? tile_walk_one<SINGLE, true>(a, s, slot, i, own, ring)
-                      : tile_walk_one<SINGLE, false>(a, s, slot, i, own, ring);
+        count = rel     ? tile_walk_one<SINGLE, kWalkRelative>(a, s, slot, i, own, ring)
+                : cubic ? tile_walk_one<SINGLE, kWalkFixedCubic>(a, s, slot, i, own, ring)
+                        : tile_walk_one<SINGLE, kWalkFixed>(a, s, slot, i, own, ring);
         const uint32_t stored = min(count, a.max_neighbors);
         a.n_neigh[i] = stored;
         a.tile_nn[slot] = stored;
@@ -525,6 +607,8 @@ template<uint32_t T, bool SINGLE> __global__ void __launch_bounds__(T) k_tile_wa
         {
         a.tile_count[t] = M;
         atomicMax(a.flags + 4, M);
+        if (rel)
+            atomicAdd(a.flags + 5, 1u);
         }
 
     // 5. ranks two per word, padded to the wavefront's longest list rounded

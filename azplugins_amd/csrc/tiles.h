@@ -83,14 +83,16 @@ struct TileWalkArgs
     const double* r_listsq;
     const float* r_listsq_lo;
     const float* r_listsq_hi;
+    double r_list_max;
     uint32_t ntypes;
     uint32_t* n_neigh;  // by particle
     uint32_t* tile_nn;  // by slot (set by tiles_walk)
     uint16_t* rows;     // by slot
     uint32_t max_neighbors; // a multiple of 8
     // [0] largest count above max_neighbors, [2] bad type (1) / position (2),
-    // [3] a tile did not fit (1: cells, 2: candidates, 4: neighbor set),
-    // [4] largest neighbor-set size
+    // [3] a tile did not fit (1: cells, 2: candidates, 4: neighbor set,
+    // 8: own cells), [4] largest neighbor-set size, [5] tiles walked with
+    // FP32 offsets from their anchor
     uint32_t* flags;
     uint32_t* tile_count;
     uint32_t* tile_jset;

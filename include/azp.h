@@ -361,6 +361,11 @@ int azp_nlist_get_tile_order(const azp_nlist* nl, const uint32_t** d_tile_order,
 /* How the last build made the list: 1 the per-block walk that writes the
  * tiled list directly, 0 the plain walk. */
 int azp_nlist_walk_kind(const azp_nlist* nl, int* kind);
+/* Diagnostics of the last per-block walk: its number of blocks, and how many
+ * of them tested candidates as FP32 offsets from a block anchor (blocks whose
+ * cells lie within L/4 and 8 r_list of the anchor) rather than as fixed-point
+ * differences; both 0 after a plain walk. */
+int azp_nlist_tile_stats(const azp_nlist* nl, uint32_t* ntiles, uint32_t* relative_tiles);
 /* Copy n_neigh[N] and the list (max_neighbors * pitch entries) to host memory
  * (synchronizes the device). */
 int azp_nlist_download(const azp_nlist* nl,

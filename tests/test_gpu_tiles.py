@@ -248,3 +248,25 @@ def test_fused_walk_forces_and_rebuild():
     assert tiled.nl.walk_kind() == 1 and tiled.nl.num_builds() == 2
     pairs = oracle.nlist(moved, Lv, 1, oracle.r_listsq_matrix([3.0], 0.4), "cell")
     assert nlist_pairs(tiled.nl, sysm.N) == set(zip(pairs[0].tolist(), pairs[1].tolist()))
+
+
+def test_fused_walk_relative_offsets_large_box():
+    """N = 262,144 at density 0.85 (19 cells per axis): most tiles lie within
+    L/4 and 8 r_list of their anchor and are walked in FP32 offsets.  Their
+    lists must still equal the plain walk's entry for entry, and the mixed
+    forces over them the plain-list ones."""
+    n = 64
+    L = (n**3 / 0.85) ** (1.0 / 3.0)
+    pos4, _ = jittered_lattice(n, L, 0.1, 1, seed=64)
+    sysm = GpuSystem(pos4, np.array([L, L, L]))
+    plain, tiled = _both("PotentialPairPerturbedLennardJones", sysm, 1, PLJ, 3.0)
+    ntiles, rel = tiled.nl.tile_stats()
+    say(f"[fused] N={sysm.N} L={L:.2f}: {rel} of {ntiles} tiles in FP32 offsets")
+    assert tiled.nl.walk_kind() == 1 and rel > ntiles // 4
+    nn_t, lst_t = tiled.nl.download(sysm.N)
+    nn_p, lst_p = plain.nl.download(sysm.N)
+    assert np.array_equal(nn_t, nn_p)
+    mask = np.arange(lst_t.shape[0])[:, None] < nn_t[None, :]
+    assert np.array_equal(lst_t[:, : sysm.N][mask], lst_p[:, : sysm.N][mask])
+    assert np.array_equal(plain.force, tiled.force)
+    assert np.array_equal(plain.virial, tiled.virial)
