@@ -233,6 +233,18 @@ def main():
         "kernel_us": round(k_mixed * 1e6, 2),
         "algorithmic_bytes": bytes_mixed,
     }
+    # the kernel is bound by VALU issue: its wavefront VALU instructions per
+    # launch (rocprofv3 counters) over the live kernel time, against the
+    # non-packed VALU issue peak
+    valu = load_valu()
+    if valu is not None and tiles[3]:
+        lane_ops = valu["insts_valu"] * 64
+        roofline["valu"] = {
+            "instructions_per_launch": valu["insts_valu"],
+            "achieved_lane_ops_per_s": lane_ops / k_mixed,
+            "peak_lane_ops_per_s": valu["peak"],
+            "frac": round(lane_ops / k_mixed / valu["peak"], 4),
+        }
 
     out = {
         "metric": METRIC,
@@ -564,6 +576,19 @@ def cpu_baseline(cells):
     res["hertz_4096_nve"] = {"value": 1000 / th, "unit": "timesteps/s", "steps": 1000, "cores": 1}
     log(f"bench: CPU oracle Hertz 4096 NVE {1000 / th:.1f} steps/s")
     return res
+
+
+def load_valu():
+    """VALU instructions per launch of the tiled mixed kernel and the issue peak, if measured."""
+    p = os.path.join(ROOT, "profiles", "pmc_issue_plj_1m.json")
+    if not os.path.exists(p):
+        return None
+    with open(p) as fh:
+        d = json.load(fh)
+    for name, v in d.get("kernels", {}).items():
+        if "k_pair<azp::EvalPerturbedLJ, true, true, true, false, true, true>" in name:
+            return {"insts_valu": v["SQ_INSTS_VALU"], "peak": d["peak_valu_lane_ops_per_s"]}
+    return None
 
 
 def load_traffic():
