@@ -802,7 +802,7 @@ static int nlist_expand(const azp_nlist* cnl, hipStream_t s)
     azp_nlist* nl = const_cast<azp_nlist*>(cnl);
     if (nl->plain_ready)
         return AZP_OK;
-    AZP_TRY(tiles_to_plain(nl->tiles, nl->built_N, nl->n_neigh.ptr, nl->rank_of.ptr, nl->pitch, nl->list.ptr, s));
+    AZP_TRY(tiles_to_plain(nl->tiles, nl->built_N, nl->n_neigh.ptr, nl->pitch, nl->list.ptr, s));
     nl->plain_ready = true;
     return AZP_OK;
     }

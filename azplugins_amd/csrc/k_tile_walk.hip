@@ -557,9 +557,7 @@ template<uint32_t T, bool SINGLE> __global__ void __launch_bounds__(T) k_tile_wa
         count = rel     ? tile_walk_one<SINGLE, kWalkRelative>(a, s, slot, i, own, ring)
                 : cubic ? tile_walk_one<SINGLE, kWalkFixedCubic>(a, s, slot, i, own, ring)
                         : tile_walk_one<SINGLE, kWalkFixed>(a, s, slot, i, own, ring);
-        const uint32_t stored = min(count, a.max_neighbors);
-        a.n_neigh[i] = stored;
-        a.tile_nn[slot] = stored;
+        a.n_neigh[i] = min(count, a.max_neighbors);
         if (count > a.max_neighbors)
             atomicMax(a.flags, count);
         }
@@ -611,15 +609,41 @@ template<uint32_t T, bool SINGLE> __global__ void __launch_bounds__(T) k_tile_wa
             atomicAdd(a.flags + 5, 1u);
         }
 
-    // 5. ranks two per word, padded to the wavefront's longest list rounded
-    // up to 8 with the tile's count
+    // 5. columns ordered by list length (longest first, ties by slot), so the
+    // pair kernel's 64-lane wavefronts run lists of nearly equal length: a
+    // bitonic sort of (length, slot) keys in the staging space (free now)
     const uint32_t nn = active ? min(count, a.max_neighbors) : 0u;
-    uint32_t wmax = nn;
-    for (int off = 32; off > 0; off >>= 1)
-        wmax = max(wmax, (uint32_t)__shfl_xor((int)wmax, off));
-    const uint32_t P = (wmax + 7u) & ~7u;
+    uint32_t* s_sort = reinterpret_cast<uint32_t*>(s_cand);
+    uint32_t* s_where = s_sort + T;
+    s_sort[tid] = (min(nn, 0xffffu) << 16) | (0xffffu - tid);
+    __syncthreads();
+    for (uint32_t size = 2; size <= T; size <<= 1)
+        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1)
+            {
+            const uint32_t p = tid ^ stride;
+            if (p > tid)
+                {
+                const bool down = (tid & size) == 0u; // descending overall
+                const uint32_t x = s_sort[tid], y = s_sort[p];
+                if ((x < y) == down)
+                    {
+                    s_sort[tid] = y;
+                    s_sort[p] = x;
+                    }
+                }
+            __syncthreads();
+            }
+    s_where[0xffffu - (s_sort[tid] & 0xffffu)] = tid;
+    __syncthreads();
+    const uint32_t place = s_where[tid];
+    const uint32_t col = t * T + place;
+    // the column wavefront's longest list: its first (longest) column's
+    const uint32_t P = (((s_sort[place & ~63u] >> 16) + 7u) & ~7u);
     if (!active)
         return;
+    a.tile_nn[col] = nn;
+    a.tile_order[col] = i;
+    a.tile_col[i] = col;
     // the row is reread in 16-byte chunks (8 entries), four in flight
     const uint4* row = reinterpret_cast<const uint4*>(a.rows + (size_t)slot * a.max_neighbors);
     auto chunk_load = [&](uint32_t kc) { return kc < nn ? row[kc / 8] : make_uint4(0u, 0u, 0u, 0u); };
@@ -633,7 +657,7 @@ template<uint32_t T, bool SINGLE> __global__ void __launch_bounds__(T) k_tile_wa
         const uint32_t w4[4] = {ch.x, ch.y, ch.z, ch.w};
 #pragma unroll
         for (uint32_t w = 0; w < 4; ++w)
-            a.tlist[(size_t)((kc >> 1) + w) * a.pitch + slot] =
+            a.tlist[(size_t)((kc >> 1) + w) * a.pitch + col] =
                 rank_of_entry(kc + 2 * w, w4[w], 0u) | (rank_of_entry(kc + 2 * w + 1, w4[w], 1u) << 16);
     };
     for (uint32_t k0 = 0; k0 < P; k0 += 32)
@@ -647,12 +671,13 @@ template<uint32_t T, bool SINGLE> __global__ void __launch_bounds__(T) k_tile_wa
         }
     }
 
-// Plain column list from a slot-ordered tiled list: lane i (a particle)
-// reads column rank_of[i]; entry k is jset[tile][rank], padding entries hold
-// i up to the longest list of the 64 particles rounded up to 8.
+// Plain column list from a tiled list of the per-tile walk: lane i (a
+// particle) reads its column tile_col[i]; entry k is jset[tile][rank],
+// padding entries hold i up to the longest list of the 64 particles rounded
+// up to 8.
 __global__ void __launch_bounds__(64) k_tiles_to_plain(uint32_t N,
                                                        const uint32_t* __restrict__ n_neigh,
-                                                       const uint32_t* __restrict__ rank_of,
+                                                       const uint32_t* __restrict__ tile_col,
                                                        uint32_t tile_size,
                                                        const uint32_t* __restrict__ tile_jset,
                                                        const uint32_t* __restrict__ tlist,
@@ -668,7 +693,7 @@ __global__ void __launch_bounds__(64) k_tiles_to_plain(uint32_t N,
     const uint32_t P = (wmax + 7u) & ~7u;
     if (!active)
         return;
-    const uint32_t s = rank_of[i];
+    const uint32_t s = tile_col[i];
     const uint32_t* js = tile_jset + (size_t)(s / tile_size) * kTileCap;
     for (uint32_t k = 0; k < P; k += 2)
         {
@@ -692,8 +717,12 @@ int tiles_walk(TileList& t, uint32_t tile_size, const TileWalkArgs& args, hipStr
     AZP_TRY(t.jset.reserve((size_t)ntiles * kTileCap));
     AZP_TRY(t.list.reserve((size_t)words * args.pitch));
     AZP_TRY(t.nn.reserve(args.N));
+    AZP_TRY(t.order_buf.reserve(args.N));
+    AZP_TRY(t.col.reserve(args.N));
     TileWalkArgs a = args;
     a.tile_nn = t.nn.ptr;
+    a.tile_order = t.order_buf.ptr;
+    a.tile_col = t.col.ptr;
     a.tile_count = t.count.ptr;
     a.tile_jset = t.jset.ptr;
     a.tlist = t.list.ptr;
@@ -714,23 +743,18 @@ int tiles_walk(TileList& t, uint32_t tile_size, const TileWalkArgs& args, hipStr
         }
     AZP_LAUNCH_CHECK();
     t.ntiles = ntiles;
-    t.order = args.members;
+    t.order = t.order_buf.ptr;
     return AZP_OK;
     }
 
-int tiles_to_plain(const TileList& t,
-                   uint32_t N,
-                   const uint32_t* d_n_neigh,
-                   const uint32_t* d_rank_of,
-                   uint32_t pitch,
-                   uint32_t* d_nlist,
+int tiles_to_plain(const TileList& t, uint32_t N, const uint32_t* d_n_neigh, uint32_t pitch, uint32_t* d_nlist,
                    hipStream_t s)
     {
     if (N == 0)
         return AZP_OK;
     if (!t.valid || !t.order)
-        return set_error(AZP_ERR_STATE, "no slot-ordered tiled list to expand");
-    hipLaunchKernelGGL(k_tiles_to_plain, dim3((N + 63) / 64), dim3(64), 0, s, N, d_n_neigh, d_rank_of, t.size,
+        return set_error(AZP_ERR_STATE, "no tiled list of the per-tile walk to expand");
+    hipLaunchKernelGGL(k_tiles_to_plain, dim3((N + 63) / 64), dim3(64), 0, s, N, d_n_neigh, t.col.ptr, t.size,
                        t.jset.ptr, t.list.ptr, pitch, d_nlist);
     AZP_LAUNCH_CHECK();
     return AZP_OK;

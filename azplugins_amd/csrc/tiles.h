@@ -32,11 +32,13 @@ struct TileList
     DeviceArray<uint32_t> jset;  // [ntiles * kTileCap]
     DeviceArray<uint32_t> list;  // [words * pitch]
     DeviceArray<uint32_t> flag;  // [0] largest distinct count above capacity, [1] largest count
-    DeviceArray<uint32_t> nn;    // slot-ordered tiles: list length of column s
-    // Slot-ordered tiles (the fused walk): column s of the list belongs to
-    // particle order[s] (the cell list's members) and has nn[s] entries.
-    // Null for particle-ordered tiles (column i is particle i, counts in the
-    // list's n_neigh).
+    DeviceArray<uint32_t> nn;        // per-tile walk: list length of column s
+    DeviceArray<uint32_t> order_buf; // per-tile walk: particle of column s
+    DeviceArray<uint32_t> col;       // per-tile walk: column of particle i
+    // Tiles of the per-tile walk: a tile's columns are its cell-list slots
+    // reordered by list length (longest first); column s belongs to particle
+    // order[s] and has nn[s] entries.  Null for tiles of consecutive
+    // particles (column i is particle i, counts in the list's n_neigh).
     const uint32_t* order = nullptr;
     uint32_t size = 0;           // columns per tile (0: no tiles)
     uint32_t slots = 0;          // LDS slots the force kernel needs: largest count + 1
@@ -85,9 +87,11 @@ struct TileWalkArgs
     const float* r_listsq_hi;
     double r_list_max;
     uint32_t ntypes;
-    uint32_t* n_neigh;  // by particle
-    uint32_t* tile_nn;  // by slot (set by tiles_walk)
-    uint16_t* rows;     // by slot
+    uint32_t* n_neigh;    // by particle
+    uint32_t* tile_nn;    // by column (set by tiles_walk)
+    uint32_t* tile_order; // particle of each column (set by tiles_walk)
+    uint32_t* tile_col;   // column of each particle (set by tiles_walk)
+    uint16_t* rows;       // by slot
     uint32_t max_neighbors; // a multiple of 8
     // [0] largest count above max_neighbors, [2] bad type (1) / position (2),
     // [3] a tile did not fit (1: cells, 2: candidates, 4: neighbor set,
@@ -108,14 +112,9 @@ int tiles_walk(TileList& t, uint32_t tile_size, const TileWalkArgs& args, hipStr
 // Most particle types tiles_walk serves.
 constexpr uint32_t kTileWalkMaxTypes = 8;
 
-// Enqueues the expansion of a valid slot-ordered tiled list into the plain
-// column list (d_nlist[k * pitch + i], padding entries hold i).
-int tiles_to_plain(const TileList& t,
-                   uint32_t N,
-                   const uint32_t* d_n_neigh,
-                   const uint32_t* d_rank_of,
-                   uint32_t pitch,
-                   uint32_t* d_nlist,
+// Enqueues the expansion of a valid tiled list of the per-tile walk into the
+// plain column list (d_nlist[k * pitch + i], padding entries hold i).
+int tiles_to_plain(const TileList& t, uint32_t N, const uint32_t* d_n_neigh, uint32_t pitch, uint32_t* d_nlist,
                    hipStream_t stream);
 
 } // namespace azp
