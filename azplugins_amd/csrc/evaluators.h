@@ -204,17 +204,20 @@ struct EvalPerturbedLJ
         }
     // The same with the cutoff mask folded in (FP32 form): both branches become
     // f *= s, e = e s + c with (s, c) = (1, wca_shift - eshift) inside the WCA
-    // radius, (asf, -eshift) outside, and (0, 0) for masked entries.
+    // radius and (asf, -eshift) outside.  A masked entry arrives with
+    // rsq = +inf: 1/rsq is then 0, so f and the LJ part of e are 0, and c is
+    // masked to 0.
     static constexpr bool kMasked = true;
     __device__ static void compute_masked(const Prep<float>& q, float rsq, bool in, float& fdivr, float& e)
         {
         const float r2inv = f_rcp(rsq);
         const float r6inv = r2inv * r2inv * r2inv;
         const bool wca = rsq < q.rwcasq;
-        const float s = in ? (wca ? 1.0f : q.asf) : 0.0f;
+        const float s = wca ? 1.0f : q.asf;
         const float c = in ? (wca ? q.wca_shift - q.eshift : -q.eshift) : 0.0f;
-        fdivr = (r2inv * r6inv * (q.lj1x12 * r6inv - q.lj2x6)) * s;
-        e = (r6inv * (q.lj1 * r6inv - q.lj2)) * s + c;
+        const float t = s * r6inv;
+        fdivr = (t * r2inv) * fmaf(q.lj1x12, r6inv, -q.lj2x6);
+        e = fmaf(t, fmaf(q.lj1, r6inv, -q.lj2), c);
         }
     };
 

@@ -14,7 +14,7 @@
 // Two precision forms:
 //   plain (FP64):  FP64 positions, min image, evaluator and accumulation.
 //   mixed:         positions as wrapped 32-bit fixed point (16-byte gather),
-//                  dx exact in integers then FP32, FP32 evaluator; every 16
+//                  dx exact in integers then FP32, FP32 evaluator; every 32
 //                  list entries are summed in FP32 and added to FP64
 //                  accumulators.  Pairs whose FP32 rsq lies within a relative
 //                  band of 1e-5 around rcutsq are re-tested on the FP64
@@ -28,6 +28,9 @@ namespace azp
 {
 
 constexpr float kCutBand = 1e-5f;
+// Mixed form: list entries summed in FP32 between additions to the FP64
+// accumulators (a multiple of the batch of 8).
+constexpr uint32_t kFlushEntries = 32;
 
 struct PairKernelArgs
     {
@@ -57,6 +60,7 @@ struct PairKernelArgs
     const uint32_t* tile_list;
     const uint32_t* tile_order;
     const uint32_t* tile_nn;
+    uint32_t tile_slots; // LDS position slots of the tiled kernel (the tables follow them)
     };
 
 // Position (double4 x, y, z, type) -> wrapped 32-bit fixed point of the box.
@@ -301,14 +305,15 @@ __device__ __forceinline__ void term_mixed(const PairKernelArgs& a,
         rsq = (float)r64;
         }
     in = in && Eval::active(q);
-    const float rs = in ? rsq : x.rcsq;
     float fdivr, e;
     if constexpr (HasMasked<Eval>::value && !XPLOR)
         {
-        Eval::compute_masked(q, rs, in, fdivr, e);
+        // masked entries at r^2 = +inf (the evaluator's terms vanish there)
+        Eval::compute_masked(q, in ? rsq : __builtin_huge_valf(), in, fdivr, e);
         accumulate32<VIRIAL>(acc, fdivr, fx, fy, fz, e);
         return;
         }
+    const float rs = in ? rsq : x.rcsq;
     if constexpr (NeedsRsq64<Eval>::value)
         {
         if (r64 < 0.0)
@@ -356,7 +361,9 @@ __global__ void __launch_bounds__(512) k_pair(const PairKernelArgs a)
     extern __shared__ double smem[];
     const uint32_t nt = a.ntypes;
     const uint32_t nt2 = nt * nt;
-    Prep* s_prep = reinterpret_cast<Prep*>(smem);
+    // tiled: the neighbor positions sit at LDS offset 0 (so a list entry,
+    // 16 x rank, is their byte address) and the type-pair tables after them
+    Prep* s_prep = reinterpret_cast<Prep*>(reinterpret_cast<char*>(smem) + (TILED ? a.tile_slots * sizeof(int4) : 0));
     TypePairExtra<R>* s_x = reinterpret_cast<TypePairExtra<R>*>(s_prep + nt2);
 
     Prep q0;
@@ -380,7 +387,7 @@ __global__ void __launch_bounds__(512) k_pair(const PairKernelArgs a)
         // stage the tile's neighbor positions, converted to fixed point here
         // (block lb is tile lb); every lane of the workgroup takes part before
         // any wavefront may leave.  Slot M is the padding entries' dummy.
-        s_pos = reinterpret_cast<int4*>(reinterpret_cast<char*>(smem) + (SINGLE ? 0 : table_bytes<Eval, R>(nt2)));
+        s_pos = reinterpret_cast<int4*>(smem);
         tile_js = a.tile_jset + (size_t)lb * kTileCap;
         tile_m = a.tile_count[lb];
         for (uint32_t m = threadIdx.x; m < tile_m; m += blockDim.x)
@@ -439,6 +446,7 @@ __global__ void __launch_bounds__(512) k_pair(const PairKernelArgs a)
         const int4 pi = fixed4(a.pos[pe], a.box);
         const uint32_t ti = (uint32_t)pi.w;
         const uint32_t col_off = ie * 4u;
+        const uint32_t tile_m16 = tile_m << 4; // the padding entry
         Acc32 part;
         for (uint32_t k = 0; k < P; k += U)
             {
@@ -459,17 +467,17 @@ __global__ void __launch_bounds__(512) k_pair(const PairKernelArgs a)
             int4 pj[U];
 #pragma unroll
             for (uint32_t u = 0; u < U; ++u)
-                pj[u] = s_pos[e[u]];
+                pj[u] = *reinterpret_cast<const int4*>(reinterpret_cast<const char*>(s_pos) + e[u]);
 #pragma unroll
             for (uint32_t u = 0; u < U; ++u)
                 {
                 const uint32_t tp = SINGLE ? 0u : ti * nt + (uint32_t)pj[u].w;
                 term_mixed<Eval, VIRIAL, XPLOR, CUBIC>(a, SINGLE ? q0 : s_prep[tp], SINGLE ? x0 : s_x[tp],
-                                                       e[u] != tile_m, pe, e[u], tile_js,
+                                                       e[u] != tile_m16, pe, e[u] >> 4, tile_js,
                                                        fixed_delta(pi.x, pj[u].x), fixed_delta(pi.y, pj[u].y),
                                                        fixed_delta(pi.z, pj[u].z), part);
                 }
-            if ((k & U) != 0 || k + U >= P)
+            if (((k + U) & (kFlushEntries - 1)) == 0 || k + U >= P)
                 {
                 flush32<VIRIAL>(acc, part);
                 part = Acc32();
@@ -509,7 +517,7 @@ __global__ void __launch_bounds__(512) k_pair(const PairKernelArgs a)
                                                        fixed_delta(pi.y, pj[u].y), fixed_delta(pi.z, pj[u].z),
                                                        part);
                 }
-            if ((k & U) != 0 || k + U >= P)
+            if (((k + U) & (kFlushEntries - 1)) == 0 || k + U >= P)
                 {
                 flush32<VIRIAL>(acc, part);
                 part = Acc32();
@@ -534,13 +542,15 @@ static int launch_t(const PairKernelArgs& ka, unsigned block, unsigned slots, hi
     using R = std::conditional_t<MIXED, float, double>;
     const size_t nt2 = (size_t)ka.ntypes * ka.ntypes;
     size_t shmem = SINGLE ? 0 : table_bytes<Eval, R>((uint32_t)nt2);
+    PairKernelArgs kb = ka;
+    kb.tile_slots = TILED ? slots : 0u;
     if (TILED)
         shmem += (size_t)slots * sizeof(int4);
     if (shmem > kTileLds)
         return set_error(AZP_ERR_UNSUPPORTED, "too many particle types (%u) for LDS staging", ka.ntypes);
     const unsigned grid = grid_for(ka.N, block);
     hipLaunchKernelGGL((k_pair<Eval, MIXED, VIRIAL, SINGLE, XPLOR, CUBIC, TILED>), dim3(grid), dim3(block), shmem,
-                       stream, ka);
+                       stream, kb);
     AZP_LAUNCH_CHECK();
     return AZP_OK;
     }

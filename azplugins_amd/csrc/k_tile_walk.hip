@@ -649,7 +649,7 @@ template<uint32_t T, bool SINGLE> __global__ void __launch_bounds__(T) k_tile_wa
     auto chunk_load = [&](uint32_t kc) { return kc < nn ? row[kc / 8] : make_uint4(0u, 0u, 0u, 0u); };
     auto rank_of_entry = [&](uint32_t k, uint32_t word, uint32_t half) {
         const uint32_t m = (word >> (16u * half)) & 0xffffu;
-        return k < nn ? (uint32_t)(s_mark[m] & ~kUsed) : M;
+        return (k < nn ? (uint32_t)(s_mark[m] & ~kUsed) : M) << 4; // stored as 16 x rank
     };
     auto chunk_store = [&](uint32_t kc, uint4 ch) {
         if (kc >= P)
@@ -700,8 +700,8 @@ __global__ void __launch_bounds__(64) k_tiles_to_plain(uint32_t N,
         uint32_t w = 0;
         if (k < nn)
             w = tlist[(size_t)(k >> 1) * pitch + s];
-        nlist[(size_t)k * pitch + i] = k < nn ? js[w & 0xffffu] : i;
-        nlist[(size_t)(k + 1) * pitch + i] = k + 1 < nn ? js[w >> 16] : i;
+        nlist[(size_t)k * pitch + i] = k < nn ? js[(w & 0xffffu) >> 4] : i;
+        nlist[(size_t)(k + 1) * pitch + i] = k + 1 < nn ? js[w >> 20] : i;
         }
     }
 

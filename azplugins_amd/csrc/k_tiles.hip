@@ -174,7 +174,7 @@ __global__ void __launch_bounds__(512) k_tile_build(uint32_t N,
         uint32_t e[kBatch];
 #pragma unroll
         for (uint32_t u = 0; u < kBatch; ++u)
-            e[u] = k0 + u < nn ? (uint32_t)rank[tile_find(keys, jj[u])] : M;
+            e[u] = (k0 + u < nn ? (uint32_t)rank[tile_find(keys, jj[u])] : M) << 4; // 16 x rank
 #pragma unroll
         for (uint32_t w = 0; w < kBatch / 2; ++w)
             tlist[(size_t)((k0 >> 1) + w) * pitch + i] = e[2 * w] | (e[2 * w + 1] << 16);

@@ -1,16 +1,19 @@
-// Tiled neighbor list: the full list of a block of consecutive particles
-// (a tile) rewritten against the tile's own set of neighbors.
+// Tiled neighbor list: the full list of a block of particles (a tile)
+// rewritten against the tile's own set of neighbors.
 //
-// For tile t (particles [t T, (t + 1) T), T = 256 or 512, in space-filling
-// curve order) the build collects the distinct neighbor indices of all its
-// particles, sorted ascending, into jset[t][0..count[t]) and stores every
-// list entry as a 16-bit index into that set, two entries per 32-bit word:
-// word m of particle i (at list[m * pitch + i]) holds entries 2m (low half)
-// and 2m + 1 (high half).  Columns are padded with the tile's own count
-// (count[t], one past its last rank) up to the wavefront's longest list
-// rounded up to 8.  The force kernel stages the tile's neighbor positions in
-// LDS once (slot count[t] holds a dummy) and gathers from LDS, and the list
-// stream shrinks from 4 to 2 bytes per entry.
+// A tile is a run of T = 512 or 256 columns.  Built by the per-tile walk
+// (k_tile_walk.hip) the columns are the tile's cell-list slots reordered by
+// list length, column s belonging to particle order[s]; built from a plain
+// list (k_tiles.hip) column i is particle i.  The build collects the distinct
+// neighbors of the tile's particles into jset[t][0..count[t]) and stores
+// every list entry as 16 x its rank in that set (the byte offset of the
+// neighbor's staged position in LDS), two entries per 32-bit word: word m of
+// column s (at list[m * pitch + s]) holds entries 2m (low half) and 2m + 1
+// (high half).  Columns are padded with 16 x count[t] (one past the last
+// rank) up to their wavefront's longest list rounded up to 8.  The force
+// kernel stages the tile's neighbor positions in LDS once (slot count[t]
+// holds a dummy) and gathers from LDS, and the list stream shrinks from 4 to
+// 2 bytes per entry.
 #pragma once
 
 #include <cstdint>
