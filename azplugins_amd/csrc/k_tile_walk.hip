@@ -50,6 +50,8 @@ constexpr uint32_t kRingWords = 10;
 // slots; each gets a 27-entry table of its neighbor cells' candidate ranges.
 constexpr uint32_t kOwnCells = 128;
 constexpr uint32_t kNoRange = 0xffffffffu; // a repeated stencil cell (grids narrower than 3)
+// Candidates tested per step of the walk (two steps' reads in flight).
+constexpr uint32_t kWalkBatch = 4;
 constexpr uint16_t kUsed = 0x8000u;    // mark bit beside a 15-bit rank
 
 __device__ __forceinline__ uint32_t cell_hash(uint32_t c)
@@ -212,10 +214,10 @@ __device__ uint32_t tile_walk_one(const TileWalkArgs& a,
     uint32_t count = 0;
     // the next cell's range and first candidates are read one step ahead
     // one batch: four candidates from m0 (entries past the cell's end masked)
-    auto batch = [&](const int4 (&pc)[4], uint32_t m0, uint32_t e) {
-        bool hit[4], band[4];
+    auto batch = [&](const int4 (&pc)[kWalkBatch], uint32_t m0, uint32_t e) {
+        bool hit[kWalkBatch], band[kWalkBatch];
 #pragma unroll
-        for (uint32_t q = 0; q < 4; ++q)
+        for (uint32_t q = 0; q < kWalkBatch; ++q)
             {
             const uint32_t m = m0 + q;
             const uint32_t tj = (uint32_t)pc[q].w;
@@ -244,11 +246,15 @@ __device__ uint32_t tile_walk_one(const TileWalkArgs& a,
             hit[q] = valid && below;
             band[q] = valid && !below && rsq <= hi;
             }
-        if (band[0] || band[1] || band[2] || band[3])
+        bool any_band = false;
+#pragma unroll
+        for (uint32_t q = 0; q < kWalkBatch; ++q)
+            any_band = any_band || band[q];
+        if (any_band)
             {
             // FP32 cannot decide: the FP64 predicate of the host oracle
 #pragma unroll
-            for (uint32_t q = 0; q < 4; ++q)
+            for (uint32_t q = 0; q < kWalkBatch; ++q)
                 if (band[q])
                     {
                     const uint32_t m = m0 + q;
@@ -268,7 +274,7 @@ __device__ uint32_t tile_walk_one(const TileWalkArgs& a,
         uint16_t* mark = s.mark + m0;
         uint32_t pos = count;
 #pragma unroll
-        for (uint32_t q = 0; q < 4; ++q)
+        for (uint32_t q = 0; q < kWalkBatch; ++q)
             {
             if (hit[q])
                 {
@@ -286,9 +292,9 @@ __device__ uint32_t tile_walk_one(const TileWalkArgs& a,
             }
         count = pos;
     };
-    auto load4 = [&](int4 (&pc)[4], uint32_t m0) {
+    auto load_batch = [&](int4 (&pc)[kWalkBatch], uint32_t m0) {
 #pragma unroll
-        for (uint32_t q = 0; q < 4; ++q)
+        for (uint32_t q = 0; q < kWalkBatch; ++q)
             pc[q] = s.cand[m0 + q];
     };
     // Two batches per iteration in two register sets: the reads of the next
@@ -300,18 +306,18 @@ __device__ uint32_t tile_walk_one(const TileWalkArgs& a,
         const uint32_t range = range_next;
         range_next = stencil[min(k + 1u, 26u)];
         const uint32_t b = range & 0xffffu, e = range >> 16;
-        int4 pa[4], pb[4];
-        load4(pa, b);
-        for (uint32_t m0 = b; m0 < e; m0 += 8)
+        int4 pa[kWalkBatch], pb[kWalkBatch];
+        load_batch(pa, b);
+        for (uint32_t m0 = b; m0 < e; m0 += 2 * kWalkBatch)
             {
-            load4(pb, m0 + 4);
+            load_batch(pb, m0 + kWalkBatch);
             __builtin_amdgcn_sched_barrier(0);
             batch(pa, m0, e);
-            if (m0 + 4 < e)
+            if (m0 + kWalkBatch < e)
                 {
-                load4(pa, m0 + 8);
+                load_batch(pa, m0 + 2 * kWalkBatch);
                 __builtin_amdgcn_sched_barrier(0);
-                batch(pb, m0 + 4, e);
+                batch(pb, m0 + kWalkBatch, e);
                 }
             }
         }
@@ -327,7 +333,7 @@ __device__ uint32_t tile_walk_one(const TileWalkArgs& a,
 template<uint32_t T, bool SINGLE> __global__ void __launch_bounds__(T) k_tile_walk(const TileWalkArgs a)
     {
     constexpr uint32_t C = TileWalkShape<T>::cand;
-    __shared__ int4 s_cand[C + 12];
+    __shared__ int4 s_cand[C + 3 * kWalkBatch];
     __shared__ uint16_t s_mark[C];
     __shared__ uint64_t s_tab[kCellHash];
     __shared__ uint64_t s_ucell[kUnionCells];
