@@ -27,6 +27,8 @@
 // expanded from it only when a consumer asks for it.
 // Rebuild check: a particle moved farther than r_buff/2 since the last build.
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cmath>
 #include <vector>
 
@@ -663,7 +665,7 @@ extern "C" int azp_nlist_create(azp_nlist** out)
     if (!out)
         return set_error(AZP_ERR_INVALID, "null out");
     azp_nlist* nl = new azp_nlist();
-    if (hipHostMalloc((void**)&nl->h_flags, 8 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess)
+    if (hipHostMalloc((void**)&nl->h_flags, 8 * sizeof(uint32_t), hipHostMallocCoherent) != hipSuccess)
         {
         nl->h_flags = nullptr;
         delete nl;
@@ -704,6 +706,9 @@ extern "C" int azp_nlist_set_r_cut(azp_nlist* nl, uint32_t ntypes, const double*
     nl->r_cut = rc;
     return AZP_OK;
     }
+
+// Value the rebuild flag never takes (it holds bits 1 and 2 only).
+constexpr uint32_t kFlagPending = 0xa5a5a5a5u;
 
 extern "C" int azp_nlist_update(azp_nlist* nl, const azp_particles* p, int force_rebuild, int* rebuilt, void* stream)
     {
@@ -765,8 +770,24 @@ extern "C" int azp_nlist_update(azp_nlist* nl, const azp_particles* p, int force
                            reinterpret_cast<const double4*>(p->d_pos), nl->last_pos.ptr, p->N,
                            make_box(p->box), half * half, nl->flags.ptr + 1);
         AZP_LAUNCH_CHECK();
+        // The host polls the pinned word for the copied flag (a sentinel until
+        // it lands) instead of waiting for the stream: the step's critical
+        // path then skips the wake-up of a blocking synchronization.  After
+        // 2 ms it falls back to synchronizing the stream.
+        volatile uint32_t* hflag = nl->h_flags + 1;
+        *hflag = kFlagPending;
+        std::atomic_thread_fence(std::memory_order_seq_cst);
         AZP_HIP(hipMemcpyAsync(nl->h_flags + 1, nl->flags.ptr + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-        AZP_HIP(hipStreamSynchronize(s));
+        const auto t0 = std::chrono::steady_clock::now();
+        while (*hflag == kFlagPending)
+            {
+            if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(2))
+                {
+                AZP_HIP(hipStreamSynchronize(s));
+                break;
+                }
+            }
+        std::atomic_thread_fence(std::memory_order_seq_cst);
         if (nl->h_flags[1] & 2u)
             return set_error(AZP_ERR_INVALID, "a particle position is not finite (NaN or inf)");
         need = nl->h_flags[1] != 0;

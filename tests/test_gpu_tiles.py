@@ -270,3 +270,41 @@ def test_fused_walk_relative_offsets_large_box():
     assert np.array_equal(lst_t[:, : sysm.N][mask], lst_p[:, : sysm.N][mask])
     assert np.array_equal(plain.force, tiled.force)
     assert np.array_equal(plain.virial, tiled.virial)
+
+
+@pytest.mark.parametrize("Lv", [(26.0, 22.0, 19.0), (70.0, 64.0, 58.0)])
+def test_fused_walk_non_cubic_box(Lv):
+    """Orthorhombic boxes: the per-tile walk in fixed point with per-axis
+    scales (small box) and in FP32 offsets (large box); lists equal the plain
+    walk's, the pair set equals the oracle's (small box), and the mixed forces
+    over the tiled list equal the plain-list ones and the FP64 oracle to 1e-5."""
+    Lv = np.array(Lv)
+    rng = np.random.default_rng(int(Lv[0]))
+    n = [int(round(l / 1.056)) for l in Lv]
+    g = [(np.arange(k) + 0.5) * (l / k) - l / 2 for k, l in zip(n, Lv)]
+    x, y, z = np.meshgrid(*g, indexing="ij")
+    p = np.stack([x.ravel(), y.ravel(), z.ravel()], 1) + rng.uniform(-0.1, 0.1, (x.size, 3))
+    p -= Lv * np.floor(p / Lv + 0.5)
+    pos4 = np.zeros((p.shape[0], 4))
+    pos4[:, :3] = p
+    sysm = GpuSystem(pos4, Lv)
+    plain, tiled = _both("PotentialPairPerturbedLennardJones", sysm, 1, PLJ, 3.0)
+    ntiles, rel = tiled.nl.tile_stats()
+    say(f"[fused] box {Lv.tolist()} N={sysm.N}: walk kind {tiled.nl.walk_kind()}, {rel} of {ntiles} tiles in FP32 offsets")
+    assert tiled.nl.walk_kind() == 1
+    nn_t, lst_t = tiled.nl.download(sysm.N)
+    nn_p, lst_p = plain.nl.download(sysm.N)
+    assert np.array_equal(nn_t, nn_p)
+    mask = np.arange(lst_t.shape[0])[:, None] < nn_t[None, :]
+    assert np.array_equal(lst_t[:, : sysm.N][mask], lst_p[:, : sysm.N][mask])
+    assert np.array_equal(plain.force, tiled.force)
+    assert np.array_equal(plain.virial, tiled.virial)
+    if sysm.N < 20000:
+        pairs = oracle.nlist(pos4, Lv, 1, oracle.r_listsq_matrix([3.0], 0.4), "cell")
+        assert nlist_pairs(tiled.nl, sysm.N) == set(zip(pairs[0].tolist(), pairs[1].tolist()))
+        fo, vo = oracle.pair_forces(oracle.PERTURBED_LJ, pos4, Lv, pairs, 1,
+                                    oracle.pack_matrix(oracle.PERTURBED_LJ, PLJ, 1), np.array([9.0]))
+        assert rel_err(tiled.force[:, :3], fo[:, :3]) < 1e-5
+        assert rel_err(tiled.virial, vo) < 1e-5
+    else:
+        assert rel > 0
