@@ -103,16 +103,24 @@ class Simulation:
         self._attached = True
 
     def run(self, steps):
-        """Advance `steps` timesteps; forces are (re)computed at the start."""
+        """Advance `steps` timesteps; forces are (re)computed at the start.
+
+        Nothing but the steps runs inside run(), so parameters, the box and
+        the particle arrays are fixed for its duration: the ctypes views of
+        the state and the neighbor-list settings are built once per call."""
         if not self._attached:
             self._attach()
         integ = self.operations.integrator
-        if integ is not None:
-            integ._prepare_run(self.timestep)
-        for _ in range(int(steps)):
+        self.state._freeze_view()
+        try:
             if integ is not None:
-                integ._update(self.timestep)
-            self.timestep += 1
+                integ._prepare_run(self.timestep)
+            for _ in range(int(steps)):
+                if integ is not None:
+                    integ._update(self.timestep)
+                self.timestep += 1
+        finally:
+            self.state._thaw_view()
 
 
 from . import md  # noqa: E402

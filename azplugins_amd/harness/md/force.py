@@ -23,11 +23,16 @@ class Force:
         self._torque = torch.zeros((max(N, 1), 4), dtype=torch.float64, device=dev)
 
     def _outputs(self):
+        """azp_outputs view of the force arrays, rebuilt when they are reallocated."""
+        o = getattr(self, "_out_c", None)
+        if o is not None and o.d_force == self._force.data_ptr():
+            return o
         o = _azp.Outputs()
         o.d_force = self._force.data_ptr()
         o.d_virial = self._virial.data_ptr()
         o.virial_pitch = self._pitch
         o.d_torque = self._torque.data_ptr()
+        self._out_c = o
         return o
 
     def _require(self):

@@ -54,13 +54,17 @@ class NeighborList:
     def _update(self, timestep):
         if self._last == timestep:
             return
-        nt, m = self._r_cut_matrix()
-        self._cpp.set_r_buff(self.buffer)
-        self._cpp.set_r_cut(nt, m)
-        tiled = self._wants_tiles()
-        if tiled != getattr(self, "_tiled", False):
-            self._cpp.set_tiled(tiled)
-            self._tiled = tiled
+        # the settings are pushed once per run() (nothing changes them inside)
+        run_id = getattr(self._sim.state, "_view", None)
+        if run_id is None or run_id is not getattr(self, "_pushed_for", None):
+            nt, m = self._r_cut_matrix()
+            self._cpp.set_r_buff(self.buffer)
+            self._cpp.set_r_cut(nt, m)
+            tiled = self._wants_tiles()
+            if tiled != getattr(self, "_tiled", False):
+                self._cpp.set_tiled(tiled)
+                self._tiled = tiled
+            self._pushed_for = run_id
         self._cpp.update(self._sim.state.particles())
         self._last = timestep
 

@@ -152,8 +152,18 @@ class State:
     def particle_types(self):
         return self.types
 
+    def _freeze_view(self):
+        """Inside Simulation.run: reuse one azp_particles view."""
+        self._view = None
+        self._view = self.particles()
+
+    def _thaw_view(self):
+        self._view = None
+
     def particles(self):
         """azp_particles view of the device arrays (valid while tensors live)."""
+        if getattr(self, "_view", None) is not None:
+            return self._view
         s = _azp.Particles()
         s.N = self.N
         s.box = self.box.to_c()
