@@ -287,6 +287,7 @@ def main():
     }
     if not args.no_tps:
         out["tps"] = run_tps(c, args.tps_steps)
+        out["tps_fp64"] = run_tps(c, args.tps_steps, "fp64")
     if not args.no_extras:
         extras = {}
         for name, fn in (("dpd_2m", bench_dpd), ("yukawa_262k_adjoint", bench_adjoint),
@@ -498,7 +499,7 @@ def bench_hertz(args):
             "method": "NVE velocity Verlet dt 0.005"}
 
 
-def run_tps(c, steps):
+def run_tps(c, steps, precision="mixed"):
     """Timesteps/s of full Langevin MD steps through the drop-in API."""
     import azplugins_amd.harness as hoomd
     from azplugins_amd import _azp
@@ -516,7 +517,7 @@ def run_tps(c, steps):
     nl = hoomd.md.nlist.Cell(buffer=0.4)
     plj = hoomd.azplugins.pair.PerturbedLennardJones(nlist=nl, default_r_cut=3.0)
     plj.params[("A", "A")] = dict(epsilon=1.0, sigma=1.0, attraction_scale_factor=0.5)
-    plj.precision = "mixed"
+    plj.precision = precision
     integ.forces = [plj]
     integ.methods = [hoomd.md.methods.Langevin(hoomd.filter.All(), kT=1.2, default_gamma=1.0)]
     sim.operations.integrator = integ
@@ -528,8 +529,8 @@ def run_tps(c, steps):
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     res = {"value": steps / dt, "unit": "timesteps/s", "steps": steps,
-           "nlist_builds": nl.num_builds - b0, "precision": "mixed", "method": "Langevin kT=1.2 gamma=1 dt=0.005"}
-    log(f"bench: TPS {steps / dt:.1f} ({nl.num_builds - b0} rebuilds in {steps} steps)")
+           "nlist_builds": nl.num_builds - b0, "precision": precision, "method": "Langevin kT=1.2 gamma=1 dt=0.005"}
+    log(f"bench: TPS {precision} {steps / dt:.1f} ({nl.num_builds - b0} rebuilds in {steps} steps)")
     return res
 
 

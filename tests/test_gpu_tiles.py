@@ -308,3 +308,27 @@ def test_fused_walk_non_cubic_box(Lv):
         assert rel_err(tiled.virial, vo) < 1e-5
     else:
         assert rel > 0
+
+
+def test_fused_walk_fallbacks():
+    """A dilute system (density 0.02: a tile spans more than 128 cells) and a
+    nine-type system (above the walk's type table) take the plain walk; the
+    tiled list, when built, and the forces still equal the plain ones."""
+    rng = np.random.default_rng(11)
+    L = 60.0
+    pos4 = np.zeros((4320, 4))
+    pos4[:, :3] = rng.uniform(-L / 2, L / 2, (pos4.shape[0], 3))
+    sysm = GpuSystem(pos4, np.array([L, L, L]))
+    plain, tiled = _both("PotentialPairPerturbedLennardJones", sysm, 1, PLJ, 3.0)
+    say(f"[fused] dilute: walk kind {tiled.nl.walk_kind()}, tile size {tiled.nl.get_tiles()[3]}")
+    assert tiled.nl.walk_kind() == 0
+    assert np.array_equal(plain.force, tiled.force)
+
+    L = 16.0
+    pos4, _ = jittered_lattice(16, L, 0.1, 9, seed=19)
+    sysm = GpuSystem(pos4, np.array([L, L, L]))
+    hz = {(a, b): {"epsilon": 1.0 + 0.1 * (a + b)} for a in range(9) for b in range(a, 9)}
+    plain, tiled = _both("PotentialPairHertz", sysm, 9, hz, 1.3)
+    say(f"[fused] nine types: walk kind {tiled.nl.walk_kind()}, tile size {tiled.nl.get_tiles()[3]}")
+    assert tiled.nl.walk_kind() == 0
+    assert np.array_equal(plain.force, tiled.force)
