@@ -12,7 +12,8 @@ beforehand (it is rebuilt every ~10-20 MD steps, not every evaluation).  The
 line also carries the plain FP64 form, the timesteps/s of full Langevin MD
 steps (kT = 1.2, gamma = 1, dt = 0.005, rebuild checks and rebuilds included),
 the mixed form's force error against the FP64 form, the roofline of the pair
-kernel and the CPU baseline (the FP64 oracle, one thread).
+kernel and the CPU baseline (the FP64 oracle on one thread, and threaded over
+particles with a full list on every thread the job may use).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-tps]
 """
@@ -561,8 +562,23 @@ def cpu_baseline(cells):
         "kind": "port",
         "sample": f"PerturbedLJ FCC {cells}^3 (N={N}) at density 0.85, r_cut 3.0, 3 force "
                   f"evaluations (median {t:.2f} s), FP64 half list with third law; host "
-                  f"{os.cpu_count()} logical CPUs",
+                  f"{os.cpu_count()} logical CPUs ({host_cpu_model()})",
     }
+    # the same forces over a full list, OpenMP-threaded over particles, on the
+    # threads this job may use (OMP_NUM_THREADS, else every logical CPU)
+    nthreads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+    csr = oracle.full_list(N, pairs)
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        oracle.pair_forces_threaded(oracle.PERTURBED_LJ, pos4, L, csr, 1, packed, np.array([9.0]),
+                                    nthreads=nthreads)
+        times.append(time.perf_counter() - t0)
+    tt = float(np.median(times))
+    res["threaded"] = {"value": n_cut / tt, "unit": "pair interactions/s", "cores": nthreads,
+                       "sample": f"the same configuration, FP64 full list threaded over particles "
+                                 f"(OpenMP, {nthreads} threads), 3 evaluations (median {tt:.3f} s)"}
+    log(f"bench: CPU oracle {nthreads} threads {n_cut / tt:.3e} pair interactions/s ({tt:.3f} s per evaluation)")
     # Hertz N = 4096, 1000 NVE steps (the CPU-only workload), same oracle, one thread
     h = configs.hertz_4096()
     rng = np.random.default_rng(1)
@@ -577,6 +593,18 @@ def cpu_baseline(cells):
     res["hertz_4096_nve"] = {"value": 1000 / th, "unit": "timesteps/s", "steps": 1000, "cores": 1}
     log(f"bench: CPU oracle Hertz 4096 NVE {1000 / th:.1f} steps/s")
     return res
+
+
+def host_cpu_model():
+    """The host CPU's model name (Linux /proc/cpuinfo), for the baseline's record."""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
 
 
 def load_valu():

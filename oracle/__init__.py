@@ -89,6 +89,9 @@ def lib():
             getattr(L, fn).argtypes = [u32, pd, pd, u32, pd, i64, pu, pu]
             getattr(L, fn).restype = i64
         L.orc_pair_forces.argtypes = [ci, u32, pd, pd, i64, pu, pu, u32, pd, pd, pd, ci, pd, pd]
+        L.orc_full_list.argtypes = [u32, i64, pu, pu, ctypes.POINTER(ctypes.c_int64), pu]
+        L.orc_pair_forces_threaded.argtypes = [ci, u32, pd, pd, ctypes.POINTER(ctypes.c_int64), pu, u32, pd, pd, pd,
+                                               ci, pd, pd, ci]
         L.orc_dpd_forces.argtypes = [
             u32, pd, pd, pu, pd, i64, pu, pu, u32, pd, pd,
             ctypes.c_uint16, u32, d, d, ci, pd, pd,
@@ -247,6 +250,37 @@ def pair_forces(evaluator, pos4, L, pairs, ntypes, packed, rcutsq, ronsq=None, m
         evaluator, N, _pd(pos4), _pd(L), pi.shape[0], _pu(pi), _pu(pj), ntypes,
         _pd(packed), _pd(rc), _pd(ron), MODE_CODES.get(mode, mode), _pd(f),
         _pd(v) if v is not None else None,
+    )
+    return f, v
+
+
+def full_list(N, pairs):
+    """CSR full (directed) list from a half list: (offsets[N + 1], neighbors)."""
+    pi = np.ascontiguousarray(pairs[0], dtype=np.uint32)
+    pj = np.ascontiguousarray(pairs[1], dtype=np.uint32)
+    offsets = np.zeros(N + 1, dtype=np.int64)
+    nbrs = np.zeros(2 * pi.shape[0], dtype=np.uint32)
+    lib().orc_full_list(N, pi.shape[0], _pu(pi), _pu(pj), offsets.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                        _pu(nbrs))
+    return offsets, nbrs
+
+
+def pair_forces_threaded(evaluator, pos4, L, csr, ntypes, packed, rcutsq, ronsq=None, mode="none",
+                         nthreads=1, with_virial=True):
+    """orc_pair_forces over a full CSR list, OpenMP-threaded over particles."""
+    pos4 = _f64(pos4)
+    L = _f64(L)
+    offsets, nbrs = csr
+    N = pos4.shape[0]
+    f = np.zeros((N, 4))
+    v = np.zeros((6, N)) if with_virial else None
+    packed = _f64(packed)
+    rc = _f64(rcutsq)
+    ron = _f64(ronsq if ronsq is not None else np.zeros_like(rc))
+    lib().orc_pair_forces_threaded(
+        evaluator, N, _pd(pos4), _pd(L), offsets.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), _pu(nbrs), ntypes,
+        _pd(packed), _pd(rc), _pd(ron), MODE_CODES.get(mode, mode), _pd(f),
+        _pd(v) if v is not None else None, int(nthreads),
     )
     return f, v
 

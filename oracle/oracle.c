@@ -820,6 +820,51 @@ static inline void add_virial_iso(double* virial, uint32_t N, uint32_t i, double
  * Parity: shift/none pinned (test_pair.py); xplor parity unpinned (no reference
  * test), checked by F = -dU/dr.  Virial pinned by W = sum dx (x) F / 2 and the
  * affine-strain derivative of U. */
+/* One pair's force_divr and energy (with shift / xplor) and its
+ * displacement d = r_i - r_j (minimum image); 0 when it does not interact. */
+static int iso_pair(int evaluator,
+                    const double* pos4,
+                    const double* L,
+                    uint32_t i,
+                    uint32_t j,
+                    uint32_t ntypes,
+                    const double* packed,
+                    const double* rcutsq_m,
+                    const double* ronsq_m,
+                    int shift_mode,
+                    double* d,
+                    double* fdivr_out,
+                    double* e_out)
+{
+    const int stride = 8; /* packed params: 8 doubles per type pair */
+    const double rsq = pair_rsq(pos4, L, i, j, d);
+    const unsigned tp = type_of(pos4, i) * ntypes + type_of(pos4, j);
+    const double rcutsq = rcutsq_m[tp];
+    const double ronsq = ronsq_m ? ronsq_m[tp] : 0.0;
+    int energy_shift = 0;
+    if (shift_mode == ORC_SHIFT_SHIFT)
+        energy_shift = 1;
+    else if (shift_mode == ORC_SHIFT_XPLOR && ronsq > rcutsq)
+        energy_shift = 1;
+    double fdivr = 0.0, e = 0.0;
+    if (!orc_eval_iso(evaluator, rsq, rcutsq, packed + stride * tp, energy_shift, &fdivr, &e))
+        return 0;
+    if (shift_mode == ORC_SHIFT_XPLOR && rsq >= ronsq && rsq < rcutsq)
+        {
+        const double old_e = e, old_f = fdivr;
+        const double denom = (rcutsq - ronsq) * (rcutsq - ronsq) * (rcutsq - ronsq);
+        const double denom_inv = 1.0 / denom;
+        const double rsq_m_rcsq = rsq - rcutsq;
+        const double S = rsq_m_rcsq * rsq_m_rcsq * (rcutsq + 2.0 * rsq - 3.0 * ronsq) * denom_inv;
+        const double dS_dr_divr = 12.0 * (rsq - ronsq) * rsq_m_rcsq * denom_inv;
+        e = old_e * S;
+        fdivr = S * old_f - dS_dr_divr * old_e;
+        }
+    *fdivr_out = fdivr;
+    *e_out = e;
+    return 1;
+}
+
 void orc_pair_forces(int evaluator,
                      uint32_t N,
                      const double* pos4,
@@ -836,34 +881,13 @@ void orc_pair_forces(int evaluator,
                      double* virial)
 {
     zero_outputs(N, force4, virial, NULL);
-    const int stride = 8; /* packed params: 8 doubles per type pair */
     double d[3];
     for (int64_t k = 0; k < npairs; ++k)
         {
         const uint32_t i = pair_i[k], j = pair_j[k];
-        const double rsq = pair_rsq(pos4, L, i, j, d);
-        const unsigned tp = type_of(pos4, i) * ntypes + type_of(pos4, j);
-        const double rcutsq = rcutsq_m[tp];
-        const double ronsq = ronsq_m ? ronsq_m[tp] : 0.0;
-        int energy_shift = 0;
-        if (shift_mode == ORC_SHIFT_SHIFT)
-            energy_shift = 1;
-        else if (shift_mode == ORC_SHIFT_XPLOR && ronsq > rcutsq)
-            energy_shift = 1;
-        double fdivr = 0.0, e = 0.0;
-        if (!orc_eval_iso(evaluator, rsq, rcutsq, packed + stride * tp, energy_shift, &fdivr, &e))
+        double fdivr, e;
+        if (!iso_pair(evaluator, pos4, L, i, j, ntypes, packed, rcutsq_m, ronsq_m, shift_mode, d, &fdivr, &e))
             continue;
-        if (shift_mode == ORC_SHIFT_XPLOR && rsq >= ronsq && rsq < rcutsq)
-            {
-            const double old_e = e, old_f = fdivr;
-            const double denom = (rcutsq - ronsq) * (rcutsq - ronsq) * (rcutsq - ronsq);
-            const double denom_inv = 1.0 / denom;
-            const double rsq_m_rcsq = rsq - rcutsq;
-            const double S = rsq_m_rcsq * rsq_m_rcsq * (rcutsq + 2.0 * rsq - 3.0 * ronsq) * denom_inv;
-            const double dS_dr_divr = 12.0 * (rsq - ronsq) * rsq_m_rcsq * denom_inv;
-            e = old_e * S;
-            fdivr = S * old_f - dS_dr_divr * old_e;
-            }
         const double fd2 = fdivr * 0.5;
         for (int a = 0; a < 3; ++a)
             {
@@ -877,6 +901,84 @@ void orc_pair_forces(int evaluator,
             add_virial_iso(virial, N, i, fd2, d);
             add_virial_iso(virial, N, j, fd2, d);
             }
+        }
+}
+
+/* Full (directed) CSR list from a half list by counting sort: the neighbors
+ * of i are nbrs[offsets[i] .. offsets[i + 1]), in the order the half list
+ * meets them.  offsets has N + 1 entries, nbrs 2 npairs. */
+void orc_full_list(uint32_t N, int64_t npairs, const uint32_t* pair_i, const uint32_t* pair_j, int64_t* offsets,
+                   uint32_t* nbrs)
+{
+    memset(offsets, 0, sizeof(int64_t) * ((size_t)N + 1));
+    for (int64_t k = 0; k < npairs; ++k)
+        {
+        offsets[pair_i[k] + 1]++;
+        offsets[pair_j[k] + 1]++;
+        }
+    for (uint32_t i = 0; i < N; ++i)
+        offsets[i + 1] += offsets[i];
+    int64_t* fill = (int64_t*)malloc(sizeof(int64_t) * ((size_t)N + 1));
+    memcpy(fill, offsets, sizeof(int64_t) * ((size_t)N + 1));
+    for (int64_t k = 0; k < npairs; ++k)
+        {
+        nbrs[fill[pair_i[k]]++] = pair_j[k];
+        nbrs[fill[pair_j[k]]++] = pair_i[k];
+        }
+    free(fill);
+}
+
+/* The same forces over a full (directed) list in CSR form, threaded over
+ * particles with OpenMP: particle i sums its own F_i, E_i/2 and W_i over its
+ * neighbors (no third law, no shared writes).  This is the CPU path threaded
+ * over particle ranges that the CPU baseline reports beside the one-thread
+ * half list; per-particle results equal orc_pair_forces up to the order of
+ * the FP64 sums. */
+void orc_pair_forces_threaded(int evaluator,
+                              uint32_t N,
+                              const double* pos4,
+                              const double* L,
+                              const int64_t* offsets,
+                              const uint32_t* nbrs,
+                              uint32_t ntypes,
+                              const double* packed,
+                              const double* rcutsq_m,
+                              const double* ronsq_m,
+                              int shift_mode,
+                              double* force4,
+                              double* virial,
+                              int nthreads)
+{
+    zero_outputs(N, force4, virial, NULL);
+#pragma omp parallel for schedule(dynamic, 256) num_threads(nthreads > 0 ? nthreads : 1)
+    for (int64_t ii = 0; ii < (int64_t)N; ++ii)
+        {
+        const uint32_t i = (uint32_t)ii;
+        double f[3] = {0.0, 0.0, 0.0}, en = 0.0, w[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        double d[3];
+        for (int64_t k = offsets[i]; k < offsets[i + 1]; ++k)
+            {
+            double fdivr, e;
+            if (!iso_pair(evaluator, pos4, L, i, nbrs[k], ntypes, packed, rcutsq_m, ronsq_m, shift_mode, d, &fdivr,
+                          &e))
+                continue;
+            const double fd2 = fdivr * 0.5;
+            for (int a = 0; a < 3; ++a)
+                f[a] += d[a] * fdivr;
+            en += e * 0.5;
+            w[0] += fd2 * d[0] * d[0];
+            w[1] += fd2 * d[0] * d[1];
+            w[2] += fd2 * d[0] * d[2];
+            w[3] += fd2 * d[1] * d[1];
+            w[4] += fd2 * d[1] * d[2];
+            w[5] += fd2 * d[2] * d[2];
+            }
+        for (int a = 0; a < 3; ++a)
+            force4[4 * (size_t)i + a] = f[a];
+        force4[4 * (size_t)i + 3] = en;
+        if (virial)
+            for (int c = 0; c < 6; ++c)
+                virial[c * (size_t)N + i] = w[c];
         }
 }
 

@@ -137,6 +137,27 @@ void orc_pair_forces(int evaluator,
                      double* force4,
                      double* virial);
 
+/* Full (directed) CSR list from a half list (counting sort). */
+void orc_full_list(uint32_t N, int64_t npairs, const uint32_t* pair_i, const uint32_t* pair_j, int64_t* offsets,
+                   uint32_t* nbrs);
+
+/* Full-list, OpenMP-threaded form of orc_pair_forces (CSR list: neighbors of
+ * i are nbrs[offsets[i] .. offsets[i + 1])). */
+void orc_pair_forces_threaded(int evaluator,
+                              uint32_t N,
+                              const double* pos4,
+                              const double* L,
+                              const int64_t* offsets,
+                              const uint32_t* nbrs,
+                              uint32_t ntypes,
+                              const double* packed,
+                              const double* rcutsq_m,
+                              const double* ronsq_m,
+                              int shift_mode,
+                              double* force4,
+                              double* virial,
+                              int nthreads);
+
 void orc_dpd_forces(uint32_t N,
                     const double* pos4,
                     const double* vel4,

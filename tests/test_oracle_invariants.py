@@ -179,6 +179,27 @@ def test_many_body_third_law_and_virial(orc, ev, mode):
             assert f[i, a] == pytest.approx(-(up - um) / (2 * hh), rel=1e-5, abs=1e-6)
 
 
+@pytest.mark.parametrize("mode", ["none", "shift", "xplor"])
+@pytest.mark.parametrize("ev", [0, 1, 2])
+def test_threaded_full_list_equals_half_list(orc, ev, mode):
+    """The CPU baseline's threaded full-list loop (orc_pair_forces_threaded over
+    orc_full_list) gives the half-list third-law forces, energies and virials
+    up to the order of the FP64 sums."""
+    pp = {0: {(0, 0): {"epsilon": 1.0}, (0, 1): {"epsilon": 2.0}, (1, 1): {"epsilon": 0.5}},
+          1: {(0, 0): {"epsilon": 1.0, "kappa": 2.0, "delta": 0.2},
+              (0, 1): {"epsilon": 0.7, "kappa": 2.0, "delta": 0.35},
+              (1, 1): {"epsilon": 1.4, "kappa": 2.0, "delta": 0.1}},
+          2: {(0, 0): {"epsilon": 1.0, "sigma": 1.0, "attraction_scale_factor": 0.5},
+              (0, 1): {"epsilon": 0.8, "sigma": 0.9, "attraction_scale_factor": 0.3},
+              (1, 1): {"epsilon": 1.2, "sigma": 0.85, "attraction_scale_factor": 0.7}}}[ev]
+    pos, Lv, packed, pairs, rcutsq, ronsq, f, v = _many_body(orc, ev, pp, 2.5, mode=mode)
+    csr = orc.full_list(pos.shape[0], pairs)
+    assert csr[0][-1] == 2 * pairs[0].shape[0]
+    ft, vt = orc.pair_forces_threaded(ev, pos, Lv, csr, 2, packed, rcutsq, ronsq, mode, nthreads=4)
+    assert np.abs(ft - f).max() <= 1e-12 * np.abs(f).max()
+    assert np.abs(vt - v).max() <= 1e-12 * np.abs(v).max()
+
+
 @pytest.mark.parametrize("L,rc,ntypes", [(9.0, 1.2, 2), (4.1, 1.5, 1), (6.0, 2.5, 2)])
 def test_nlist_cell_equals_brute(orc, L, rc, ntypes):
     pos = random_fluid(600, L, ntypes, seed=int(L * 10))
