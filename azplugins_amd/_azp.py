@@ -186,6 +186,10 @@ SIGNATURES = [
     ("azp_nlist_get_tile_order", _i, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
     ("azp_nlist_walk_kind", _i, [_vp, ctypes.POINTER(_i)]),
     ("azp_nlist_tile_stats", _i, [_vp, ctypes.POINTER(_u32), ctypes.POINTER(_u32)]),
+    ("azp_sorter_create", _i, [ctypes.POINTER(_vp)]),
+    ("azp_sorter_destroy", _i, [_vp]),
+    ("azp_sorter_order", _i, [_vp, ctypes.POINTER(Particles), _u32, _vp, _vp]),
+    ("azp_gather_rows", _i, [_vp, _vp, _vp, _u32, _u32, _vp]),
     ("azp_pair_create", _i, [ctypes.POINTER(_vp), ctypes.c_char_p, _u32]),
     ("azp_pair_destroy", _i, [_vp]),
     ("azp_pair_set_params", _i, [_vp, _u32, _u32, _pd]),

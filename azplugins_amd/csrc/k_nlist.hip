@@ -982,3 +982,141 @@ extern "C" int azp_nlist_download(const azp_nlist* nl,
     AZP_HIP(hipMemcpy(h_nlist, nl->list.ptr, (size_t)need * sizeof(uint32_t), hipMemcpyDeviceToHost));
     return AZP_OK;
     }
+
+// ---- particle sort ------------------------------------------------------
+// Particles reordered along the Morton curve of a grid of bins so that
+// particles close in space are close in memory (the role of HOOMD's
+// SFCPackTuner, which sorts along a Hilbert curve).  The bins reuse the cell
+// list's kernels: bin, scan, scatter, then each bin's members by index.
+
+struct azp_sorter
+    {
+    DeviceArray<uint32_t> cell_of, slot_of, count, start, scan_sums, cell_rank;
+    int rank_grid[3] = {0, 0, 0};
+    };
+
+namespace azp
+{
+
+template<typename T>
+__global__ void k_gather_rows(const T* __restrict__ src, T* __restrict__ dst, const uint32_t* __restrict__ order,
+                              uint32_t N)
+    {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= N)
+        return;
+    const uint32_t j = order[k];
+    if (j < N)
+        dst[k] = src[j];
+    }
+
+struct Row32
+    {
+    uint4 a, b;
+    };
+
+} // namespace azp
+
+extern "C" int azp_sorter_create(azp_sorter** out)
+    {
+    if (!out)
+        return set_error(AZP_ERR_INVALID, "null out");
+    *out = new azp_sorter();
+    return AZP_OK;
+    }
+
+extern "C" int azp_sorter_destroy(azp_sorter* so)
+    {
+    delete so;
+    return AZP_OK;
+    }
+
+extern "C" int azp_sorter_order(azp_sorter* so, const azp_particles* p, uint32_t grid, uint32_t* d_order, void* stream)
+    {
+    if (!so || !p)
+        return set_error(AZP_ERR_INVALID, "null argument");
+    if (!box_valid(p->box))
+        return set_error(AZP_ERR_BOX, "box lengths must be positive");
+    if (grid == 0 || grid > 1024)
+        return set_error(AZP_ERR_INVALID, "grid must be in [1, 1024], got %u", grid);
+    const uint32_t N = p->N;
+    if (N == 0)
+        return AZP_OK;
+    if (!p->d_pos || !d_order)
+        return set_error(AZP_ERR_INVALID, "null positions or order");
+    hipStream_t s = (hipStream_t)stream;
+    const double L[3] = {p->box.Lx, p->box.Ly, p->box.Lz};
+    const double w = std::max(L[0], std::max(L[1], L[2])) / grid;
+    // bins of width >= w, about 4 per particle at most
+    const CellGrid g = cell_grid_for(L, w, N);
+    const size_t nbin = (size_t)g.n[0] * g.n[1] * g.n[2];
+    AZP_TRY(so->cell_of.reserve(N));
+    AZP_TRY(so->slot_of.reserve(N));
+    AZP_TRY(so->count.reserve(nbin));
+    AZP_TRY(so->start.reserve(nbin + 1));
+    AZP_TRY(so->scan_sums.reserve(scan_chunks((uint32_t)nbin)));
+    if (g.n[0] != so->rank_grid[0] || g.n[1] != so->rank_grid[1] || g.n[2] != so->rank_grid[2])
+        {
+        std::vector<uint32_t> rank;
+        morton_cell_rank(g, rank);
+        AZP_TRY(so->cell_rank.reserve(nbin));
+        AZP_HIP(hipMemcpyAsync(so->cell_rank.ptr, rank.data(), nbin * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+        AZP_HIP(hipStreamSynchronize(s)); // the host vector goes out of scope
+        for (int k = 0; k < 3; ++k)
+            so->rank_grid[k] = g.n[k];
+        }
+    const unsigned B = 256;
+    const double4* pos = reinterpret_cast<const double4*>(p->d_pos);
+    const DevBox box = make_box(p->box);
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(nbin, B)), dim3(B), 0, s, so->count.ptr, 0u, nbin);
+    hipLaunchKernelGGL(k_bin, dim3(grid_for(N, B)), dim3(B), 0, s, pos, N, box, g, so->cell_rank.ptr,
+                       so->cell_of.ptr, so->slot_of.ptr, so->count.ptr);
+    enqueue_scan(so->count.ptr, (uint32_t)nbin, so->scan_sums.ptr, so->start.ptr, nullptr, s);
+    hipLaunchKernelGGL(k_scatter, dim3(grid_for(N, B)), dim3(B), 0, s, N, so->cell_of.ptr, so->cell_rank.ptr,
+                       so->slot_of.ptr, so->start.ptr, d_order);
+    hipLaunchKernelGGL(k_sort_cells, dim3(grid_for(nbin, 4)), dim3(256), 0, s, (uint32_t)nbin, so->start.ptr,
+                       d_order);
+    AZP_LAUNCH_CHECK();
+    return AZP_OK;
+    }
+
+extern "C" int azp_gather_rows(const void* d_src,
+                               void* d_dst,
+                               const uint32_t* d_order,
+                               uint32_t N,
+                               uint32_t row_bytes,
+                               void* stream)
+    {
+    if (N == 0)
+        return AZP_OK;
+    if (!d_src || !d_dst || !d_order)
+        return set_error(AZP_ERR_INVALID, "null argument");
+    if (d_src == d_dst)
+        return set_error(AZP_ERR_INVALID, "source and destination must differ");
+    hipStream_t s = (hipStream_t)stream;
+    const unsigned B = 256;
+    const dim3 grid(grid_for(N, B));
+    switch (row_bytes)
+        {
+        case 4:
+            hipLaunchKernelGGL(k_gather_rows<uint32_t>, grid, dim3(B), 0, s, (const uint32_t*)d_src,
+                               (uint32_t*)d_dst, d_order, N);
+            break;
+        case 8:
+            hipLaunchKernelGGL(k_gather_rows<uint2>, grid, dim3(B), 0, s, (const uint2*)d_src, (uint2*)d_dst,
+                               d_order, N);
+            break;
+        case 16:
+            hipLaunchKernelGGL(k_gather_rows<uint4>, grid, dim3(B), 0, s, (const uint4*)d_src, (uint4*)d_dst,
+                               d_order, N);
+            break;
+        case 32:
+            hipLaunchKernelGGL(k_gather_rows<Row32>, grid, dim3(B), 0, s, (const Row32*)d_src, (Row32*)d_dst,
+                               d_order, N);
+            break;
+        default:
+            return set_error(AZP_ERR_INVALID, "row_bytes must be 4, 8, 16 or 32, got %u", row_bytes);
+        }
+    AZP_LAUNCH_CHECK();
+    return AZP_OK;
+    }

@@ -5,7 +5,8 @@ provides the subset the pair-force path and its tests use — ``Snapshot``,
 ``Simulation`` with ``create_state_from_snapshot``/``run``/``timestep``/
 ``seed``, ``md.nlist.Cell``, ``md.Integrator``, ``md.methods.ConstantVolume``
 and ``md.methods.Langevin``, ``md.compute.ThermodynamicQuantities``,
-``filter.All`` and ``variant.Constant``.  Particle data lives in HBM as torch
+``tune.ParticleSorter``, ``trigger.Periodic``, ``filter.All`` and
+``variant.Constant``.  Particle data lives in HBM as torch
 tensors (device memory only; all arithmetic runs in libazp.so kernels).
 
 Conventions that differ from HOOMD-blue are documented where they occur:
@@ -18,7 +19,7 @@ import warnings
 
 import numpy as np
 
-from . import filter, variant  # noqa: F401
+from . import filter, trigger, tune, variant  # noqa: F401
 from .state import Box, Snapshot, State  # noqa: F401
 
 
@@ -39,10 +40,15 @@ class GPU(Device):
 
 
 class Operations:
+    """The simulation's operations.  A new Operations holds one
+    ``tune.ParticleSorter`` with its default trigger, so simulations keep
+    their particles sorted unless it is removed from ``tuners``."""
+
     def __init__(self, sim):
         self._sim = sim
         self._integrator = None
         self.computes = []
+        self.tuners = [tune.ParticleSorter()]
 
     @property
     def integrator(self):
@@ -102,6 +108,14 @@ class Simulation:
             c._attach(self)
         self._attached = True
 
+    def _reorder(self, order):
+        """A tuner moved particle order[k] to index k: every per-particle
+        array follows, force outputs included, and neighbor lists rebuild."""
+        scratch = self.state._reorder(order)
+        integ = self.operations.integrator
+        if integ is not None and integ._sim is self:
+            integ._reorder(order, scratch)
+
     def run(self, steps):
         """Advance `steps` timesteps; forces are (re)computed at the start.
 
@@ -115,7 +129,13 @@ class Simulation:
         try:
             if integ is not None:
                 integ._prepare_run(self.timestep)
+            tuners = list(self.operations.tuners)
+            for t in tuners:
+                if t._sim is not self:
+                    t._attach(self)
             for _ in range(int(steps)):
+                for t in tuners:
+                    t._act(self.timestep)
                 if integ is not None:
                     integ._update(self.timestep)
                 self.timestep += 1

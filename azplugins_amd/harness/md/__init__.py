@@ -53,6 +53,18 @@ class Integrator:
             self._sum_virial.add_(f._virial)
             self._sum_torque.add_(f._torque)
 
+    def _reorder(self, order, scratch):
+        from ..state import permute_rows
+
+        N = self._sim.state.N
+        for f in self.forces:
+            f._reorder(order, scratch)
+        if len(self.forces) != 1:
+            permute_rows(self._sum_force, order, N, scratch)
+            permute_rows(self._sum_torque, order, N, scratch)
+            for r in range(6):
+                permute_rows(self._sum_virial[r], order, N, scratch)
+
     def _prepare_run(self, timestep):
         self._compute_net_force(timestep)
         # accelerations of the current forces (second half step with dt = 0)

@@ -35,6 +35,22 @@ class Force:
         self._out_c = o
         return o
 
+    def _reorder(self, order, scratch):
+        """Particles were reordered: move the outputs with them and make the
+        neighbor list rebuild."""
+        from ..state import permute_rows
+
+        if self._force is None:
+            return
+        N = self._sim.state.N
+        permute_rows(self._force, order, N, scratch)
+        permute_rows(self._torque, order, N, scratch)
+        for r in range(6):
+            permute_rows(self._virial[r], order, N, scratch)
+        nl = getattr(self, "nlist", None)
+        if nl is not None:
+            nl._invalidate()
+
     def _require(self):
         if not self._attached:
             raise RuntimeError("the force is not attached to a running simulation")
@@ -42,14 +58,14 @@ class Force:
     @property
     def forces(self):
         self._require()
-        N = self._sim.state.N
-        return self._force[:N, :3].cpu().numpy()
+        st = self._sim.state
+        return st._by_tag(self._force[:st.N, :3].cpu().numpy())
 
     @property
     def energies(self):
         self._require()
-        N = self._sim.state.N
-        return self._force[:N, 3].cpu().numpy()
+        st = self._sim.state
+        return st._by_tag(self._force[:st.N, 3].cpu().numpy())
 
     @property
     def energy(self):
@@ -63,11 +79,11 @@ class Force:
             # every step computes it, and this step's is computed here
             self._sim._virial_requested = True
             self._compute(self._sim.timestep, virial=True)
-        N = self._sim.state.N
-        return self._virial[:, :N].T.cpu().numpy()
+        st = self._sim.state
+        return st._by_tag(self._virial[:, :st.N].T.cpu().numpy())
 
     @property
     def torques(self):
         self._require()
-        N = self._sim.state.N
-        return self._torque[:N, :3].cpu().numpy()
+        st = self._sim.state
+        return st._by_tag(self._torque[:st.N, :3].cpu().numpy())

@@ -28,6 +28,11 @@ class NeighborList:
             self._last = None
             self._tiled = False
 
+    def _invalidate(self):
+        """Particles were reordered: the next update rebuilds."""
+        self._force_next = True
+        self._last = None
+
     def _register(self, pair):
         if pair not in self._users:
             self._users.append(pair)
@@ -65,7 +70,8 @@ class NeighborList:
                 self._cpp.set_tiled(tiled)
                 self._tiled = tiled
             self._pushed_for = run_id
-        self._cpp.update(self._sim.state.particles())
+        self._cpp.update(self._sim.state.particles(), force_rebuild=getattr(self, "_force_next", False))
+        self._force_next = False
         self._last = timestep
 
     @property

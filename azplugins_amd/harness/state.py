@@ -7,6 +7,17 @@ from .. import _azp
 torch = _azp.torch
 
 
+def permute_rows(t, order, N, scratch):
+    """t[k] = t[order[k]] for k < N, in place (through `scratch`, a byte
+    tensor of at least N * row bytes); rows of 4, 8, 16 or 32 bytes."""
+    rows = t[:N]
+    row_bytes = rows[0].numel() * rows.element_size()
+    tmp = scratch[:N * row_bytes].view(rows.dtype).view(rows.shape)
+    _azp.call("azp_gather_rows", _azp.dptr(rows), _azp.dptr(tmp), _azp.dptr(order), N, row_bytes,
+              _azp.stream_ptr())
+    rows.copy_(tmp)
+
+
 class Box:
     """Orthorhombic periodic box (tilt factors must be zero)."""
 
@@ -143,6 +154,42 @@ class State:
         self.accel = torch.zeros((N, 4), dtype=torch.float64, device=dev)
         self.pos_fixed = None
 
+    # ---- particle order (a sorter moves particles; tags travel with them) ----
+    def _scratch(self, nbytes):
+        sc = getattr(self, "_scratch_buf", None)
+        if sc is None or sc.numel() < nbytes:
+            sc = torch.empty(nbytes, dtype=torch.uint8, device=self.pos.device)
+            self._scratch_buf = sc
+        return sc
+
+    def _reorder(self, order):
+        """Moves particle order[k] to index k in every per-particle array."""
+        N = self.N
+        sc = self._scratch(32 * N)
+        for t in (self.pos, self.vel, self.tag, self.orientation, self.angmom, self.inertia, self.image,
+                  self.accel):
+            permute_rows(t, order, N, sc)
+        self._tag_host = None
+        self._sorted = True
+        return sc
+
+    def _tags(self):
+        """Host copy of the tags, or None while index == tag."""
+        if not getattr(self, "_sorted", False):
+            return None
+        if getattr(self, "_tag_host", None) is None:
+            self._tag_host = self.tag.cpu().numpy().astype(np.int64)
+        return self._tag_host
+
+    def _by_tag(self, a):
+        """Rows of a per-index host array rearranged by tag."""
+        tags = self._tags()
+        if tags is None:
+            return a
+        out = np.empty_like(a)
+        out[tags] = a
+        return out
+
     def _wrap_host(self, pos):
         L = self.box.L
         for a in range(3):
@@ -196,16 +243,17 @@ class State:
         snap.configuration.box = [self.box.Lx, self.box.Ly, self.box.Lz, 0.0, 0.0, 0.0]
         snap.particles.types = list(self.types)
         snap.particles.N = self.N
-        pos = self.pos.cpu().numpy()
-        vel = self.vel.cpu().numpy()
+        bt = self._by_tag
+        pos = bt(self.pos.cpu().numpy())
+        vel = bt(self.vel.cpu().numpy())
         snap.particles.position[:] = pos[:, :3]
         snap.particles.typeid[:] = pos[:, 3].astype(np.uint32)
         snap.particles.velocity[:] = vel[:, :3]
         snap.particles.mass[:] = vel[:, 3]
-        snap.particles.orientation[:] = self.orientation.cpu().numpy()
-        snap.particles.image[:] = self.image.cpu().numpy()[:, :3]
+        snap.particles.orientation[:] = bt(self.orientation.cpu().numpy())
+        snap.particles.image[:] = bt(self.image.cpu().numpy())[:, :3]
         ang = np.zeros((self.N, 4))
-        ang[:, 1:] = self.angmom.cpu().numpy()[:, :3]
+        ang[:, 1:] = bt(self.angmom.cpu().numpy())[:, :3]
         snap.particles.angmom[:] = ang
-        snap.particles.moment_inertia[:] = self.inertia.cpu().numpy()[:, None]
+        snap.particles.moment_inertia[:] = bt(self.inertia.cpu().numpy())[:, None]
         return snap

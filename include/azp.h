@@ -374,6 +374,24 @@ int azp_nlist_download(const azp_nlist* nl,
                        uint32_t* h_nlist,
                        uint64_t nlist_capacity);
 
+/* ---- particle sort (HOOMD's SFCPackTuner, the particle sorter every
+ * simulation of the reference runs [U: hoomd/SFCPackTuner.h, not in the
+ * reference tree]; azplugins itself has no counterpart).  The sorter bins
+ * the particles on a grid of about `grid` bins along the box's longest
+ * axis (coarsened to at most ~4 bins per particle) and writes d_order[k],
+ * the particle that moves to index k: bins in Morton order, each bin's
+ * members by current index.  d_order is a permutation of [0, N).
+ * azp_gather_rows then applies it to any per-particle array:
+ * dst[k] = src[order[k]] for rows of row_bytes (4, 8, 16 or 32); src and
+ * dst must not overlap.  Enqueued on `stream`; the sorter's buffers are
+ * reused between calls. ---- */
+typedef struct azp_sorter azp_sorter;
+int azp_sorter_create(azp_sorter** out);
+int azp_sorter_destroy(azp_sorter* so);
+int azp_sorter_order(azp_sorter* so, const azp_particles* p, uint32_t grid, uint32_t* d_order, void* stream);
+int azp_gather_rows(const void* d_src, void* d_dst, const uint32_t* d_order, uint32_t N, uint32_t row_bytes,
+                    void* stream);
+
 /* ---- pair potential objects (replace PotentialPair<Name>GPU,
  * PotentialPairDPDThermo<Name>GPU, AnisoPotentialPair<Name>GPU) ---- */
 typedef struct azp_pair azp_pair;

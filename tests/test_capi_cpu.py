@@ -174,3 +174,48 @@ def test_batch_host_side_validation():
         assert L.azp_batch_set_langevin(h, _azp.dbl_array([1.0] * 3), 0.0, 1.0, 1.0, 1.0, 0) == 1
     finally:
         L.azp_batch_destroy(h)
+
+
+def test_sorter_argument_checks():
+    """Argument checks of the particle sort entry points (no GPU work)."""
+    from azplugins_amd import _azp
+
+    L = _azp.lib()
+    h = ctypes.c_void_p()
+    _azp.call("azp_sorter_create", ctypes.byref(h))
+    try:
+        p = _azp.Particles()
+        p.N = 10
+        p.box = _azp.Box(10.0, 10.0, 10.0)
+        p.d_pos = 0x1000  # never dereferenced: validation fails first
+        assert L.azp_sorter_order(h, ctypes.byref(p), 0, ctypes.c_void_p(0x1000), None) == 1
+        assert L.azp_sorter_order(h, ctypes.byref(p), 1025, ctypes.c_void_p(0x1000), None) == 1
+        assert L.azp_sorter_order(h, ctypes.byref(p), 256, None, None) == 1
+        assert L.azp_sorter_order(None, ctypes.byref(p), 256, ctypes.c_void_p(0x1000), None) == 1
+        p.box = _azp.Box(10.0, 0.0, 10.0)
+        assert L.azp_sorter_order(h, ctypes.byref(p), 256, ctypes.c_void_p(0x1000), None) != 0
+        a, b, o = (ctypes.c_void_p(x) for x in (0x1000, 0x2000, 0x3000))
+        assert L.azp_gather_rows(a, b, o, 10, 12, None) == 1
+        assert b"row_bytes" in L.azp_last_error()
+        assert L.azp_gather_rows(a, a, o, 10, 16, None) == 1
+        assert L.azp_gather_rows(a, b, None, 10, 16, None) == 1
+        assert L.azp_gather_rows(None, None, None, 0, 16, None) == 0
+    finally:
+        L.azp_sorter_destroy(h)
+
+
+def test_particle_sorter_and_trigger():
+    import azplugins_amd.harness as hoomd
+
+    t = hoomd.trigger.Periodic(5, phase=2)
+    assert [s for s in range(20) if t(s)] == [2, 7, 12, 17]
+    assert hoomd.trigger.as_trigger(3)(9) and not hoomd.trigger.as_trigger(3)(10)
+    with pytest.raises(ValueError):
+        hoomd.trigger.Periodic(0)
+    sim = hoomd.Simulation(seed=1)
+    (sorter,) = sim.operations.tuners
+    assert isinstance(sorter, hoomd.tune.ParticleSorter)
+    assert sorter.grid == 256 and sorter.trigger(0) and sorter.trigger(400) and not sorter.trigger(100)
+    assert hoomd.tune.ParticleSorter(trigger=10, grid=64).grid == 64
+    with pytest.raises(ValueError):
+        hoomd.tune.ParticleSorter(grid=0)
