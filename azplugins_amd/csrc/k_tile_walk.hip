@@ -14,7 +14,8 @@
 //               27 cells from LDS (the same FP32 test with the FP64 decision
 //               near r_list as the plain walk, so the pair set is the same)
 //               and appends the candidate's LDS index to its own row of
-//               16-bit entries; every listed candidate is marked,
+//               16-bit entries; then every listed candidate is marked from
+//               the finished rows,
 //   4. compact: an exclusive scan of the marks gives each neighbor its rank
 //               in the tile's neighbor set (Morton cell order), whose
 //               particle indices go to jset,
@@ -212,83 +213,108 @@ __device__ uint32_t tile_walk_one(const TileWalkArgs& a,
     uint16_t* ring16 = reinterpret_cast<uint16_t*>(ring);
     uint16_t* row = a.rows + (size_t)slot * a.max_neighbors;
     uint32_t count = 0;
+    // a completed group read from the ring, not yet stored to the row
+    bool pend = false;
+    uint4 pend_data = make_uint4(0u, 0u, 0u, 0u);
+    uint32_t pend_at = 0;
+    auto store_pending = [&]() {
+        if (pend)
+            *reinterpret_cast<uint4*>(row + pend_at) = pend_data;
+        pend = false;
+    };
     // the next cell's range and first candidates are read one step ahead
-    // one batch: four candidates from m0 (entries past the cell's end masked)
+    // one batch: four candidates from m0 (entries past the cell's end masked).
+    // The flags are named scalars, not arrays: an array of flags becomes a
+    // packed byte vector in VGPRs, while scalars stay lane masks.
+    static_assert(kWalkBatch == 4, "the batch below is written out for four candidates");
     auto batch = [&](const int4 (&pc)[kWalkBatch], uint32_t m0, uint32_t e) {
-        bool hit[kWalkBatch], band[kWalkBatch];
-#pragma unroll
-        for (uint32_t q = 0; q < kWalkBatch; ++q)
-            {
-            const uint32_t m = m0 + q;
-            const uint32_t tj = (uint32_t)pc[q].w;
+        // offsets from m0: candidate q is in the range when q < rem and is
+        // the lane's own particle when q == dself (which wraps to a large
+        // value when self < m0)
+        const uint32_t rem = e - m0;
+        const uint32_t dself = self - m0;
+        auto test = [&](const int4& c, uint32_t q, bool& hit, bool& band) {
+            const uint32_t tj = (uint32_t)c.w;
             // with one type, a candidate of an invalid type can only exist
             // when its own lane reports it, which fails the build
-            const bool valid = (m < e) & (m != self) & (SINGLE || tj < a.ntypes);
+            const bool valid = (q < rem) & (q != dself) & (SINGLE || tj < a.ntypes);
             const uint32_t tp = SINGLE ? 0u : ti * a.ntypes + min(tj, a.ntypes - 1);
             const float lo = SINGLE ? lo0 : s.lohi[tp].x * unit2;
             const float hi = SINGLE ? hi0 : s.lohi[tp].y * unit2;
             float dx, dy, dz;
             if constexpr (REL)
                 {
-                dx = __int_as_float(pc[q].x) - rxi;
-                dy = __int_as_float(pc[q].y) - ryi;
-                dz = __int_as_float(pc[q].z) - rzi;
+                dx = __int_as_float(c.x) - rxi;
+                dy = __int_as_float(c.y) - ryi;
+                dz = __int_as_float(c.z) - rzi;
                 }
             else
                 {
-                dx = (float)fixed_delta(fxi, pc[q].x) * (CUBIC ? 1.0f : sx);
-                dy = (float)fixed_delta(fyi, pc[q].y) * (CUBIC ? 1.0f : sy);
-                dz = (float)fixed_delta(fzi, pc[q].z) * (CUBIC ? 1.0f : sz);
+                dx = (float)fixed_delta(fxi, c.x) * (CUBIC ? 1.0f : sx);
+                dy = (float)fixed_delta(fyi, c.y) * (CUBIC ? 1.0f : sy);
+                dz = (float)fixed_delta(fzi, c.z) * (CUBIC ? 1.0f : sz);
                 }
             // FP32 rounding here is far inside the 1e-5 band
             const float rsq = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
             const bool below = rsq < lo;
-            hit[q] = valid && below;
-            band[q] = valid && !below && rsq <= hi;
-            }
-        bool any_band = false;
-#pragma unroll
-        for (uint32_t q = 0; q < kWalkBatch; ++q)
-            any_band = any_band || band[q];
-        if (any_band)
-            {
-            // FP32 cannot decide: the FP64 predicate of the host oracle
-#pragma unroll
-            for (uint32_t q = 0; q < kWalkBatch; ++q)
-                if (band[q])
-                    {
-                    const uint32_t m = m0 + q;
-                    const uint32_t u = union_cell_of(s.off, s.U, m);
-                    const uint32_t j = a.members[s.gs[u] + (m - s.off[u])];
-                    const double4 p64 = a.pos[j];
-                    const uint32_t tp = SINGLE ? 0u : ti * a.ntypes + (uint32_t)pc[q].w;
-                    const double rl = SINGLE ? rl0 : a.r_listsq[tp];
-                    const double qx = min_image_1d(pi.x - p64.x, a.box.L[0], a.box.Linv[0]);
-                    const double qy = min_image_1d(pi.y - p64.y, a.box.L[1], a.box.Linv[1]);
-                    const double qz = min_image_1d(pi.z - p64.z, a.box.L[2], a.box.Linv[2]);
-                    hit[q] = rsq3_nofma(qx, qy, qz) <= rl;
-                    }
-            }
-        // append the hits in candidate order: positions are a running sum
-        // of the hit flags; the marks share one base address
-        uint16_t* mark = s.mark + m0;
-        uint32_t pos = count;
-#pragma unroll
-        for (uint32_t q = 0; q < kWalkBatch; ++q)
-            {
-            if (hit[q])
+            hit = valid & below;
+            band = valid & !below & (rsq <= hi);
+        };
+        // The FP64 predicate of the host oracle for a candidate FP32 cannot
+        // decide; only the loads are lane-divergent.
+        auto retest = [&](const int4& c, uint32_t q, bool band) {
+            double rsq64 = 0.0, rl = 0.0;
+            if (band)
                 {
-                mark[q] = 1u;
-                ring16[pos & 15u] = (uint16_t)(m0 + q);
+                const uint32_t m = m0 + q;
+                const uint32_t u = union_cell_of(s.off, s.U, m);
+                const uint32_t j = a.members[s.gs[u] + (m - s.off[u])];
+                const double4 p64 = a.pos[j];
+                const uint32_t tp = SINGLE ? 0u : ti * a.ntypes + (uint32_t)c.w;
+                rl = SINGLE ? rl0 : a.r_listsq[tp];
+                const double qx = min_image_1d(pi.x - p64.x, a.box.L[0], a.box.Linv[0]);
+                const double qy = min_image_1d(pi.y - p64.y, a.box.L[1], a.box.Linv[1]);
+                const double qz = min_image_1d(pi.z - p64.z, a.box.L[2], a.box.Linv[2]);
+                rsq64 = rsq3_nofma(qx, qy, qz);
                 }
-            pos += (uint32_t)hit[q];
+            return band & (rsq64 <= rl);
+        };
+        bool h0, h1, h2, h3, b0, b1, b2, b3;
+        test(pc[0], 0u, h0, b0);
+        test(pc[1], 1u, h1, b1);
+        test(pc[2], 2u, h2, b2);
+        test(pc[3], 3u, h3, b3);
+        // taken by the whole wavefront when any lane has such a candidate
+        if (__ballot(b0 | b1 | b2 | b3) != 0ull)
+            {
+            h0 = h0 | retest(pc[0], 0u, b0);
+            h1 = h1 | retest(pc[1], 1u, b1);
+            h2 = h2 | retest(pc[2], 2u, b2);
+            h3 = h3 | retest(pc[3], 3u, b3);
             }
-        // a completed group of eight leaves as one 16-byte store
+        // Append the hits in candidate order.  Every candidate is written to
+        // the ring slot after the last hit; a miss is overwritten by the next
+        // candidate, so the ring needs no per-candidate branch.
+        uint32_t pos = count;
+        auto put = [&](uint32_t q, bool h) {
+            ring16[pos & 15u] = (uint16_t)(m0 + q);
+            pos += h ? 1u : 0u;
+        };
+        put(0u, h0);
+        put(1u, h1);
+        put(2u, h2);
+        put(3u, h3);
+        // A completed group of eight leaves as one 16-byte store.  The ring
+        // is read here, the store is issued after the next batch's reads
+        // (store_pending): waiting for this read right away would also wait
+        // for every LDS access issued before it.
         if (((pos ^ count) & ~7u) != 0u && count < a.max_neighbors)
             {
             const uint2* h = reinterpret_cast<const uint2*>(ring + ((count >> 1) & 4u));
             const uint2 w0 = h[0], w1 = h[1];
-            *reinterpret_cast<uint4*>(row + (count & ~7u)) = make_uint4(w0.x, w0.y, w1.x, w1.y);
+            pend_data = make_uint4(w0.x, w0.y, w1.x, w1.y);
+            pend_at = count & ~7u;
+            pend = true;
             }
         count = pos;
     };
@@ -312,15 +338,18 @@ __device__ uint32_t tile_walk_one(const TileWalkArgs& a,
             {
             load_batch(pb, m0 + kWalkBatch);
             __builtin_amdgcn_sched_barrier(0);
+            store_pending();
             batch(pa, m0, e);
             if (m0 + kWalkBatch < e)
                 {
                 load_batch(pa, m0 + 2 * kWalkBatch);
                 __builtin_amdgcn_sched_barrier(0);
+                store_pending();
                 batch(pb, m0 + kWalkBatch, e);
                 }
             }
         }
+    store_pending();
     if ((count & 7u) != 0u && count < a.max_neighbors)
         {
         const uint2* h = reinterpret_cast<const uint2*>(ring + ((count >> 1) & 4u));
@@ -566,6 +595,23 @@ template<uint32_t T, bool SINGLE> __global__ void __launch_bounds__(T) k_tile_wa
         a.n_neigh[i] = min(count, a.max_neighbors);
         if (count > a.max_neighbors)
             atomicMax(a.flags, count);
+        // mark the listed candidates, reading the row back in 16-byte chunks
+        // (once per entry here instead of once per tested candidate above)
+        const uint32_t nl = min(count, a.max_neighbors);
+        const uint4* row = reinterpret_cast<const uint4*>(a.rows + (size_t)slot * a.max_neighbors);
+        for (uint32_t k0 = 0; k0 < nl; k0 += 8)
+            {
+            const uint4 ch = row[k0 / 8];
+            const uint32_t w4[4] = {ch.x, ch.y, ch.z, ch.w};
+#pragma unroll
+            for (uint32_t w = 0; w < 4; ++w)
+                {
+                if (k0 + 2 * w < nl)
+                    s_mark[w4[w] & 0xffffu] = 1u;
+                if (k0 + 2 * w + 1 < nl)
+                    s_mark[w4[w] >> 16] = 1u;
+                }
+            }
         }
     __syncthreads();
 
