@@ -171,6 +171,7 @@ SIGNATURES = [
     ("azp_gpu_integrate_rot_two", _i, [_vp, _vp, _vp, _vp, _u32, ctypes.c_double, _i, ctypes.c_double,
                                        ctypes.c_double, ctypes.c_uint16, _u64, _vp]),
     ("azp_thermo_workspace_size", _i, []),
+    ("azp_gpu_kinetic_tensor", _i, [_vp, _u32, _vp, _vp, _vp]),
     ("azp_gpu_thermo_reduce", _i, [_vp, _vp, _vp, _u32, _vp, _vp, _u32, _vp, _vp, _vp]),
     ("azp_nlist_create", _i, [ctypes.POINTER(_vp)]),
     ("azp_nlist_destroy", _i, [_vp]),

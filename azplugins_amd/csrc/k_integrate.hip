@@ -246,10 +246,48 @@ __global__ void __launch_bounds__(kReduceThreads) k_thermo_partial(const double4
         partial[(size_t)threadIdx.x * kReduceBlocks + blockIdx.x] = s[threadIdx.x][0];
     }
 
-__global__ void __launch_bounds__(kReduceThreads) k_thermo_final(const double* __restrict__ partial, double* __restrict__ out)
+// Kinetic tensor: fields 0..5 = sum of m v_a v_b for xx xy xz yy yz zz.
+constexpr int kKineticFields = 6;
+__global__ void __launch_bounds__(kReduceThreads) k_kinetic_partial(const double4* __restrict__ vel,
+                                                                    uint32_t N,
+                                                                    double* __restrict__ partial)
+    {
+    double acc[kKineticFields];
+#pragma unroll
+    for (int f = 0; f < kKineticFields; ++f)
+        acc[f] = 0.0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x)
+        {
+        const double4 v = vel[i];
+        acc[0] += v.w * v.x * v.x;
+        acc[1] += v.w * v.x * v.y;
+        acc[2] += v.w * v.x * v.z;
+        acc[3] += v.w * v.y * v.y;
+        acc[4] += v.w * v.y * v.z;
+        acc[5] += v.w * v.z * v.z;
+        }
+    __shared__ double s[kKineticFields][kReduceThreads];
+#pragma unroll
+    for (int f = 0; f < kKineticFields; ++f)
+        s[f][threadIdx.x] = acc[f];
+    __syncthreads();
+    for (int off = kReduceThreads / 2; off > 0; off >>= 1)
+        {
+        if ((int)threadIdx.x < off)
+            for (int f = 0; f < kKineticFields; ++f)
+                s[f][threadIdx.x] += s[f][threadIdx.x + off];
+        __syncthreads();
+        }
+    if (threadIdx.x < kKineticFields)
+        partial[(size_t)threadIdx.x * kReduceBlocks + blockIdx.x] = s[threadIdx.x][0];
+    }
+
+__global__ void __launch_bounds__(kReduceThreads) k_thermo_final(const double* __restrict__ partial,
+                                                                 int nfields,
+                                                                 double* __restrict__ out)
     {
     __shared__ double s[kReduceThreads];
-    for (int f = 0; f < kThermoFields; ++f)
+    for (int f = 0; f < nfields; ++f)
         {
         double a = 0.0;
         for (int b = threadIdx.x; b < kReduceBlocks; b += kReduceThreads)
@@ -387,7 +425,19 @@ extern "C" int azp_gpu_thermo_reduce(const double* d_vel,
     hipLaunchKernelGGL(k_thermo_partial, dim3(kReduceBlocks), dim3(kReduceThreads), 0, s,
                        reinterpret_cast<const double4*>(d_vel), reinterpret_cast<const double4*>(d_force),
                        d_virial, virial_pitch, reinterpret_cast<const double4*>(d_angmom), d_inertia, N, d_work);
-    hipLaunchKernelGGL(k_thermo_final, dim3(1), dim3(kReduceThreads), 0, s, d_work, d_out);
+    hipLaunchKernelGGL(k_thermo_final, dim3(1), dim3(kReduceThreads), 0, s, d_work, kThermoFields, d_out);
+    AZP_LAUNCH_CHECK();
+    return AZP_OK;
+    }
+
+extern "C" int azp_gpu_kinetic_tensor(const double* d_vel, uint32_t N, double* d_work, double* d_out, void* stream)
+    {
+    if (!d_vel || !d_work || !d_out)
+        return set_error(AZP_ERR_INVALID, "bad reduction arguments");
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_kinetic_partial, dim3(kReduceBlocks), dim3(kReduceThreads), 0, s,
+                       reinterpret_cast<const double4*>(d_vel), N, d_work);
+    hipLaunchKernelGGL(k_thermo_final, dim3(1), dim3(kReduceThreads), 0, s, d_work, kKineticFields, d_out);
     AZP_LAUNCH_CHECK();
     return AZP_OK;
     }

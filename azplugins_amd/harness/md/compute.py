@@ -70,12 +70,15 @@ class ThermodynamicQuantities:
 
     @property
     def pressure_tensor(self):
+        """(sum of m v_a v_b + W_ab) / V in the order xx, xy, xz, yy, yz, zz."""
         r = self._reduce()
-        V = self._sim.state.box.volume
-        # kinetic part needs m v_a v_b; use the isotropic share for the diagonal
-        k = 2.0 * r[0] / 3.0
+        st = self._sim.state
+        _azp.call("azp_gpu_kinetic_tensor", _azp.dptr(st.vel), st.N, _azp.dptr(self._work),
+                  _azp.dptr(self._out), _azp.stream_ptr())
+        K = self._out[:6].cpu().numpy()
+        V = st.box.volume
         W = r[3:9]
-        return [(k + W[0]) / V, W[1] / V, W[2] / V, (k + W[3]) / V, W[4] / V, (k + W[5]) / V]
+        return [float((K[c] + W[c]) / V) for c in range(6)]
 
     @property
     def pressure(self):

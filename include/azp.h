@@ -314,6 +314,10 @@ int azp_gpu_thermo_reduce(const double* d_vel,
                           double* d_work,
                           double* d_out,
                           void* stream);
+/* Kinetic tensor for the pressure tensor (upstream ComputeThermo's
+ * Σ m v_a v_b): out[6] = xx xy xz yy yz zz, FP64 in a deterministic order;
+ * d_work as for azp_gpu_thermo_reduce. */
+int azp_gpu_kinetic_tensor(const double* d_vel, uint32_t N, double* d_work, double* d_out, void* stream);
 
 /* ---- neighbor list (replaces upstream CellListGPU + NeighborListGPUBinned;
  * reached from the pair classes through `nlist`, pair.py:110-111) ---- */

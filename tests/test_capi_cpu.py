@@ -215,7 +215,7 @@ def test_particle_sorter_and_trigger():
     sim = hoomd.Simulation(seed=1)
     (sorter,) = sim.operations.tuners
     assert isinstance(sorter, hoomd.tune.ParticleSorter)
-    assert sorter.grid == 256 and sorter.trigger(0) and sorter.trigger(400) and not sorter.trigger(100)
+    assert sorter.grid is None and sorter.trigger(0) and sorter.trigger(400) and not sorter.trigger(100)
     assert hoomd.tune.ParticleSorter(trigger=10, grid=64).grid == 64
     with pytest.raises(ValueError):
         hoomd.tune.ParticleSorter(grid=0)

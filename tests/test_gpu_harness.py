@@ -49,3 +49,29 @@ def test_virials_on_demand(precision):
     say(f"[harness] {precision}: on-demand virial max |v| {np.abs(got).max():.3e}")
     assert np.array_equal(got, ref.virial.T)
     assert np.array_equal(plj.forces, ref.force[:, :3])
+
+
+def test_pressure_tensor():
+    """P_ab = (sum of m v_a v_b + sum of W_ab) / V, and P = trace / 3."""
+    import azplugins_amd.harness as hoomd
+
+    L = 10.0
+    pos4, rng = jittered_lattice(10, L, 0.1, 1, seed=23)
+    sim, plj = _sim(pos4, L, "fp64")
+    vel = rng.normal(size=(pos4.shape[0], 3))
+    snap = sim.state.get_snapshot()
+    snap.particles.velocity[:] = vel
+    state_vel = np.zeros((pos4.shape[0], 4))
+    state_vel[:, :3] = vel
+    state_vel[:, 3] = 1.0
+    sim.state.vel.copy_(hoomd.md.torch.tensor(state_vel, device="cuda"))
+    thermo = hoomd.md.compute.ThermodynamicQuantities(hoomd.filter.All())
+    sim.operations.computes.append(thermo)
+    sim.run(0)
+    got = np.array(thermo.pressure_tensor)
+    K = np.array([(vel[:, a] * vel[:, b]).sum() for a, b in ((0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2))])
+    W = plj.virials.sum(axis=0)
+    ref = (K + W) / L**3
+    say(f"[harness] pressure tensor {got}")
+    assert np.allclose(got, ref, rtol=1e-10, atol=1e-12 * np.abs(ref).max())
+    assert thermo.pressure == pytest.approx((got[0] + got[3] + got[5]) / 3.0, rel=1e-10)
