@@ -167,6 +167,8 @@ SIGNATURES = [
     ("azp_gpu_integrate_vv_one", _i, [_vp, _vp, _vp, _vp, _u32, ctypes.POINTER(Box), ctypes.c_double, _vp]),
     ("azp_gpu_integrate_vv_two", _i, [_vp, _vp, _vp, _vp, _u32, ctypes.c_double, _i, ctypes.c_double,
                                       ctypes.c_double, ctypes.c_uint16, _u64, _vp]),
+    ("azp_gpu_integrate_vv_two_one", _i, [_vp, _vp, _vp, _vp, _vp, _u32, ctypes.POINTER(Box), ctypes.c_double, _i,
+                                          ctypes.c_double, ctypes.c_double, ctypes.c_uint16, _u64, _vp]),
     ("azp_gpu_integrate_rot_one", _i, [_vp, _vp, _vp, _vp, _u32, ctypes.c_double, _vp]),
     ("azp_gpu_integrate_rot_two", _i, [_vp, _vp, _vp, _vp, _u32, ctypes.c_double, _i, ctypes.c_double,
                                        ctypes.c_double, ctypes.c_uint16, _u64, _vp]),

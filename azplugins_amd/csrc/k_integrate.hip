@@ -105,6 +105,75 @@ __global__ void k_vv_two(double4* __restrict__ vel,
     accel[i] = acc;
     }
 
+// The second half step of step t and the first half step of step t + 1 in
+// one pass (nothing runs between them): the same FP64 operations in the same
+// order as k_vv_two followed by k_nve_one, so the trajectory is bit-identical,
+// without storing the intermediate velocity and the acceleration.
+__global__ void k_vv_two_one(double4* __restrict__ pos,
+                             double4* __restrict__ vel,
+                             int4* __restrict__ image,
+                             const double4* __restrict__ force,
+                             const uint32_t* __restrict__ tag,
+                             uint32_t N,
+                             DevBox box,
+                             double dt,
+                             int langevin,
+                             double gamma,
+                             double kT,
+                             uint16_t seed,
+                             uint64_t timestep)
+    {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N)
+        return;
+    double4 v = vel[i];
+    const double4 f = force[i];
+    double fx = f.x, fy = f.y, fz = f.z;
+    if (langevin)
+        {
+        uint32_t k0, k1;
+        hoomd_seed_key(kRngLangevin, timestep, seed, k0, k1);
+        u32x4 c;
+        c.v[0] = tag[i];
+        c.v[1] = 0u;
+        c.v[2] = 0u;
+        c.v[3] = 0u;
+        const u32x4 r0 = philox4x32_10(c, k0, k1);
+        c.v[3] = 1u;
+        const u32x4 r1 = philox4x32_10(c, k0, k1);
+        u32x4 a, b, d;
+        a.v[0] = r0.v[0]; a.v[1] = r0.v[1];
+        b.v[0] = r0.v[2]; b.v[1] = r0.v[3];
+        d.v[0] = r1.v[0]; d.v[1] = r1.v[1];
+        const double coeff = sqrt(6.0 * gamma * kT / dt);
+        fx += -gamma * v.x + coeff * uniform_m1_1(a);
+        fy += -gamma * v.y + coeff * uniform_m1_1(b);
+        fz += -gamma * v.z + coeff * uniform_m1_1(d);
+        }
+    const double minv = 1.0 / v.w;
+    const double ax = fx * minv, ay = fy * minv, az = fz * minv;
+    // second half of step t
+    v.x += 0.5 * dt * ax;
+    v.y += 0.5 * dt * ay;
+    v.z += 0.5 * dt * az;
+    // first half of step t + 1
+    v.x += 0.5 * dt * ax;
+    v.y += 0.5 * dt * ay;
+    v.z += 0.5 * dt * az;
+    double4 p = pos[i];
+    p.x += dt * v.x;
+    p.y += dt * v.y;
+    p.z += dt * v.z;
+    int4 im = image ? image[i] : make_int4(0, 0, 0, 0);
+    wrap_1d(p.x, im.x, box.L[0]);
+    wrap_1d(p.y, im.y, box.L[1]);
+    wrap_1d(p.z, im.z, box.L[2]);
+    pos[i] = p;
+    vel[i] = v;
+    if (image)
+        image[i] = im;
+    }
+
 // Spherical tops: L += dt tau / 2, then q <- exp(dt omega / 2) q with omega = L / I.
 __global__ void k_rot_one(double4* __restrict__ orientation,
                           double4* __restrict__ angmom,
@@ -354,6 +423,36 @@ extern "C" int azp_gpu_integrate_vv_two(double* d_vel,
                        reinterpret_cast<double4*>(d_vel), reinterpret_cast<double4*>(d_accel),
                        reinterpret_cast<const double4*>(d_force), d_tag, N, dt, langevin, gamma, kT, seed,
                        timestep);
+    AZP_LAUNCH_CHECK();
+    return AZP_OK;
+    }
+
+extern "C" int azp_gpu_integrate_vv_two_one(double* d_pos,
+                                            double* d_vel,
+                                            int32_t* d_image,
+                                            const double* d_force,
+                                            const uint32_t* d_tag,
+                                            uint32_t N,
+                                            const azp_box* box,
+                                            double dt,
+                                            int langevin,
+                                            double gamma,
+                                            double kT,
+                                            uint16_t seed,
+                                            uint64_t timestep,
+                                            void* stream)
+    {
+    if (N == 0)
+        return AZP_OK;
+    if (!d_pos || !d_vel || !d_force || !box || !box_valid(*box))
+        return set_error(AZP_ERR_INVALID, "bad integration arguments");
+    if (langevin && !d_tag)
+        return set_error(AZP_ERR_INVALID, "the Langevin step needs tags");
+    const unsigned B = 256;
+    hipLaunchKernelGGL(k_vv_two_one, dim3(grid_for(N, B)), dim3(B), 0, (hipStream_t)stream,
+                       reinterpret_cast<double4*>(d_pos), reinterpret_cast<double4*>(d_vel),
+                       reinterpret_cast<int4*>(d_image), reinterpret_cast<const double4*>(d_force), d_tag, N,
+                       make_box(*box), dt, langevin, gamma, kT, seed, timestep);
     AZP_LAUNCH_CHECK();
     return AZP_OK;
     }

@@ -133,11 +133,16 @@ class Simulation:
             for t in tuners:
                 if t._sim is not self:
                     t._attach(self)
-            for _ in range(int(steps)):
+            n = int(steps)
+            for k in range(n):
                 for t in tuners:
                     t._act(self.timestep)
                 if integ is not None:
-                    integ._update(self.timestep)
+                    # the next step's first half step may join this step's
+                    # second half when no tuner acts in between
+                    nxt = self.timestep + 1
+                    fuse = k + 1 < n and not any(t.trigger(nxt) for t in tuners)
+                    integ._update(self.timestep, fuse_next=fuse)
                 self.timestep += 1
         finally:
             self.state._thaw_view()

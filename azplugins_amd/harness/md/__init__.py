@@ -19,6 +19,7 @@ class Integrator:
         self.forces = list(forces) if forces is not None else []
         self.methods = list(methods) if methods is not None else []
         self._sim = None
+        self._one_done = False
 
     def _attach(self, sim):
         self._sim = sim
@@ -70,10 +71,21 @@ class Integrator:
         # accelerations of the current forces (second half step with dt = 0)
         for m in self.methods:
             m._step_two(self, timestep, dt=0.0)
+        self._one_done = False
 
-    def _update(self, timestep):
-        for m in self.methods:
-            m._step_one(self, timestep)
+    # Steps whose second half runs straight into the next step's first half
+    # (nothing acts between them) fuse the two in one kernel per particle.
+    fuse_half_steps = True
+
+    def _update(self, timestep, fuse_next=False):
+        if not self._one_done:
+            for m in self.methods:
+                m._step_one(self, timestep)
         self._compute_net_force(timestep + 1)
-        for m in self.methods:
-            m._step_two(self, timestep + 1)
+        if fuse_next and self.fuse_half_steps and len(self.methods) == 1:
+            self.methods[0]._step_two_one(self, timestep + 1)
+            self._one_done = True
+        else:
+            for m in self.methods:
+                m._step_two(self, timestep + 1)
+            self._one_done = False

@@ -41,6 +41,24 @@ class Method:
                       timestep, _azp.stream_ptr())
 
 
+    def _step_two_one(self, integ, timestep):
+        """_step_two(timestep) then _step_one(timestep), the translational
+        part in one kernel (bit-identical to the two)."""
+        st = self._sim.state
+        lang, gamma, gamma_r, kT = self._langevin(timestep)
+        seed = self._sim.seed
+        box = st.box.to_c()
+        _azp.call("azp_gpu_integrate_vv_two_one", _azp.dptr(st.pos), _azp.dptr(st.vel), _azp.dptr(st.image),
+                  _azp.dptr(integ._net_force), _azp.dptr(st.tag), st.N, _azp.ctypes.byref(box), integ.dt, lang,
+                  gamma, kT, seed, timestep, _azp.stream_ptr())
+        if integ.integrate_rotational_dof:
+            _azp.call("azp_gpu_integrate_rot_two", _azp.dptr(st.angmom), _azp.dptr(integ._net_torque),
+                      _azp.dptr(st.inertia), _azp.dptr(st.tag), st.N, integ.dt, lang, gamma_r, kT, seed,
+                      timestep, _azp.stream_ptr())
+            _azp.call("azp_gpu_integrate_rot_one", _azp.dptr(st.orientation), _azp.dptr(st.angmom),
+                      _azp.dptr(integ._net_torque), _azp.dptr(st.inertia), st.N, integ.dt, _azp.stream_ptr())
+
+
 class ConstantVolume(Method):
     """Velocity Verlet at constant volume (NVE when thermostat is None)."""
 

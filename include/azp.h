@@ -278,6 +278,24 @@ int azp_gpu_integrate_vv_two(double* d_vel,
                              uint16_t seed,
                              uint64_t timestep,
                              void* stream);
+/* azp_gpu_integrate_vv_two at `timestep` followed by azp_gpu_integrate_vv_one
+ * in one pass, for the steps where nothing runs between the two (this
+ * library's own addition): the same FP64 operations in the same order, so
+ * the trajectory is bit-identical; the acceleration is not stored. */
+int azp_gpu_integrate_vv_two_one(double* d_pos,
+                                 double* d_vel,
+                                 int32_t* d_image,
+                                 const double* d_force,
+                                 const uint32_t* d_tag,
+                                 uint32_t N,
+                                 const azp_box* box,
+                                 double dt,
+                                 int langevin,
+                                 double gamma,
+                                 double kT,
+                                 uint16_t seed,
+                                 uint64_t timestep,
+                                 void* stream);
 /* spherical tops: angmom double4[N] space-frame L, inertia double[N] (isotropic I).
  * rot_one: L += dt tau / 2, then rotate q by omega dt (omega = L / I).
  * rot_two: L += dt tau / 2 [+ dt (-gamma_r L / I + sqrt(6 gamma_r kT / dt) U(-1,1))] */
