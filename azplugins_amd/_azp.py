@@ -193,6 +193,13 @@ SIGNATURES = [
     ("azp_sorter_destroy", _i, [_vp]),
     ("azp_sorter_order", _i, [_vp, ctypes.POINTER(Particles), _u32, _vp, _vp]),
     ("azp_gather_rows", _i, [_vp, _vp, _vp, _u32, _u32, _vp]),
+    ("azp_wall_create", _i, [ctypes.POINTER(_vp), ctypes.c_char_p, _u32]),
+    ("azp_wall_destroy", _i, [_vp]),
+    ("azp_wall_param_nvalues", _i, [_vp]),
+    ("azp_wall_set_params", _i, [_vp, _u32, _pd]),
+    ("azp_wall_get_params", _i, [_vp, _u32, _pd]),
+    ("azp_wall_set_geometry", _i, [_vp, _u32, _pd, _u32, _pd, _u32, _pd]),
+    ("azp_wall_compute", _i, [_vp, ctypes.POINTER(Particles), ctypes.POINTER(Outputs), _i, _vp]),
     ("azp_pair_create", _i, [ctypes.POINTER(_vp), ctypes.c_char_p, _u32]),
     ("azp_pair_destroy", _i, [_vp]),
     ("azp_pair_set_params", _i, [_vp, _u32, _u32, _pd]),

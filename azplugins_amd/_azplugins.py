@@ -102,6 +102,58 @@ for _n in _NAMES:
 del _n, _suffix, _cls
 
 
+class WallsPotentialGPUBase:
+    """Owns one ``azp_wall`` handle (the reference's WallsPotential<Name>[GPU],
+    export_PotentialExternalWall.cc.inc)."""
+
+    cpp_class_name = None
+
+    def __init__(self, ntypes):
+        self._h = ctypes.c_void_p()
+        _azp.call("azp_wall_create", ctypes.byref(self._h), self.cpp_class_name.encode(), int(ntypes))
+        self.ntypes = int(ntypes)
+        self.nvalues = _azp.lib().azp_wall_param_nvalues(self._h)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                _azp.lib().azp_wall_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    def setParams(self, t, values):
+        _azp.call("azp_wall_set_params", self._h, int(t), _azp.dbl_array(values))
+
+    def getParams(self, t):
+        arr = (ctypes.c_double * self.nvalues)()
+        _azp.call("azp_wall_get_params", self._h, int(t), arr)
+        return list(arr)
+
+    def setGeometry(self, spheres, cylinders, planes):
+        """Rows as in include/azp.h (sphere 6, cylinder 9, plane 7 values)."""
+        def flat(rows, width):
+            vals = [float(x) for r in rows for x in r]
+            assert len(vals) == width * len(rows)
+            return len(rows), (_azp.dbl_array(vals) if vals else None)
+
+        ns, sp = flat(spheres, 6)
+        nc, cy = flat(cylinders, 9)
+        np_, pl = flat(planes, 7)
+        _azp.call("azp_wall_set_geometry", self._h, ns, sp, nc, cy, np_, pl)
+
+    def compute(self, particles, outputs, compute_virial=True, stream=None):
+        _azp.call("azp_wall_compute", self._h, ctypes.byref(particles), ctypes.byref(outputs),
+                  1 if compute_virial else 0, _azp.stream_ptr(stream))
+
+
+for _n in ("WallsPotentialLJ93", "WallsPotentialColloid"):
+    for _suffix in ("", "GPU"):
+        globals()[_n + _suffix] = type(_n + _suffix, (WallsPotentialGPUBase,), {"cpp_class_name": _n + _suffix})
+del _n, _suffix
+
+
 class NeighborListGPU:
     """Owns one ``azp_nlist`` handle (cell list + full neighbor list)."""
 

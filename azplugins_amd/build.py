@@ -38,7 +38,7 @@ COMMON = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 
-NO_CONTRACT = {"k_nlist.hip", "k_tile_walk.hip", "k_dpd.hip", "k_integrate.hip"}
+NO_CONTRACT = {"k_nlist.hip", "k_tile_walk.hip", "k_dpd.hip", "k_integrate.hip", "k_wall.hip"}
 
 
 def sources():

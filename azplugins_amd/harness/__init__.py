@@ -5,7 +5,8 @@ provides the subset the pair-force path and its tests use — ``Snapshot``,
 ``Simulation`` with ``create_state_from_snapshot``/``run``/``timestep``/
 ``seed``, ``md.nlist.Cell``, ``md.Integrator``, ``md.methods.ConstantVolume``
 and ``md.methods.Langevin``, ``md.compute.ThermodynamicQuantities``,
-``tune.ParticleSorter``, ``trigger.Periodic``, ``filter.All`` and
+``tune.ParticleSorter``, ``trigger.Periodic``, ``wall`` (Sphere,
+Cylinder, Plane), ``md.external.wall.WallPotential``, ``filter.All`` and
 ``variant.Constant``.  Particle data lives in HBM as torch
 tensors (device memory only; all arithmetic runs in libazp.so kernels).
 
@@ -19,7 +20,7 @@ import warnings
 
 import numpy as np
 
-from . import filter, trigger, tune, variant  # noqa: F401
+from . import filter, trigger, tune, variant, wall  # noqa: F401
 from .state import Box, Snapshot, State  # noqa: F401
 
 
@@ -150,8 +151,9 @@ class Simulation:
 
 from . import md  # noqa: E402
 from .. import pair as _pair  # noqa: E402
+from .. import wall as _wall  # noqa: E402
 
-# ``hoomd.azplugins.pair`` and ``hoomd.device.GPU`` spellings
-azplugins = _types.SimpleNamespace(pair=_pair)
+# ``hoomd.azplugins.pair``, ``hoomd.azplugins.wall`` and ``hoomd.device.GPU`` spellings
+azplugins = _types.SimpleNamespace(pair=_pair, wall=_wall)
 device = _types.SimpleNamespace(GPU=GPU, Device=Device)
 _ = np

@@ -30,14 +30,35 @@ def _expand(key):
     raise KeyError(f"type-pair key expected, got {key!r}")
 
 
+def _expand1(key):
+    """Accept "A" or a list of type names."""
+    if isinstance(key, (list, tuple)):
+        out = []
+        for k in key:
+            out.extend(_expand1(k))
+        return out
+    if isinstance(key, str):
+        return [key]
+    raise KeyError(f"type key expected, got {key!r}")
+
+
 class TypeParameterDict:
-    """Schema-checked mapping from type pairs to parameter dicts."""
+    """Schema-checked mapping from type pairs (len_keys=2) or single types
+    (len_keys=1) to parameter dicts.  A schema entry is a type (required) or
+    a default value (optional, of that value's type), as in HOOMD."""
 
     def __init__(self, len_keys=2, **schema):
-        if len_keys != 2:
-            raise ValueError("only type-pair parameters are supported")
-        self._schema = schema
+        if len_keys not in (1, 2):
+            raise ValueError("len_keys must be 1 or 2")
+        self._len_keys = len_keys
+        self._schema = {k: (v if isinstance(v, type) else type(v)) for k, v in schema.items()}
+        self._defaults = {k: v for k, v in schema.items() if not isinstance(v, type)}
         self._data = {}
+
+    def _keys(self, key):
+        if self._len_keys == 1:
+            return _expand1(key)
+        return [_canon(k) for k in _expand(key)]
 
     def _convert(self, value):
         if not isinstance(value, dict):
@@ -45,6 +66,7 @@ class TypeParameterDict:
         extra = set(value) - set(self._schema)
         if extra:
             raise KeyError(f"unexpected parameter keys {sorted(extra)}")
+        value = {**self._defaults, **value}
         missing = [k for k in self._schema if k not in value]
         if missing:
             raise KeyError(f"missing required parameter keys {missing}")
@@ -65,14 +87,14 @@ class TypeParameterDict:
         return self._data.keys()
 
     def __contains__(self, key):
-        return _canon(key) in self._data
+        return (key if self._len_keys == 1 else _canon(key)) in self._data
 
     def __setitem__(self, key, value):
-        for k in _expand(key):
-            self._data[_canon(k)] = self._convert(value)
+        for k in self._keys(key):
+            self._data[k] = self._convert(value)
 
     def __getitem__(self, key):
-        return dict(self._data[_canon(key)])
+        return dict(self._data[key if self._len_keys == 1 else _canon(key)])
 
     def items(self):
         return self._data.items()

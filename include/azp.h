@@ -414,6 +414,37 @@ int azp_sorter_order(azp_sorter* so, const azp_particles* p, uint32_t grid, uint
 int azp_gather_rows(const void* d_src, void* d_dst, const uint32_t* d_order, uint32_t N, uint32_t row_bytes,
                     void* stream);
 
+/* ---- wall potentials (replace the reference's WallsPotentialLJ93[GPU] and
+ * WallsPotentialColloid[GPU], export_PotentialExternalWall.cc.inc and
+ * PotentialExternalWallGPUKernel.cu.inc:26-27, i.e. HOOMD's
+ * PotentialExternal<EvaluatorWalls<WallEvaluator*>> with the evaluators of
+ * WallEvaluatorLJ93.h / WallEvaluatorColloid.h).  cpp_class_name is
+ * "WallsPotentialLJ93" or "WallsPotentialColloid" (a "GPU" suffix is
+ * accepted).  Parameters per particle type in the schema order of the
+ * reference's wall.py: LJ93 (A, sigma, r_cut, r_extrap), colloid (A, a,
+ * sigma, r_cut, r_extrap); stored as the reference's derived fields and
+ * read back through its asDict formulas.  Geometry rows (host arrays):
+ * sphere {origin x y z, radius, inside, open}, cylinder {origin x y z,
+ * axis x y z, radius, inside, open}, plane {origin x y z, normal x y z,
+ * open}; inside / open as 0 or 1 (HOOMD's hoomd.wall attributes).  compute
+ * writes force.xyz, the energy (shifted to zero at r_cut, all of it on the
+ * particle) in force.w and, when asked, the virial 1/2 (s n) x F; it
+ * synchronizes the stream (it reads a type-error flag). ---- */
+typedef struct azp_wall azp_wall;
+int azp_wall_create(azp_wall** out, const char* cpp_class_name, uint32_t ntypes);
+int azp_wall_destroy(azp_wall* w);
+int azp_wall_param_nvalues(const azp_wall* w);
+int azp_wall_set_params(azp_wall* w, uint32_t type, const double* values);
+int azp_wall_get_params(const azp_wall* w, uint32_t type, double* values);
+int azp_wall_set_geometry(azp_wall* w,
+                          uint32_t n_spheres,
+                          const double* spheres,
+                          uint32_t n_cylinders,
+                          const double* cylinders,
+                          uint32_t n_planes,
+                          const double* planes);
+int azp_wall_compute(azp_wall* w, const azp_particles* p, const azp_outputs* out, int compute_virial, void* stream);
+
 /* ---- pair potential objects (replace PotentialPair<Name>GPU,
  * PotentialPairDPDThermo<Name>GPU, AnisoPotentialPair<Name>GPU) ---- */
 typedef struct azp_pair azp_pair;

@@ -101,6 +101,7 @@ def lib():
         L.orc_count_pairs_in_cut.restype = i64
         L.orc_run_nve.argtypes = [ci, u32, pd, pd, pd, u32, pd, pd, d, ci, d, ctypes.c_uint64, pd, pd]
         L.orc_run_nve.restype = i64
+        L.orc_wall_forces.argtypes = [ci, u32, pd, u32, pd, u32, pd, u32, pd, u32, pd, pd, pd]
         _lib = L
     return _lib
 
@@ -346,3 +347,26 @@ def run_nve(evaluator, pos4, vel4, L, ntypes, packed, r_cut, r_buff, mode, dt, s
         MODE_CODES.get(mode, mode), dt, steps, _pd(f), _pd(v),
     )
     return pos4, vel4, f, v, nb
+
+
+WALL_LJ93 = 0
+WALL_COLLOID = 1
+WALL_KEYS = {WALL_LJ93: ("A", "sigma", "r_cut", "r_extrap"),
+             WALL_COLLOID: ("A", "a", "sigma", "r_cut", "r_extrap")}
+
+
+def wall_forces(kind, pos4, values, spheres=(), cylinders=(), planes=(), with_virial=True):
+    """orc_wall_forces: values[t] in wall.py schema order; geometry rows as in
+    the C header.  Returns (force4 (N, 4), virial (6, N) or None)."""
+    pos4 = _f64(pos4)
+    N = pos4.shape[0]
+    vals = _f64(np.asarray(values, dtype=np.float64).reshape(-1))
+    ntypes = vals.shape[0] // len(WALL_KEYS[kind])
+    sp = _f64(np.asarray(spheres, dtype=np.float64).reshape(-1, 6))
+    cy = _f64(np.asarray(cylinders, dtype=np.float64).reshape(-1, 9))
+    pl = _f64(np.asarray(planes, dtype=np.float64).reshape(-1, 7))
+    f = np.zeros((N, 4))
+    v = np.zeros((6, N)) if with_virial else None
+    lib().orc_wall_forces(kind, N, _pd(pos4), ntypes, _pd(vals), sp.shape[0], _pd(sp), cy.shape[0], _pd(cy),
+                          pl.shape[0], _pd(pl), _pd(f), _pd(v) if v is not None else None)
+    return f, v

@@ -1235,3 +1235,197 @@ int64_t orc_run_nve(int evaluator,
     free(pj);
     return nbuilds;
 }
+
+/* ------------------------------------------------------------------------ */
+/* wall potentials                                                           */
+/* ------------------------------------------------------------------------ */
+
+/* Derived fields from the wall.py schema values (per type):
+ *   LJ93 (A, sigma, r_cut, r_extrap): sigma_3 = sigma^3 (WallEvaluatorLJ93.h:33-38)
+ *   colloid (A, a, sigma, r_cut, r_extrap): c_1 = A sigma^6 / 7560, c_2 = A / 6
+ *   (WallEvaluatorColloid.h:34-42).
+ * Energies and f/r: WallEvaluatorLJ93.h:105-131 (lj1 = 2/15 A sigma^9,
+ * lj2 = A sigma^3) and WallEvaluatorColloid.h:124-185 (computePotential),
+ * both shifted to zero at r_cut as the reference's golden values require
+ * (test_wall.py:22-56).  Parity: pinned by those four golden cases,
+ * F = -dE/ds by central differences and E(r_cut) = 0 (tests/test_oracle_walls.py). */
+static int wall_energy(int kind, const double* v, double rsq, double rcutsq, double* fdivr, double* e)
+{
+    if (kind == ORC_WALL_LJ93)
+        {
+        double sigma = v[1], A = v[0];
+        double s3 = sigma * sigma * sigma;
+        double lj1 = (2.0 / 15.0) * A * s3 * s3 * s3;
+        double lj2 = A * s3;
+        double r2i, r3i, r6i, c2i, c3i, c6i;
+        if (!(rsq < rcutsq && lj1 != 0.0))
+            return 0;
+        r2i = 1.0 / rsq;
+        r3i = r2i * sqrt(r2i);
+        r6i = r3i * r3i;
+        *fdivr = r2i * r3i * (9.0 * lj1 * r6i - 3.0 * lj2);
+        c2i = 1.0 / rcutsq;
+        c3i = c2i * sqrt(c2i);
+        c6i = c3i * c3i;
+        *e = r3i * (lj1 * r6i - lj2) - c3i * (lj1 * c6i - lj2);
+        return 1;
+        }
+    else
+        {
+        double A = v[0], a = v[1], sigma = v[2];
+        double s3 = sigma * sigma * sigma;
+        double c1 = A * s3 * s3 / 7560.0, c2 = A / 6.0;
+        double r, at;
+        int pass;
+        if (!(rsq < rcutsq && c1 != 0.0 && a > 0.0))
+            return 0;
+        *e = 0.0;
+        /* pass 0 at r, pass 1 at r_cut (the shift) */
+        for (pass = 0; pass < 2; ++pass)
+            {
+            double x2 = pass == 0 ? rsq : rcutsq;
+            double um, up, u2, um2, um6, up2, up6, en;
+            r = sqrt(x2);
+            at = a / r;
+            um = 1.0 / (r - a);
+            up = 1.0 / (r + a);
+            u2 = um * up;
+            um2 = um * um;
+            um6 = um2 * um2 * um2;
+            up2 = up * up;
+            up6 = up2 * up2 * up2;
+            if (pass == 0)
+                {
+                double a8 = 8.0 * at;
+                double f = 6.0 * c1 * ((a8 - 1.0) * um2 * um6 + (a8 + 1.0) * up2 * up6);
+                f -= c2 * (4.0 * a * a * at * u2 * u2);
+                *fdivr = f;
+                }
+            en = c1 * ((7.0 * a - r) * um * um6 + (7.0 * a + r) * up * up6);
+            en -= c2 * (2.0 * a * r * u2 + log(up / um));
+            *e += pass == 0 ? en : -en;
+            }
+        return 1;
+        }
+}
+
+/* One wall acting on one particle: signed distance s (positive inside),
+ * unit normal n into the inside.  Standard evaluation at r = s inside
+ * (s > 0, or s == 0 on a closed wall); with r_extrap > 0, every s <= r_extrap
+ * takes the linear extrapolation E(r_e) + f(r_e) (r_e - s), F = f(r_e) n —
+ * HOOMD's EvaluatorWalls (hoomd/md/EvaluatorWalls.h, not in the reference
+ * tree) as documented for hoomd.md.external.wall.  Parity unpinned for
+ * r_extrap > 0 and for the virial (1/2 (s n) x F): no reference test covers them. */
+static void wall_one(int kind, const double* v, double s, const double* n, int open, double* F, double* E,
+                     double* W)
+{
+    double r_cut = v[kind == ORC_WALL_LJ93 ? 2 : 3];
+    double r_e = v[kind == ORC_WALL_LJ93 ? 3 : 4];
+    double rcutsq = r_cut * r_cut;
+    double fdivr, e, f;
+    double d[3];
+    int k;
+    if (r_e > 0.0 && s <= r_e)
+        {
+        if (!wall_energy(kind, v, r_e * r_e, rcutsq, &fdivr, &e))
+            return;
+        f = fdivr * r_e;
+        e = e + f * (r_e - s);
+        }
+    else if (s > 0.0 || (s == 0.0 && !open))
+        {
+        if (!wall_energy(kind, v, s * s, rcutsq, &fdivr, &e))
+            return;
+        f = fdivr * s;
+        }
+    else
+        return;
+    for (k = 0; k < 3; ++k)
+        {
+        F[k] += f * n[k];
+        d[k] = s * n[k];
+        }
+    *E += e;
+    W[0] += 0.5 * d[0] * (f * n[0]);
+    W[1] += 0.5 * d[0] * (f * n[1]);
+    W[2] += 0.5 * d[0] * (f * n[2]);
+    W[3] += 0.5 * d[1] * (f * n[1]);
+    W[4] += 0.5 * d[1] * (f * n[2]);
+    W[5] += 0.5 * d[2] * (f * n[2]);
+}
+
+/* Geometry (hoomd.wall): sphere {origin, radius, inside, open}: s = R - |x - o|
+ * inside (n toward the center), |x - o| - R outside; cylinder {origin, axis,
+ * radius, inside, open}: the same with the distance from the axis; plane
+ * {origin, normal, open}: s = n . (x - o).  Axes and normals are normalized
+ * here.  Positions are used as given (walls do not wrap). */
+void orc_wall_forces(int kind,
+                     uint32_t N,
+                     const double* pos4,
+                     uint32_t ntypes,
+                     const double* values,
+                     uint32_t ns,
+                     const double* spheres,
+                     uint32_t nc,
+                     const double* cylinders,
+                     uint32_t np,
+                     const double* planes,
+                     double* force4,
+                     double* virial)
+{
+    int nv = kind == ORC_WALL_LJ93 ? 4 : 5;
+    uint32_t i, k;
+    for (i = 0; i < N; ++i)
+        {
+        const double* x = pos4 + 4 * (size_t)i;
+        uint32_t t = (uint32_t)x[3];
+        const double* v = values + (size_t)nv * (t < ntypes ? t : 0);
+        double F[3] = {0.0, 0.0, 0.0}, E = 0.0, W[6] = {0, 0, 0, 0, 0, 0};
+        double n[3];
+        int c;
+        for (k = 0; k < ns; ++k)
+            {
+            const double* g = spheres + 6 * (size_t)k;
+            double d[3] = {x[0] - g[0], x[1] - g[1], x[2] - g[2]};
+            double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+            int inside = g[4] != 0.0;
+            double s = inside ? g[3] - r : r - g[3];
+            for (c = 0; c < 3; ++c)
+                n[c] = r > 0.0 ? (inside ? -d[c] : d[c]) / r : 0.0;
+            wall_one(kind, v, s, n, g[5] != 0.0, F, &E, W);
+            }
+        for (k = 0; k < nc; ++k)
+            {
+            const double* g = cylinders + 9 * (size_t)k;
+            double an = sqrt(g[3] * g[3] + g[4] * g[4] + g[5] * g[5]);
+            double ax[3] = {g[3] / an, g[4] / an, g[5] / an};
+            double d[3] = {x[0] - g[0], x[1] - g[1], x[2] - g[2]};
+            double proj = d[0] * ax[0] + d[1] * ax[1] + d[2] * ax[2];
+            double q[3] = {d[0] - proj * ax[0], d[1] - proj * ax[1], d[2] - proj * ax[2]};
+            double r = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
+            int inside = g[7] != 0.0;
+            double s = inside ? g[6] - r : r - g[6];
+            for (c = 0; c < 3; ++c)
+                n[c] = r > 0.0 ? (inside ? -q[c] : q[c]) / r : 0.0;
+            wall_one(kind, v, s, n, g[8] != 0.0, F, &E, W);
+            }
+        for (k = 0; k < np; ++k)
+            {
+            const double* g = planes + 7 * (size_t)k;
+            double nn = sqrt(g[3] * g[3] + g[4] * g[4] + g[5] * g[5]);
+            double s;
+            n[0] = g[3] / nn;
+            n[1] = g[4] / nn;
+            n[2] = g[5] / nn;
+            s = (x[0] - g[0]) * n[0] + (x[1] - g[1]) * n[1] + (x[2] - g[2]) * n[2];
+            wall_one(kind, v, s, n, g[6] != 0.0, F, &E, W);
+            }
+        force4[4 * (size_t)i + 0] = F[0];
+        force4[4 * (size_t)i + 1] = F[1];
+        force4[4 * (size_t)i + 2] = F[2];
+        force4[4 * (size_t)i + 3] = E;
+        if (virial)
+            for (c = 0; c < 6; ++c)
+                virial[(size_t)c * N + i] = W[c];
+        }
+}

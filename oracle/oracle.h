@@ -223,4 +223,26 @@ int64_t orc_run_nve(int evaluator,
 }
 #endif
 
+/* wall potentials (WallEvaluatorLJ93.h, WallEvaluatorColloid.h on hoomd.wall
+ * geometries); values per type in wall.py schema order: LJ93 (A, sigma, r_cut,
+ * r_extrap), colloid (A, a, sigma, r_cut, r_extrap).  virial is 6 x N or NULL. */
+enum
+    {
+    ORC_WALL_LJ93 = 0,
+    ORC_WALL_COLLOID = 1
+    };
+void orc_wall_forces(int kind,
+                     uint32_t N,
+                     const double* pos4,
+                     uint32_t ntypes,
+                     const double* values,
+                     uint32_t ns,
+                     const double* spheres,
+                     uint32_t nc,
+                     const double* cylinders,
+                     uint32_t np,
+                     const double* planes,
+                     double* force4,
+                     double* virial);
+
 #endif
