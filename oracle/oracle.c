@@ -7,7 +7,8 @@
  *   PairEvaluatorHertz.h, PairEvaluatorExpandedYukawa.h,
  *   PairEvaluatorPerturbedLennardJones.h, PairEvaluatorColloid.h,
  *   DPDPairEvaluatorGeneralWeight.h, RNGIdentifiers.h,
- *   AnisoPairEvaluatorTwoPatchMorse.h, pair.py (parameter keys / modes).
+ *   AnisoPairEvaluatorTwoPatchMorse.h, pair.py (parameter keys / modes),
+ *   WallEvaluatorLJ93.h, WallEvaluatorColloid.h, wall.py (wall potentials).
  * Upstream HOOMD-blue v7.0.1 driver behaviour (PotentialPair,
  * PotentialPairDPDThermo, AnisoPotentialPair, NeighborListBinned, BoxDim,
  * RandomNumbers) is not in the reference tree; it is restated from its

@@ -4,7 +4,9 @@
  * CPU restatement (plain C, FP64, single thread) of azplugins' short-range
  * pair-force path: the isotropic, DPD and anisotropic evaluators and the
  * upstream HOOMD-blue driver semantics (half neighbor list, Newton's third law,
- * energy split 1/2-1/2, per-particle virial).  It is the checker for the HIP
+ * energy split 1/2-1/2, per-particle virial), and the wall potentials on
+ * HOOMD's wall geometries (their r_extrap extrapolation and virial: parity
+ * unpinned).  It is the checker for the HIP
  * product and the "port" CPU baseline.  Only tests/, __graft_entry__.smoke()
  * and bench.py's cpu_baseline leg may load it.  The product never does.
  *
