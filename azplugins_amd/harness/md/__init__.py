@@ -1,7 +1,7 @@
 """Molecular dynamics: integrator, methods, neighbor lists, pair forces."""
 
 from ... import _azp
-from . import compute, external, methods, nlist, pair  # noqa: F401
+from . import compute, external, methods, nlist, pair, tune  # noqa: F401
 
 torch = _azp.torch
 

@@ -20,6 +20,15 @@ class NeighborList:
         self._users = []
         self._last = None
 
+    @property
+    def buffer(self):
+        return self._buffer
+
+    @buffer.setter
+    def buffer(self, value):
+        self._buffer = float(value)
+        self._pushed_for = None  # push the settings again at the next update
+
     def _attach(self, sim):
         if self._cpp is None or self._sim is not sim:
             self._cpp = _azplugins.NeighborListGPU(self.buffer)

@@ -219,3 +219,34 @@ def test_particle_sorter_and_trigger():
     assert hoomd.tune.ParticleSorter(trigger=10, grid=64).grid == 64
     with pytest.raises(ValueError):
         hoomd.tune.ParticleSorter(grid=0)
+
+
+def test_buffer_tuner_grid():
+    """The buffer tuner's grid logic, driven without a GPU (timing stubbed)."""
+    import azplugins_amd.harness as hoomd
+    from azplugins_amd.harness.md import tune as mdtune
+
+    class FakeNlist:
+        buffer = 0.4
+
+    nl = FakeNlist()
+    t = hoomd.md.tune.NeighborListBuffer.with_grid(trigger=10, nlist=nl, maximum_buffer=0.8, minimum_buffer=0.2,
+                                                   n_bins=4, n_rounds=1)
+    speed = {0.2: 100.0, 0.4: 300.0, 0.6: 250.0, 0.8: 90.0, 0.3: 280.0, 0.5: 320.0}
+    clock = [0.0]
+    orig_sync, orig_time = mdtune.torch.cuda.synchronize, mdtune.time.perf_counter
+    mdtune.torch.cuda.synchronize = lambda: None
+    mdtune.time.perf_counter = lambda: clock[0]
+    try:
+        step = 0
+        while not t.tuned and step < 1000:
+            if t._current is not None and t.trigger(step):
+                clock[0] += 10.0 / speed[round(t._current, 10)]
+            t._act(step)
+            step += 10
+    finally:
+        mdtune.torch.cuda.synchronize, mdtune.time.perf_counter = orig_sync, orig_time
+    assert t.tuned
+    assert t.best_buffer == pytest.approx(0.5)
+    assert nl.buffer == pytest.approx(0.5)
+    assert set(round(b, 10) for b in t.results) == {0.2, 0.4, 0.6, 0.8, 0.3, 0.5}

@@ -103,3 +103,32 @@ def test_fused_half_steps_are_bit_identical(method):
     say(f"[harness] fused half steps ({method}): max |dx| {np.abs(runs[0][0] - runs[1][0]).max():.3e}")
     for a, b in zip(runs[0], runs[1]):
         assert np.array_equal(a, b)
+
+
+def test_buffer_tuner_in_md():
+    """The buffer tuner picks one of its grid values during a run; the forces
+    with the tuned buffer equal those of a fresh list."""
+    import azplugins_amd.harness as hoomd
+
+    L = 16.0
+    pos4, rng = jittered_lattice(16, L, 0.1, 1, seed=41)
+    sim, plj = _sim(pos4, L, "fp64")
+    integ = sim.operations.integrator
+    integ.methods = [hoomd.md.methods.Langevin(hoomd.filter.All(), kT=1.0)]
+    nl = plj.nlist
+    tuner = hoomd.md.tune.NeighborListBuffer.with_grid(trigger=30, nlist=nl, maximum_buffer=0.6,
+                                                       minimum_buffer=0.2, n_bins=3, n_rounds=0)
+    sim.operations.tuners.append(tuner)
+    sim.state.thermalize_particle_momenta(hoomd.filter.All(), kT=1.0)
+    sim.run(150)
+    say(f"[harness] buffer tuner: {tuner.results} -> {tuner.best_buffer}")
+    assert tuner.tuned and tuner.best_buffer in tuner.results
+    assert nl.buffer == tuner.best_buffer
+    f_run = plj.forces
+    snap = sim.state.get_snapshot()
+    p4 = np.zeros((snap.particles.N, 4))
+    p4[:, :3] = snap.particles.position
+    ref = gpu_pair("PotentialPairPerturbedLennardJones", GpuSystem(p4, np.array([L, L, L])), 1,
+                   {(0, 0): {"epsilon": 1.0, "sigma": 1.0, "attraction_scale_factor": 0.5}}, 3.0,
+                   r_buff=tuner.best_buffer)
+    np.testing.assert_allclose(f_run, ref.force[:, :3], rtol=1e-10, atol=1e-10 * np.abs(ref.force).max())
