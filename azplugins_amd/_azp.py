@@ -211,6 +211,8 @@ SIGNATURES = [
     ("azp_pair_set_thermostat", _i, [_vp, ctypes.c_uint16, ctypes.c_double, ctypes.c_double]),
     ("azp_pair_get_r_cut", _i, [_vp, _pd]),
     ("azp_pair_compute", _i, [_vp, _vp, ctypes.POINTER(Particles), ctypes.POINTER(Outputs), _u64, _i, _vp]),
+    ("azp_pair_update_compute", _i, [_vp, _vp, ctypes.POINTER(Particles), ctypes.POINTER(Outputs), _u64, _i, _i,
+                                     ctypes.POINTER(_i), _vp]),
     ("azp_batch_create", _i, [ctypes.POINTER(_vp), _u32, ctypes.POINTER(_u32), ctypes.POINTER(Box), _u32, _i]),
     ("azp_batch_destroy", _i, [_vp]),
     ("azp_batch_size", _i, [_vp, ctypes.POINTER(_u32), ctypes.POINTER(_u32)]),

@@ -72,6 +72,15 @@ class PotentialPairGPUBase:
         _azp.call("azp_pair_get_r_cut", self._h, arr)
         return list(arr)
 
+    def update_compute(self, nlist_handle, particles, outputs, timestep, compute_virial=True, force_rebuild=False,
+                       stream=None):
+        """azp_pair_update_compute: the neighbor list's update, then the force."""
+        rebuilt = ctypes.c_int()
+        _azp.call("azp_pair_update_compute", self._h, nlist_handle, ctypes.byref(particles), ctypes.byref(outputs),
+                  int(timestep), 1 if compute_virial else 0, 1 if force_rebuild else 0, ctypes.byref(rebuilt),
+                  _azp.stream_ptr(stream))
+        return bool(rebuilt.value)
+
     def compute(self, nlist_handle, particles, outputs, timestep, compute_virial=True, stream=None):
         _azp.call(
             "azp_pair_compute",

@@ -515,3 +515,17 @@ extern "C" int azp_pair_compute(azp_pair* pp,
         return set_error(AZP_ERR_STATE, "unknown driver");
         }
     }
+
+extern "C" int azp_pair_update_compute(azp_pair* pp,
+                                       azp_nlist* nl,
+                                       const azp_particles* p,
+                                       const azp_outputs* out,
+                                       uint64_t timestep,
+                                       int compute_virial,
+                                       int force_rebuild,
+                                       int* rebuilt,
+                                       void* stream)
+    {
+    AZP_TRY(azp_nlist_update(nl, p, force_rebuild, rebuilt, stream));
+    return azp_pair_compute(pp, nl, p, out, timestep, compute_virial, stream);
+    }

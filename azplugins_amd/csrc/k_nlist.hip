@@ -551,6 +551,11 @@ static int nlist_build(azp_nlist* nl, const azp_particles* p, hipStream_t s)
                 ta.flags = nl->flags.ptr;
                 ta.pitch = nl->pitch;
                 AZP_TRY(tiles_walk(nl->tiles, ts, ta, s));
+                // the positions of this build, queued before the flags are
+                // read so that no host round trip separates it from the walk
+                // (a failed attempt copies them again)
+                hipLaunchKernelGGL(k_copy_d4, dim3(grid_for(N, B)), dim3(B), 0, s, pos, nl->last_pos.ptr, N);
+                AZP_LAUNCH_CHECK();
                 AZP_HIP(hipMemcpyAsync(nl->h_flags, nl->flags.ptr, 6 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
                 AZP_HIP(hipStreamSynchronize(s));
                 if (nl->h_flags[2] & 2u)
@@ -584,8 +589,6 @@ static int nlist_build(azp_nlist* nl, const azp_particles* p, hipStream_t s)
                 }
             if (done)
                 {
-                hipLaunchKernelGGL(k_copy_d4, dim3(grid_for(N, B)), dim3(B), 0, s, pos, nl->last_pos.ptr, N);
-                AZP_LAUNCH_CHECK();
                 nl->built_N = N;
                 nl->built_box = p->box;
                 nl->have_list = true;

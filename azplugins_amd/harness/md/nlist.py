@@ -65,10 +65,11 @@ class NeighborList:
         return any(getattr(u, "_precision", "fp64") == "mixed" and u._cpp is not None
                    and u._cpp.driver in (_azp.DRIVER_PAIR, _azp.DRIVER_CONSERVATIVE) for u in self._users)
 
-    def _update(self, timestep):
+    def _needs_update(self, timestep):
+        """Whether this step's update (rebuild check) is still due; pushes the
+        settings first when they may have changed (once per run() otherwise)."""
         if self._last == timestep:
-            return
-        # the settings are pushed once per run() (nothing changes them inside)
+            return False
         run_id = getattr(self._sim.state, "_view", None)
         if run_id is None or run_id is not getattr(self, "_pushed_for", None):
             nt, m = self._r_cut_matrix()
@@ -79,8 +80,17 @@ class NeighborList:
                 self._cpp.set_tiled(tiled)
                 self._tiled = tiled
             self._pushed_for = run_id
-        self._cpp.update(self._sim.state.particles(), force_rebuild=getattr(self, "_force_next", False))
+        return True
+
+    def _take_force_flag(self):
+        f = getattr(self, "_force_next", False)
         self._force_next = False
+        return f
+
+    def _update(self, timestep):
+        if not self._needs_update(timestep):
+            return
+        self._cpp.update(self._sim.state.particles(), force_rebuild=self._take_force_flag())
         self._last = timestep
 
     @property

@@ -128,13 +128,19 @@ class Pair(Force):
         """Hook for pair forces with a thermostat (DPD)."""
 
     def _compute(self, timestep, virial=None):
-        self.nlist._update(timestep)
         self._thermostat(timestep)
         # the virial only when something reads it (HOOMD's pressure_tensor flag)
         virial = self._sim._needs_virial() if virial is None else virial
-        self._cpp.compute(
-            self.nlist.handle, self._sim.state.particles(), self._outputs(), timestep, compute_virial=virial
-        )
+        nl = self.nlist
+        if nl._needs_update(timestep):
+            # the rebuild check and the force in one library call: the force
+            # launches as soon as the check's flag arrives
+            self._cpp.update_compute(nl.handle, self._sim.state.particles(), self._outputs(), timestep,
+                                     compute_virial=virial, force_rebuild=nl._take_force_flag())
+            nl._last = timestep
+        else:
+            self._cpp.compute(nl.handle, self._sim.state.particles(), self._outputs(), timestep,
+                              compute_virial=virial)
         self._virial_current = bool(virial)
 
 

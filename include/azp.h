@@ -467,6 +467,19 @@ int azp_pair_compute(azp_pair* pp,
                      int compute_virial,
                      void* stream);
 
+/* azp_nlist_update followed by azp_pair_compute in one call (this library's
+ * own addition): the force launches as soon as the rebuild check's flag
+ * arrives, without a return to the caller in between. */
+int azp_pair_update_compute(azp_pair* pp,
+                            azp_nlist* nl,
+                            const azp_particles* p,
+                            const azp_outputs* out,
+                            uint64_t timestep,
+                            int compute_virial,
+                            int force_rebuild,
+                            int* rebuilt,
+                            void* stream);
+
 /* ---- batched replicas (no single reference counterpart: R independent
  * AnisoPotentialPair<AnisoPairEvaluatorTwoPatchMorse> systems,
  * AnisoPotentialPairGPUKernel.cu.inc:21-25, each with its own periodic box,
