@@ -13,6 +13,7 @@
 //       E = E(r_extrap) + f(r_extrap) (r_extrap - s)
 // with the energy shifted to zero at r_cut, the whole energy on the particle
 // and the virial 1/2 (s n) x F.
+#include <atomic>
 #include <cmath>
 #include <vector>
 
@@ -208,6 +209,9 @@ __global__ void __launch_bounds__(256) k_wall(const double4* __restrict__ pos,
 
 using namespace azp;
 
+// Value the type-error flag never takes (it holds bit 1 only).
+constexpr uint32_t kWallFlagPending = 0xa5a5a5a5u;
+
 struct azp_wall
     {
     int kind = kWallLJ93;
@@ -221,6 +225,7 @@ struct azp_wall
     DeviceArray<double> d_params, d_geom;
     DeviceArray<uint32_t> d_flags;
     uint32_t* h_flags = nullptr;
+    bool check_pending = false; // a type-error flag copy was queued by the last compute
 
     ~azp_wall()
         {
@@ -406,6 +411,19 @@ extern "C" int azp_wall_compute(azp_wall* w, const azp_particles* p, const azp_o
     if (compute_virial && out->virial_pitch < N)
         return set_error(AZP_ERR_INVALID, "virial pitch %u is smaller than N %u", out->virial_pitch, N);
     hipStream_t s = (hipStream_t)stream;
+    // The type-error flag of the previous call, read without waiting: its
+    // copy has landed once the stream passed it (a sentinel until then).
+    if (w->check_pending)
+        {
+        const uint32_t v = *reinterpret_cast<volatile uint32_t*>(w->h_flags);
+        if (v != kWallFlagPending)
+            {
+            w->check_pending = false;
+            if (v & 1u)
+                return set_error(AZP_ERR_INVALID,
+                                 "a particle type index is outside [0, %u) (pos.w must hold the type)", w->ntypes);
+            }
+        }
     if (!w->h_flags && hipHostMalloc((void**)&w->h_flags, sizeof(uint32_t), hipHostMallocCoherent) != hipSuccess)
         {
         w->h_flags = nullptr;
@@ -437,10 +455,12 @@ extern "C" int azp_wall_compute(azp_wall* w, const azp_particles* p, const azp_o
     else
         launch(k_wall<kWallColloid>);
     AZP_LAUNCH_CHECK();
-    AZP_HIP(hipMemcpyAsync(w->h_flags, w->d_flags.ptr, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    AZP_HIP(hipStreamSynchronize(s));
-    if (*w->h_flags & 1u)
-        return set_error(AZP_ERR_INVALID, "a particle type index is outside [0, %u) (pos.w must hold the type)",
-                         w->ntypes);
+    if (!w->check_pending)
+        {
+        *reinterpret_cast<volatile uint32_t*>(w->h_flags) = kWallFlagPending;
+        std::atomic_thread_fence(std::memory_order_seq_cst);
+        AZP_HIP(hipMemcpyAsync(w->h_flags, w->d_flags.ptr, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        w->check_pending = true;
+        }
     return AZP_OK;
     }

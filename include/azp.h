@@ -428,8 +428,10 @@ int azp_gather_rows(const void* d_src, void* d_dst, const uint32_t* d_order, uin
  * axis x y z, radius, inside, open}, plane {origin x y z, normal x y z,
  * open}; inside / open as 0 or 1 (HOOMD's hoomd.wall attributes).  compute
  * writes force.xyz, the energy (shifted to zero at r_cut, all of it on the
- * particle) in force.w and, when asked, the virial 1/2 (s n) x F; it
- * synchronizes the stream (it reads a type-error flag). ---- */
+ * particle) in force.w and, when asked, the virial 1/2 (s n) x F.  It does
+ * not wait for the stream: a particle type outside [0, ntypes) is reported
+ * by a later call, once the flag that the kernel raised has reached the
+ * host. ---- */
 typedef struct azp_wall azp_wall;
 int azp_wall_create(azp_wall** out, const char* cpp_class_name, uint32_t ntypes);
 int azp_wall_destroy(azp_wall* w);

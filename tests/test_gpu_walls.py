@@ -151,3 +151,24 @@ def test_unset_type_parameters_fail():
     sim.operations.integrator = integ
     with pytest.raises(RuntimeError):
         sim.run(0)
+
+
+def test_bad_type_is_reported_by_a_later_call():
+    """The wall compute does not wait for the stream; a type index outside the
+    table is reported once the kernel's flag reaches the host."""
+    from azplugins_amd import _azp, _azplugins
+
+    torch = _azp.torch
+    pos4 = np.array([[0.0, 0.0, 1.0, 0.0], [0.0, 0.0, 1.5, 3.0]])
+    w = _azplugins.WallsPotentialLJ93GPU(1)
+    w.setParams(0, [1.0, 1.0, 3.0, 0.0])
+    w.setGeometry([], [], [[0, 0, 0, 0, 0, 1, 1]])
+    sysm = GpuSystem(pos4, np.array([10.0, 10.0, 10.0]))
+    force = torch.zeros((2, 4), dtype=torch.float64, device="cuda")
+    out = _azp.Outputs()
+    out.d_force, out.d_virial, out.virial_pitch, out.d_torque = force.data_ptr(), None, 64, None
+    w.compute(sysm.particles(), out, compute_virial=False)
+    torch.cuda.synchronize()
+    with pytest.raises(_azp.AzpError):
+        w.compute(sysm.particles(), out, compute_virial=False)
+    assert force[1].abs().sum().item() == 0.0 and force[0, 2].item() != 0.0
