@@ -34,6 +34,6 @@ def use_as_hoomd():
     sys.modules["hoomd.azplugins.pair"] = pair
     sys.modules["hoomd.azplugins.wall"] = wall
     for name in ("md", "md.external", "md.nlist", "md.methods", "md.compute", "md.tune", "wall", "tune",
-                 "trigger", "filter", "variant"):
+                 "trigger", "filter", "variant", "write"):
         sys.modules["hoomd." + name] = importlib.import_module(harness.__name__ + "." + name)
     return harness

@@ -6,7 +6,8 @@ provides the subset the pair-force path and its tests use — ``Snapshot``,
 ``seed``, ``md.nlist.Cell``, ``md.Integrator``, ``md.methods.ConstantVolume``
 and ``md.methods.Langevin``, ``md.compute.ThermodynamicQuantities``,
 ``tune.ParticleSorter``, ``trigger.Periodic``, ``wall`` (Sphere,
-Cylinder, Plane), ``md.external.wall.WallPotential``, ``filter.All`` and
+Cylinder, Plane), ``md.external.wall.WallPotential``, ``write.GSD`` and
+``create_state_from_gsd`` (GSD files, module ``gsd``), ``filter.All`` and
 ``variant.Constant``.  Particle data lives in HBM as torch
 tensors (device memory only; all arithmetic runs in libazp.so kernels).
 
@@ -20,7 +21,7 @@ import warnings
 
 import numpy as np
 
-from . import filter, trigger, tune, variant, wall  # noqa: F401
+from . import filter, gsd, trigger, tune, variant, wall, write  # noqa: F401
 from .state import Box, Snapshot, State  # noqa: F401
 
 
@@ -50,6 +51,7 @@ class Operations:
         self._integrator = None
         self.computes = []
         self.tuners = [tune.ParticleSorter()]
+        self.writers = []
 
     @property
     def integrator(self):
@@ -89,6 +91,16 @@ class Simulation:
             raise RuntimeError("state already created")
         self.state = State(self, snapshot)
         return self.state
+
+    def create_state_from_gsd(self, filename, frame=-1):
+        """The state from one frame of a GSD file; the timestep becomes the
+        frame's step, so counter-based random forces continue."""
+        if self.state is not None:
+            raise RuntimeError("state already created")
+        with gsd.open(filename, "r") as traj:
+            fr = traj[frame]
+        self.timestep = int(fr.configuration.step)
+        return self.create_state_from_snapshot(Snapshot.from_gsd_frame(fr))
 
     def _needs_virial(self):
         """Whether force computes fill the virial: a thermodynamic compute is
@@ -131,20 +143,26 @@ class Simulation:
             if integ is not None:
                 integ._prepare_run(self.timestep)
             tuners = list(self.operations.tuners)
-            for t in tuners:
+            writers = list(self.operations.writers)
+            for t in tuners + writers:
                 if t._sim is not self:
                     t._attach(self)
             n = int(steps)
             for k in range(n):
                 for t in tuners:
                     t._act(self.timestep)
+                nxt = self.timestep + 1
                 if integ is not None:
                     # the next step's first half step may join this step's
-                    # second half when no tuner acts in between
-                    nxt = self.timestep + 1
-                    fuse = k + 1 < n and not any(t.trigger(nxt) for t in tuners)
+                    # second half when no tuner or writer acts in between
+                    fuse = k + 1 < n and not any(t.trigger(nxt) for t in tuners + writers)
                     integ._update(self.timestep, fuse_next=fuse)
-                self.timestep += 1
+                self.timestep = nxt
+                for w in writers:
+                    if w.trigger(nxt):
+                        w._act(nxt)
+            for w in writers:
+                w.flush()
         finally:
             self.state._thaw_view()
 

@@ -56,6 +56,7 @@ class _Configuration:
     def __init__(self):
         self.box = [1.0, 1.0, 1.0, 0.0, 0.0, 0.0]
         self.dimensions = 3
+        self.step = 0
 
 
 class _Particles:
@@ -70,6 +71,7 @@ class _Particles:
         "image": (3, np.int32, 0),
         "diameter": (None, np.float64, 1.0),
         "charge": (None, np.float64, 0.0),
+        "body": (None, np.int32, -1),
     }
 
     def __init__(self):
@@ -95,6 +97,10 @@ class _Particles:
             setattr(self, name, a)
 
 
+_GSD_FIELDS = ("position", "velocity", "mass", "typeid", "orientation", "moment_inertia", "angmom", "image",
+               "diameter", "charge", "body")
+
+
 class Snapshot:
     """Host-side system description (HOOMD's Snapshot layout)."""
 
@@ -102,6 +108,38 @@ class Snapshot:
         self.configuration = _Configuration()
         self.particles = _Particles()
         self.communicator = communicator if communicator is not None else _Comm()
+
+    @classmethod
+    def from_gsd_frame(cls, frame, communicator=None):
+        """A snapshot holding one frame of a GSD file (hoomd schema)."""
+        frame.validate()
+        snap = cls(communicator)
+        c, p = frame.configuration, frame.particles
+        snap.configuration.box = [float(v) for v in c.box]
+        snap.configuration.dimensions = int(c.dimensions)
+        snap.configuration.step = int(c.step)
+        snap.particles.types = list(p.types)
+        snap.particles.N = p.N
+        sp = snap.particles
+        for name in _GSD_FIELDS:
+            getattr(sp, name)[:] = getattr(p, name)
+        return snap
+
+    def _gsd_frame(self, step=0):
+        """This snapshot as a GSD frame (hoomd schema, 32-bit fields)."""
+        from .gsd import Frame
+
+        fr = Frame()
+        c, p = fr.configuration, fr.particles
+        c.step = int(step)
+        c.dimensions = int(self.configuration.dimensions)
+        c.box = self.configuration.box
+        sp = self.particles
+        p.N = sp.N
+        p.types = list(sp.types)
+        for name in _GSD_FIELDS:
+            setattr(p, name, getattr(sp, name))
+        return fr.validate()
 
 
 class _Comm:
@@ -153,6 +191,10 @@ class State:
         self.image = torch.tensor(img, device=dev)
         self.accel = torch.zeros((N, 4), dtype=torch.float64, device=dev)
         self.pos_fixed = None
+        # per-tag fields the force path does not read, kept for snapshots
+        self._diameter = np.array(p.diameter, dtype=np.float64).reshape(N)
+        self._charge = np.array(p.charge, dtype=np.float64).reshape(N)
+        self._body = np.array(p.body, dtype=np.int32).reshape(N)
 
     # ---- particle order (a sorter moves particles; tags travel with them) ----
     def _scratch(self, nbytes):
@@ -256,4 +298,7 @@ class State:
         ang[:, 1:] = bt(self.angmom.cpu().numpy())[:, :3]
         snap.particles.angmom[:] = ang
         snap.particles.moment_inertia[:] = bt(self.inertia.cpu().numpy())[:, None]
+        snap.particles.diameter[:] = self._diameter
+        snap.particles.charge[:] = self._charge
+        snap.particles.body[:] = self._body
         return snap
