@@ -38,96 +38,157 @@ struct DPDKernelArgs
     double4* force;
     double* virial;
     uint32_t vpitch;
+    uint32_t queue_len; // in-cut neighbor indices per lane held in LDS
     };
 
+// One type pair's parameters with the quantities that depend on the pair
+// type alone (1/r_cut, r_cut, s/2, the random prefactor) evaluated once per
+// launch, by the same operations the evaluator applies per pair.
 struct DPDPair
     {
-    double A, gamma, s, rcutsq;
+    double A, gamma, half_s, rcutsq, rcutinv, rcut, rand_pre;
     };
 
-// DPDPairEvaluatorGeneralWeight::evalForceEnergyThermo (:198-255).
-__device__ __forceinline__ bool dpd_eval(const DPDPair& q,
+// DPDPairEvaluatorGeneralWeight::evalForceEnergyThermo (:198-255) for a pair
+// already known to be inside r_cut.
+__device__ __forceinline__ void dpd_eval(const DPDPair& q,
                                          double rsq,
                                          uint16_t seed,
                                          uint32_t ti,
                                          uint32_t tj,
                                          uint32_t timestep,
-                                         double dt,
-                                         double kT,
                                          double rdotv,
                                          double& fdivr,
                                          double& fdivr_cons,
                                          double& e)
     {
-    if (!(rsq < q.rcutsq))
-        return false;
     const double rinv = 1.0 / sqrt(rsq);
     const double r = 1.0 / rinv;
-    const double rcutinv = 1.0 / sqrt(q.rcutsq);
-    const double rcut = 1.0 / rcutinv;
     const double alpha = dpd_alpha(seed, timestep, ti, tj);
-    fdivr = q.A * (rinv - rcutinv);
+    fdivr = q.A * (rinv - q.rcutinv);
     fdivr_cons = fdivr;
-    const double base = 1.0 - r * rcutinv;
-    const double wR = azp_ppow(base, 0.5 * q.s) * rinv;
+    const double base = 1.0 - r * q.rcutinv;
+    const double wR = azp_ppow(base, q.half_s) * rinv;
     fdivr -= q.gamma * wR * wR * rdotv;
-    fdivr += (1.0 / sqrt(dt / (kT * q.gamma * 6.0))) * wR * alpha;
-    e = q.A * (rcut - r) - 0.5 * q.A * rcutinv * (q.rcutsq - rsq);
-    return true;
+    fdivr += q.rand_pre * wR * alpha;
+    e = q.A * (q.rcut - r) - 0.5 * q.A * q.rcutinv * (q.rcutsq - rsq);
     }
 
+// One thread per particle over its full-list row.  Only about a third of
+// the entries lie inside r_cut (the list radius is r_cut + r_buff), while a
+// wave would pay the full evaluation on every entry that any of its lanes
+// has in cut.  So the kernel alternates two phases.  A scan reads the row
+// four entries at a time (four position gathers in flight per lane) and
+// appends the indices of the entries inside r_cut to the lane's queue in LDS
+// (column tid of a [queue_len][blockDim] array).  An evaluation pass then
+// takes the queued neighbors in order; the wave evaluates as many pairs as
+// the lane with the most in-cut entries has, not as many as its longest
+// row.  Each particle still sums its pair terms in row order.
 template<bool VIRIAL>
 __global__ void __launch_bounds__(512) k_dpd(const DPDKernelArgs a)
     {
     extern __shared__ double smem[];
     DPDPair* s_p = reinterpret_cast<DPDPair*>(smem);
     const uint32_t nt = a.ntypes;
+    uint32_t* s_q = reinterpret_cast<uint32_t*>(s_p + nt * nt);
     for (uint32_t t = threadIdx.x; t < nt * nt; t += blockDim.x)
         {
-        s_p[t].A = a.params[t].A;
-        s_p[t].gamma = a.params[t].gamma;
-        s_p[t].s = a.params[t].s;
-        s_p[t].rcutsq = a.rcutsq[t];
+        DPDPair q;
+        q.A = a.params[t].A;
+        q.gamma = a.params[t].gamma;
+        q.half_s = 0.5 * a.params[t].s;
+        q.rcutsq = a.rcutsq[t];
+        q.rcutinv = 1.0 / sqrt(q.rcutsq);
+        q.rcut = 1.0 / q.rcutinv;
+        q.rand_pre = 1.0 / sqrt(a.dt / (a.kT * q.gamma * 6.0));
+        s_p[t] = q;
         }
     __syncthreads();
     const uint32_t i = xcd_logical_block() * blockDim.x + threadIdx.x;
     if (i >= a.N)
         return;
+    const uint32_t bs = blockDim.x;
+    uint32_t* q_col = s_q + threadIdx.x;
+    const uint32_t qlen = a.queue_len;
     const double4 pi = a.pos[i];
     const double4 vi = a.vel[i];
     const uint32_t tagi = a.tag[i];
     const uint32_t ti = (uint32_t)pi.w;
+    const double* rc_row = &s_p[ti * nt].rcutsq;
+    const uint32_t* row = a.nlist + i;
+    const size_t pitch = a.nlist_pitch;
     double fx = 0, fy = 0, fz = 0, pe = 0;
     double v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0, v5 = 0;
     const uint32_t nn = a.n_neigh[i];
-    for (uint32_t k = 0; k < nn; ++k)
-        {
-        const uint32_t j = a.nlist[(size_t)k * a.nlist_pitch + i];
-        const double4 pj = a.pos[j];
+    auto in_cut = [&](const double4& pj) {
         const double dx = min_image_1d(pi.x - pj.x, a.box.L[0], a.box.Linv[0]);
         const double dy = min_image_1d(pi.y - pj.y, a.box.L[1], a.box.Linv[1]);
         const double dz = min_image_1d(pi.z - pj.z, a.box.L[2], a.box.Linv[2]);
         const double rsq = dx * dx + dy * dy + dz * dz;
-        const DPDPair q = s_p[ti * nt + (uint32_t)pj.w];
-        if (!(rsq < q.rcutsq))
-            continue;
-        const double4 vj = a.vel[j];
-        const double rdotv = dx * (vi.x - vj.x) + dy * (vi.y - vj.y) + dz * (vi.z - vj.z);
-        double fdivr, fcons, e;
-        dpd_eval(q, rsq, a.seed, tagi, a.tag[j], a.timestep, a.dt, a.kT, rdotv, fdivr, fcons, e);
-        fx += dx * fdivr;
-        fy += dy * fdivr;
-        fz += dz * fdivr;
-        pe += e * 0.5;
-        if (VIRIAL)
+        return rsq < rc_row[(uint32_t)pj.w * (sizeof(DPDPair) / sizeof(double))];
+    };
+    uint32_t k = 0;
+    while (k < nn)
+        {
+        // scan: fill the queue with the next in-cut neighbors
+        uint32_t cnt = 0;
+        while (k + 4 <= nn && cnt + 4 <= qlen)
             {
-            const double fd2 = fcons * 0.5;
-            v0 += fd2 * dx * dx;
-            v1 += fd2 * dx * dy;
-            v2 += fd2 * dx * dz;
-            v3 += fd2 * dy * dy;
-            v4 += fd2 * dy * dz;
-            v5 += fd2 * dz * dz;
+            const uint32_t j0 = row[(size_t)k * pitch];
+            const uint32_t j1 = row[(size_t)(k + 1) * pitch];
+            const uint32_t j2 = row[(size_t)(k + 2) * pitch];
+            const uint32_t j3 = row[(size_t)(k + 3) * pitch];
+            const double4 p0 = a.pos[j0];
+            const double4 p1 = a.pos[j1];
+            const double4 p2 = a.pos[j2];
+            const double4 p3 = a.pos[j3];
+            const bool c0 = in_cut(p0), c1 = in_cut(p1), c2 = in_cut(p2), c3 = in_cut(p3);
+            if (c0)
+                q_col[(cnt++) * bs] = j0;
+            if (c1)
+                q_col[(cnt++) * bs] = j1;
+            if (c2)
+                q_col[(cnt++) * bs] = j2;
+            if (c3)
+                q_col[(cnt++) * bs] = j3;
+            k += 4;
+            }
+        while (k < nn && cnt < qlen)
+            {
+            const uint32_t j = row[(size_t)k * pitch];
+            ++k;
+            if (in_cut(a.pos[j]))
+                q_col[(cnt++) * bs] = j;
+            }
+        // evaluate the queued pairs in row order
+        for (uint32_t c = 0; c < cnt; ++c)
+            {
+            const uint32_t j = q_col[c * bs];
+            const double4 pj = a.pos[j];
+            const double4 vj = a.vel[j];
+            const uint32_t tagj = a.tag[j];
+            const double dx = min_image_1d(pi.x - pj.x, a.box.L[0], a.box.Linv[0]);
+            const double dy = min_image_1d(pi.y - pj.y, a.box.L[1], a.box.Linv[1]);
+            const double dz = min_image_1d(pi.z - pj.z, a.box.L[2], a.box.Linv[2]);
+            const double rsq = dx * dx + dy * dy + dz * dz;
+            const DPDPair q = s_p[ti * nt + (uint32_t)pj.w];
+            const double rdotv = dx * (vi.x - vj.x) + dy * (vi.y - vj.y) + dz * (vi.z - vj.z);
+            double fdivr, fcons, e;
+            dpd_eval(q, rsq, a.seed, tagi, tagj, a.timestep, rdotv, fdivr, fcons, e);
+            fx += dx * fdivr;
+            fy += dy * fdivr;
+            fz += dz * fdivr;
+            pe += e * 0.5;
+            if (VIRIAL)
+                {
+                const double fd2 = fcons * 0.5;
+                v0 += fd2 * dx * dx;
+                v1 += fd2 * dx * dy;
+                v2 += fd2 * dx * dz;
+                v3 += fd2 * dy * dy;
+                v4 += fd2 * dy * dz;
+                v5 += fd2 * dz * dz;
+                }
             }
         }
     a.force[i] = make_double4(fx, fy, fz, pe);
@@ -192,9 +253,15 @@ extern "C" int azp_gpu_compute_dpd_forces(const azp_pair_args* args, const void*
     ka.virial = args->out.d_virial;
     ka.vpitch = args->out.virial_pitch;
     const unsigned block = args->block_size ? args->block_size : 256;
-    const size_t shmem = (size_t)args->ntypes * args->ntypes * sizeof(DPDPair);
-    if (shmem > 64 * 1024)
+    const size_t table = (size_t)args->ntypes * args->ntypes * sizeof(DPDPair);
+    if (table > 16 * 1024)
         return set_error(AZP_ERR_UNSUPPORTED, "too many particle types for LDS staging");
+    // queue of 32 indices per lane, fewer for blocks above 256 threads
+    // (queue and table stay within 64 KiB of LDS)
+    ka.queue_len = std::min<uint32_t>(32u, (uint32_t)((48 * 1024) / (block * sizeof(uint32_t))));
+    if (ka.queue_len < 4)
+        return set_error(AZP_ERR_INVALID, "DPD block size too large");
+    const size_t shmem = table + (size_t)ka.queue_len * block * sizeof(uint32_t);
     hipStream_t s = (hipStream_t)args->stream;
     if (args->compute_virial)
         hipLaunchKernelGGL(k_dpd<true>, dim3(grid_for(p.N, block)), dim3(block), shmem, s, ka);
