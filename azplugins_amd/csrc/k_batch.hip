@@ -125,30 +125,64 @@ __global__ void k_b_scatter(uint32_t N,
     members[start[cell_of[i]] + slot_of[i]] = i;
     }
 
-// Insertion sort of each cell's members by particle index (deterministic order).
-__global__ void k_b_sort_cells(uint32_t ncell,
-                               const uint32_t* __restrict__ cell_rep,
-                               const uint32_t* __restrict__ rep_flag,
-                               const uint32_t* __restrict__ start,
-                               uint32_t* __restrict__ members,
-                               const uint32_t* gate)
+// Each cell's members in increasing particle index (a deterministic order).
+// One wavefront per cell: a member's place is the number of the cell's
+// members with a smaller index (indices are distinct), so after copying the
+// cell to LDS every lane writes its members straight to their places.  A
+// cell with more members than the copy holds is sorted by the wavefront's
+// first lane (insertion sort).  Every exit is uniform over the wavefront and
+// lanes exchange data only through their wavefront's LDS row.
+constexpr uint32_t kSortWaves = 4;
+constexpr uint32_t kSortCap = 256;
+
+__global__ void __launch_bounds__(256) k_b_sort_cells(uint32_t ncell,
+                                                      const uint32_t* __restrict__ cell_rep,
+                                                      const uint32_t* __restrict__ rep_flag,
+                                                      const uint32_t* __restrict__ start,
+                                                      uint32_t* __restrict__ members,
+                                                      const uint32_t* gate)
     {
+    __shared__ uint32_t s_m[kSortWaves][kSortCap];
     if (gated_off(gate))
         return;
-    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t w = threadIdx.x >> 6;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t c = blockIdx.x * kSortWaves + w;
     if (c >= ncell || !rep_flag[cell_rep[c]])
         return;
-    const uint32_t b = start[c], e = start[c + 1];
-    for (uint32_t k = b + 1; k < e; ++k)
+    const uint32_t b = start[c];
+    const uint32_t n = start[c + 1] - b;
+    if (n < 2)
+        return;
+    if (n > kSortCap)
         {
-        const uint32_t v = members[k];
-        uint32_t m = k;
-        while (m > b && members[m - 1] > v)
-            {
-            members[m] = members[m - 1];
-            --m;
-            }
-        members[m] = v;
+        if (lane == 0)
+            for (uint32_t k = b + 1; k < b + n; ++k)
+                {
+                const uint32_t v = members[k];
+                uint32_t m = k;
+                while (m > b && members[m - 1] > v)
+                    {
+                    members[m] = members[m - 1];
+                    --m;
+                    }
+                members[m] = v;
+                }
+        return;
+        }
+    uint32_t* sm = s_m[w];
+    for (uint32_t k = lane; k < n; k += 64)
+        sm[k] = members[b + k];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (uint32_t k = lane; k < n; k += 64)
+        {
+        const uint32_t v = sm[k];
+        uint32_t rank = 0;
+        for (uint32_t t = 0; t < n; ++t)
+            rank += sm[t] < v ? 1u : 0u;
+        members[b + rank] = v;
         }
     }
 
@@ -174,7 +208,7 @@ struct BatchWalkArgs
 
 // Particle i's neighbors inside its own replica (same cell order, predicate
 // and rounding as the single-system walk).
-__device__ uint32_t b_walk_one(const BatchWalkArgs& a, uint32_t i)
+__device__ uint32_t b_walk_one(const BatchWalkArgs& a, uint32_t i, const double* __restrict__ r_listsq)
     {
     const double4 pi = a.pos[i];
     const uint32_t ti = checked_type(pi.w, a.ntypes);
@@ -185,6 +219,10 @@ __device__ uint32_t b_walk_one(const BatchWalkArgs& a, uint32_t i)
         return 0;
         }
     const RepInfo& R = a.reps[a.rep_of[i]];
+    // the box in registers: the list stores below may not be assumed apart
+    // from the replica table, which would otherwise be reloaded per candidate
+    const double L0 = R.L[0], L1 = R.L[1], L2 = R.L[2];
+    const double Li0 = R.Linv[0], Li1 = R.Linv[1], Li2 = R.Linv[2];
     const uint32_t lc = a.cell_of[i] - R.cell_base;
     const int cx = (int)(lc % (uint32_t)R.n[0]);
     const int cy = (int)((lc / (uint32_t)R.n[0]) % (uint32_t)R.n[1]);
@@ -194,34 +232,53 @@ __device__ uint32_t b_walk_one(const BatchWalkArgs& a, uint32_t i)
     const int ny = stencil_axis(cy, R.n[1], oy);
     const int nz = stencil_axis(cz, R.n[2], oz);
     uint32_t count = 0;
+    auto consider = [&](uint32_t j, const double4& pj) {
+        if (j == i)
+            return;
+        const uint32_t tj = checked_type(pj.w, a.ntypes);
+        if (tj == a.ntypes)
+            return;
+        const double rl = r_listsq[ti * a.ntypes + tj];
+        if (!(rl > 0.0))
+            return;
+        const double dx = min_image_1d(pi.x - pj.x, L0, Li0);
+        const double dy = min_image_1d(pi.y - pj.y, L1, Li1);
+        const double dz = min_image_1d(pi.z - pj.z, L2, Li2);
+        const double rsq = rsq3_nofma(dx, dy, dz);
+        if (rsq <= rl)
+            {
+            if (count < a.max_neighbors)
+                a.nlist[(size_t)count * a.pitch + i] = j;
+            ++count;
+            }
+    };
     for (int iz = 0; iz < nz; ++iz)
         for (int iy = 0; iy < ny; ++iy)
             for (int ix = 0; ix < nx; ++ix)
                 {
                 const uint32_t nb = R.cell_base + (uint32_t)((oz[iz] * R.n[1] + oy[iy]) * R.n[0] + ox[ix]);
                 const uint32_t b = a.start[nb], e = a.start[nb + 1];
-                for (uint32_t m = b; m < e; ++m)
+                // four candidates at a time: their member and position
+                // loads are in flight together; they are tested in order
+                uint32_t m = b;
+                for (; m + 4 <= e; m += 4)
+                    {
+                    uint32_t jj[4];
+                    double4 pp[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+                        jj[u] = a.members[m + u];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+                        pp[u] = a.pos[jj[u]];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+                        consider(jj[u], pp[u]);
+                    }
+                for (; m < e; ++m)
                     {
                     const uint32_t j = a.members[m];
-                    if (j == i)
-                        continue;
-                    const double4 pj = a.pos[j];
-                    const uint32_t tj = checked_type(pj.w, a.ntypes);
-                    if (tj == a.ntypes)
-                        continue;
-                    const double rl = a.r_listsq[ti * a.ntypes + tj];
-                    if (!(rl > 0.0))
-                        continue;
-                    const double dx = min_image_1d(pi.x - pj.x, R.L[0], R.Linv[0]);
-                    const double dy = min_image_1d(pi.y - pj.y, R.L[1], R.Linv[1]);
-                    const double dz = min_image_1d(pi.z - pj.z, R.L[2], R.Linv[2]);
-                    const double rsq = rsq3_nofma(dx, dy, dz);
-                    if (rsq <= rl)
-                        {
-                        if (count < a.max_neighbors)
-                            a.nlist[(size_t)count * a.pitch + i] = j;
-                        ++count;
-                        }
+                    consider(j, a.pos[j]);
                     }
                 }
     return count;
@@ -231,11 +288,21 @@ __device__ uint32_t b_walk_one(const BatchWalkArgs& a, uint32_t i)
 // index up to its longest list rounded up to 8 (as the single-system walk).
 __global__ void __launch_bounds__(256) k_b_walk(const BatchWalkArgs a)
     {
+    // the list radii (up to 8 types) in LDS
+    __shared__ double s_rl[64];
     if (gated_off(a.gate))
         return;
+    const uint32_t nt2 = a.ntypes * a.ntypes;
+    const bool staged = nt2 <= 64u;
+    if (staged)
+        {
+        for (uint32_t t = threadIdx.x; t < nt2; t += blockDim.x)
+            s_rl[t] = a.r_listsq[t];
+        __syncthreads();
+        }
     const uint32_t i = xcd_logical_block() * blockDim.x + threadIdx.x;
     const bool active = i < a.N && a.rep_flag[a.rep_of[i]] != 0u;
-    const uint32_t count = active ? b_walk_one(a, i) : 0u;
+    const uint32_t count = active ? b_walk_one(a, i, staged ? s_rl : a.r_listsq) : 0u;
     if (active)
         {
         // a list longer than the capacity is truncated (and reported), so
@@ -889,8 +956,8 @@ void enqueue_rebuild(azp_batch* b, hipStream_t s)
     enqueue_scan(b->cell_count.ptr, b->ncell, b->scan_sums.ptr, b->cell_start.ptr, gate, s);
     hipLaunchKernelGGL(k_b_scatter, dim3(grid_for(N, B)), dim3(B), 0, s, N, b->rep_of.ptr, rf, b->cell_of.ptr,
                        b->slot_of.ptr, b->cell_start.ptr, b->members.ptr, gate);
-    hipLaunchKernelGGL(k_b_sort_cells, dim3(grid_for(b->ncell, B)), dim3(B), 0, s, b->ncell, b->cell_rep.ptr, rf,
-                       b->cell_start.ptr, b->members.ptr, gate);
+    hipLaunchKernelGGL(k_b_sort_cells, dim3((b->ncell + kSortWaves - 1) / kSortWaves), dim3(64 * kSortWaves), 0, s,
+                       b->ncell, b->cell_rep.ptr, rf, b->cell_start.ptr, b->members.ptr, gate);
     BatchWalkArgs wa;
     wa.pos = pos;
     wa.N = N;
