@@ -292,7 +292,8 @@ def main():
     if not args.no_extras:
         extras = {}
         for name, fn in (("dpd_2m", bench_dpd), ("yukawa_262k_adjoint", bench_adjoint),
-                         ("tpm_batch_512", bench_batch), ("hertz_4096", bench_hertz)):
+                         ("tpm_batch_512", bench_batch), ("hertz_4096", bench_hertz),
+                         ("walls_1m", bench_walls), ("gsd_frame_1m", bench_gsd)):
             try:
                 extras[name] = fn(args)
             except Exception as exc:  # a failing extra must not hide the headline line
@@ -498,6 +499,77 @@ def bench_hertz(args):
     log(f"bench: hertz_4096 GPU NVE {1000 / dt:.1f} steps/s")
     return {"value": 1000 / dt, "unit": "timesteps/s", "steps": 1000, "N": N, "precision": "fp64",
             "method": "NVE velocity Verlet dt 0.005"}
+
+
+def bench_walls(args):
+    """LJ 9-3 walls on the 1M-particle PerturbedLJ configuration: two planes
+    and a sphere, forces and energies without the virial (FP64)."""
+    from azplugins_amd import _azp, _azplugins, configs
+
+    torch = _azp.torch
+    dev = torch.device("cuda:0")
+    c = configs.plj_1m()
+    pos4, L = c["pos4"], c["L"]
+    N = pos4.shape[0]
+    pos = torch.tensor(pos4, device=dev)
+    parts = _azp.Particles(N, _azp.Box(*L), pos.data_ptr(), None, None, None, None)
+    w = _azplugins.WallsPotentialLJ93GPU(1)
+    w.setParams(0, [1.0, 1.0, 3.0, 0.0])
+    h = 0.45 * float(L[2])
+    w.setGeometry(spheres=[[0.0, 0.0, 0.0, 0.45 * float(L[0]), 1.0, 0.0]], cylinders=[],
+                  planes=[[0.0, 0.0, -h, 0.0, 0.0, 1.0, 1.0], [0.0, 0.0, h, 0.0, 0.0, -1.0, 1.0]])
+    force = torch.zeros((N, 4), dtype=torch.float64, device=dev)
+    out = _azp.Outputs(force.data_ptr(), None, 0, None)
+    t = _events(torch, lambda: w.compute(parts, out, compute_virial=False), max(10, args.steps))
+    bytes_alg = N * (32 + 32)
+    res = {"value": N / t, "unit": "particle evaluations/s", "us_per_eval": t * 1e6, "N": N, "walls": 3,
+           "achieved_GBs": bytes_alg / t / 1e9, "algorithmic_bytes": bytes_alg,
+           "note": "algorithmic bytes: position 32 B read + force 32 B written per particle"}
+    log(f"bench: walls_1m {t * 1e6:.1f} us/eval, {bytes_alg / t / 1e9:.0f} GB/s")
+    return res
+
+
+def bench_gsd(args):
+    """One 1M-particle GSD frame: the state written (device to host and the
+    file) and a new state created from it."""
+    import tempfile
+
+    import azplugins_amd.harness as hoomd
+    from azplugins_amd import _azp, configs
+
+    torch = _azp.torch
+    c = configs.plj_1m()
+    pos4, L = c["pos4"], c["L"]
+    N = pos4.shape[0]
+    snap = hoomd.Snapshot()
+    snap.configuration.box = [*L, 0, 0, 0]
+    snap.particles.N = N
+    snap.particles.position[:] = pos4[:, :3]
+    snap.particles.velocity[:] = np.random.default_rng(5).normal(size=(N, 3))
+    sim = hoomd.Simulation(seed=1)
+    sim.create_state_from_snapshot(snap)
+    with tempfile.TemporaryDirectory() as d:
+        fn = os.path.join(d, "frame.gsd")
+        tw = []
+        for _ in range(3):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            hoomd.write.GSD.write(state=sim.state, filename=fn)
+            tw.append(time.perf_counter() - t0)
+        size = os.path.getsize(fn)
+        tr = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            s2 = hoomd.Simulation(seed=1)
+            s2.create_state_from_gsd(fn)
+            torch.cuda.synchronize()
+            tr.append(time.perf_counter() - t0)
+            del s2
+    w, r = float(np.median(tw)), float(np.median(tr))
+    log(f"bench: gsd_frame_1m write {w * 1e3:.1f} ms, create state {r * 1e3:.1f} ms ({size / 1e6:.1f} MB)")
+    return {"value": size / w / 1e6, "unit": "MB/s written", "write_ms": w * 1e3, "create_state_ms": r * 1e3,
+            "file_MB": size / 1e6, "N": N,
+            "note": "host path: device-to-host copy of the state, float32 schema fields, file write (median of 3)"}
 
 
 def run_tps(c, steps, precision="mixed"):
