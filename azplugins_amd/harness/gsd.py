@@ -408,18 +408,18 @@ class HOOMDTrajectory:
         later frame to those chunks; the step is always written."""
         first = len(self) == 0
         init = {} if first else self._initial_chunks()
-        ref = Frame()
-        ref.particles.N = frame.particles.N
-        defaults = dict(ref._chunks())
-        defaults["particles/N"] = np.zeros(1, dtype=np.uint32)
         for name, data in frame._chunks():
-            if not first and name != "configuration/step":
+            if name == "configuration/step":
+                self.file.write_chunk(name, data)
+                continue
+            if not first:
                 if only is not None and name not in only:
                     continue
-                same_as = init.get(name, defaults[name])
-            else:
-                same_as = defaults[name]
-            if data.size and (name == "configuration/step" or not _equal(data, same_as)):
+                if name in init:
+                    if not _equal(data, init[name]):
+                        self.file.write_chunk(name, data)
+                    continue
+            if data.size and not _is_default(name, data):
                 self.file.write_chunk(name, data)
         self.file.end_frame()
         if first:
@@ -436,6 +436,20 @@ class HOOMDTrajectory:
 
     def __exit__(self, *exc):
         self.close()
+
+
+def _is_default(name, data):
+    """Whether a chunk holds the schema's default (per row, for per-particle
+    chunks), compared without building a default array."""
+    if name == "particles/types":
+        return _types_list(data) == ["A"]
+    if name == "configuration/box":
+        return np.array_equal(data, _DEFAULT_BOX)
+    if name == "particles/N":
+        return int(data[0]) == 0
+    dt, w, d = SCHEMA[name]
+    row = np.asarray(d, dtype=dt)
+    return bool(np.all(data == row))
 
 
 def _equal(a, b):
