@@ -318,15 +318,30 @@ __global__ void __launch_bounds__(256) k_adjoint(const AdjArgs a)
         }
     }
 
-__global__ void k_adjoint_final(const double* __restrict__ partial, uint32_t nblocks, uint32_t nq, double* __restrict__ out)
+// Sum of the blocks' partial parameter gradients: one workgroup of 256
+// threads per gradient component q.  Thread t adds partials t, t + 256, ...
+// and the 256 sums are added in a fixed tree in LDS, so the result is the
+// same on every run.
+constexpr uint32_t kFinalThreads = 256;
+
+__global__ void __launch_bounds__(kFinalThreads)
+k_adjoint_final(const double* __restrict__ partial, uint32_t nblocks, uint32_t nq, double* __restrict__ out)
     {
-    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= nq)
-        return;
+    __shared__ double s_sum[kFinalThreads];
+    const uint32_t q = blockIdx.x;
     double s = 0.0;
-    for (uint32_t b = 0; b < nblocks; ++b)
+    for (uint32_t b = threadIdx.x; b < nblocks; b += kFinalThreads)
         s += partial[(size_t)b * nq + q];
-    out[q] = s;
+    s_sum[threadIdx.x] = s;
+    __syncthreads();
+    for (uint32_t w = kFinalThreads / 2; w > 0; w >>= 1)
+        {
+        if (threadIdx.x < w)
+            s_sum[threadIdx.x] += s_sum[threadIdx.x + w];
+        __syncthreads();
+        }
+    if (threadIdx.x == 0)
+        out[q] = s_sum[0];
     }
 
 template<class Adj, typename R>
@@ -404,7 +419,7 @@ extern "C" int azp_gpu_pair_adjoint(const azp_pair_args* args,
         st = mixed ? launch_adj<PLJAdj, float>(ka, nblocks, s) : launch_adj<PLJAdj, double>(ka, nblocks, s);
     if (st != AZP_OK)
         return st;
-    hipLaunchKernelGGL(k_adjoint_final, dim3((nq + 63) / 64), dim3(64), 0, s, d_work, nblocks, nq, d_grad_params);
+    hipLaunchKernelGGL(k_adjoint_final, dim3(nq), dim3(kFinalThreads), 0, s, d_work, nblocks, nq, d_grad_params);
     AZP_LAUNCH_CHECK();
     return AZP_OK;
     }
