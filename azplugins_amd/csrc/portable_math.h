@@ -6,8 +6,8 @@
  * fast::pow (DPDPairEvaluatorGeneralWeight.h:242), i.e. the platform pow, whose
  * last bits differ between libm and any GPU math library.  These routines are
  * built only from IEEE-754 operations that are correctly rounded everywhere
- * (+, -, *, /, fma, rint, frexp, ldexp), evaluated in a fixed order with no
- * contraction, so gcc on x86-64 and hipcc on gfx950 produce the same bits.
+ * (+, -, *, /, sqrt, fma, rint, frexp, ldexp), evaluated in a fixed order with
+ * no contraction, so gcc on x86-64 and hipcc on gfx950 produce the same bits.
  *
  * Methods (textbook, no vendor code):
  *   exp: Cody & Waite (1980) reduction x = k ln2 + r with ln2 split into a
@@ -15,7 +15,9 @@
  *        degree 13 for |r| <= ln2/2 evaluated by Horner's rule in fma.
  *   log: x = 2^e m with m in [sqrt(1/2), sqrt(2)); log m = 2 atanh(s),
  *        s = (m-1)/(m+1), summed as an odd series in s up to s^23.
- *   pow: exp(y log x) for x > 0; exact special cases for x == 0, y == 0.
+ *   pow: exp(y log x) for x > 0; exact special cases for x == 0, y == 0;
+ *        y = 1, 1/2, 1/4 as x, sqrt(x), sqrt(sqrt(x)) (the weight exponent s/2
+ *        of s = 2, 1, 1/2), each within one ulp of the exact power.
  * Accuracy: within a few ulp of a correctly rounded result for the arguments
  * the DPD weight uses (x in (0,1], y in [0, 4]).
  *
@@ -121,6 +123,12 @@ AZP_PM_FN double azp_ppow(double x, double y)
         return (y > 0.0) ? 0.0 : HUGE_VAL;
     if (x == 1.0)
         return 1.0;
+    if (y == 1.0)
+        return x;
+    if (y == 0.5)
+        return sqrt(x);
+    if (y == 0.25)
+        return sqrt(sqrt(x));
     return azp_pexp(y * azp_plog(x));
 }
 
