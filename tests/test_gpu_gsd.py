@@ -113,3 +113,47 @@ def test_write_static_and_create_state(tmp_path):
     assert np.array_equal(s2.typeid, snap.particles.typeid)
     for name in ("position", "mass", "diameter"):
         assert np.array_equal(getattr(s2, name), getattr(snap.particles, name).astype(np.float32)), name
+
+
+def test_writer_dynamic_categories_and_truncate(tmp_path):
+    """Later frames hold the step and the dynamic categories only; other
+    chunks read back from frame 0.  truncate=True keeps one frame."""
+    import azplugins_amd.harness as hoomd
+
+    L = 8.0
+    pos4, rng = jittered_lattice(8, L, 0.05, 1, seed=7)
+    snap = hoomd.Snapshot()
+    snap.configuration.box = [L, L, L, 0, 0, 0]
+    snap.particles.N = pos4.shape[0]
+    snap.particles.position[:] = pos4[:, :3]
+    snap.particles.velocity[:] = rng.normal(size=(pos4.shape[0], 3))
+    snap.particles.mass[:] = 2.0
+    sim = _dpd(snap)
+    fn = str(tmp_path / "dyn.gsd")
+    fn_t = str(tmp_path / "last.gsd")
+    w = hoomd.write.GSD(hoomd.trigger.Periodic(5), fn, mode="wb")  # dynamic: property only
+    wt = hoomd.write.GSD(hoomd.trigger.Periodic(5), fn_t, mode="wb", truncate=True,
+                         dynamic=["property", "momentum"])
+    sim.operations.writers += [w, wt]
+    say(f"[gsd] writer categories N={pos4.shape[0]}")
+    sim.run(15)
+    w.close()
+    wt.close()
+    last = sim.state.get_snapshot().particles
+    with hoomd.gsd.open(fn, "rb") as t:
+        assert len(t) == 3
+        f = t.file
+        assert f.chunk_exists(0, "particles/velocity") and f.chunk_exists(0, "particles/mass")
+        assert not f.chunk_exists(2, "particles/velocity") and not f.chunk_exists(2, "particles/mass")
+        assert f.chunk_exists(2, "particles/position")
+        fr0, fr2 = t[0], t[2]
+    assert fr2.configuration.step == 15
+    assert np.array_equal(fr2.particles.position, last.position.astype(np.float32))
+    # velocity is not a "property": frame 2 reads frame 0's
+    assert np.array_equal(fr2.particles.velocity, fr0.particles.velocity)
+    assert np.all(fr2.particles.mass == np.float32(2.0))
+    with hoomd.gsd.open(fn_t, "rb") as t:
+        assert len(t) == 1
+        only = t[0]
+    assert only.configuration.step == 15
+    assert np.array_equal(only.particles.velocity, last.velocity.astype(np.float32))
