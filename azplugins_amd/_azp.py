@@ -101,6 +101,7 @@ class PairArgs(ctypes.Structure):
         ("tile_slots", ctypes.c_uint32),
         ("d_tile_order", ctypes.c_void_p),
         ("d_tile_nn", ctypes.c_void_p),
+        ("d_head_list", ctypes.c_void_p),
     ]
 
 
@@ -200,6 +201,7 @@ SIGNATURES = [
     ("azp_wall_get_params", _i, [_vp, _u32, _pd]),
     ("azp_wall_set_geometry", _i, [_vp, _u32, _pd, _u32, _pd, _u32, _pd]),
     ("azp_wall_compute", _i, [_vp, ctypes.POINTER(Particles), ctypes.POINTER(Outputs), _i, _vp]),
+    ("azp_wall_check", _i, [_vp, _vp]),
     ("azp_pair_create", _i, [ctypes.POINTER(_vp), ctypes.c_char_p, _u32]),
     ("azp_pair_destroy", _i, [_vp]),
     ("azp_pair_set_params", _i, [_vp, _u32, _u32, _pd]),
@@ -235,7 +237,7 @@ SIGNATURES = [
 ]
 
 # must equal AZP_ABI_VERSION of include/azp.h (azp_pair_args layout)
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 _lib = None
 

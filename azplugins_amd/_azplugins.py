@@ -156,6 +156,10 @@ class WallsPotentialGPUBase:
         _azp.call("azp_wall_compute", self._h, ctypes.byref(particles), ctypes.byref(outputs),
                   1 if compute_virial else 0, _azp.stream_ptr(stream))
 
+    def check(self, stream=None):
+        """Waits for the stream and raises the type error a compute flagged."""
+        _azp.call("azp_wall_check", self._h, _azp.stream_ptr(stream))
+
 
 for _n in ("WallsPotentialLJ93", "WallsPotentialColloid"):
     for _suffix in ("", "GPU"):

@@ -14,6 +14,10 @@
 namespace azp
 {
 
+// Device buffer of at least N int4 owned by the launchers (one per device,
+// kept until the process ends): fixed-point positions they pack themselves.
+int launcher_heap_scratch(uint32_t N, int4** out);
+
 // Records a message for azp_last_error() and returns `code`.
 int set_error(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
 void clear_error();

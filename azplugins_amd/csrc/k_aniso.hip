@@ -24,6 +24,7 @@ struct AnisoKernelArgs
     const uint32_t* n_neigh;
     const uint32_t* nlist;
     uint32_t nlist_pitch;
+    const uint64_t* head_list; // HOOMD's layout when non-null
     const azp_param_two_patch_morse* params;
     const double* rcutsq;
     uint32_t ntypes;
@@ -70,7 +71,9 @@ __global__ void __launch_bounds__(512) k_aniso(const AnisoKernelArgs a)
     AnisoAcc acc;
     const uint32_t nn = i < a.N ? a.n_neigh[i] : 0u;
     const uint32_t trips = __builtin_amdgcn_readfirstlane((tpm_wave_max(nn) + 3u) & ~3u);
-    tpm_list<MIXED, VIRIAL, SINGLE, false>(ie, nn, trips, a.nlist, a.nlist_pitch, a.pos, a.pos_fixed, a.orientation,
+    const uint32_t* row = a.head_list ? a.nlist + a.head_list[ie] : a.nlist + ie;
+    const size_t stride = a.head_list ? 1 : a.nlist_pitch;
+    tpm_list<MIXED, VIRIAL, SINGLE, false>(ie, nn, trips, row, stride, a.pos, a.pos_fixed, a.orientation,
                                            nullptr, me, box, nt, q0, s_q, acc);
     if (i < a.N)
         aniso_store<VIRIAL>(a.force, a.torque, a.virial, a.vpitch, i, acc);
@@ -101,6 +104,15 @@ static int launch_aniso(const AnisoKernelArgs& ka, bool vir, unsigned block, hip
 
 int validate_common(const azp_pair_args* args);
 
+__global__ void k_pack_fixed_aniso(const double4* __restrict__ pos, uint32_t N, DevBox box, int4* __restrict__ out)
+    {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N)
+        return;
+    const double4 p = pos[i];
+    out[i] = make_int4(to_fixed(p.x, box.Linv[0]), to_fixed(p.y, box.Linv[1]), to_fixed(p.z, box.Linv[2]), (int)p.w);
+    }
+
 } // namespace azp
 
 using namespace azp;
@@ -116,8 +128,8 @@ extern "C" int azp_gpu_compute_pair_aniso_forces(const azp_pair_args* args, int 
     if (!d_params || !p.d_orientation || !args->out.d_torque)
         return set_error(AZP_ERR_INVALID, "aniso needs params, orientations and a torque array");
     const bool mixed = args->precision == AZP_PREC_MIXED;
-    if (mixed && !p.d_pos_fixed)
-        return set_error(AZP_ERR_INVALID, "mixed precision needs d_pos_fixed");
+    if (!args->d_nlist)
+        return set_error(AZP_ERR_INVALID, "the anisotropic kernel reads the plain list (d_nlist)");
     if (args->shift_mode == AZP_SHIFT_XPLOR)
         return set_error(AZP_ERR_UNSUPPORTED, "anisotropic pairs accept modes none and shift");
     AnisoKernelArgs ka;
@@ -131,6 +143,7 @@ extern "C" int azp_gpu_compute_pair_aniso_forces(const azp_pair_args* args, int 
     ka.n_neigh = args->d_n_neigh;
     ka.nlist = args->d_nlist;
     ka.nlist_pitch = args->nlist_pitch;
+    ka.head_list = args->d_head_list;
     ka.params = static_cast<const azp_param_two_patch_morse*>(d_params);
     ka.rcutsq = args->d_rcutsq;
     ka.ntypes = args->ntypes;
@@ -141,6 +154,15 @@ extern "C" int azp_gpu_compute_pair_aniso_forces(const azp_pair_args* args, int 
     ka.vpitch = args->out.virial_pitch;
     const unsigned block = args->block_size ? args->block_size : 256;
     hipStream_t s = (hipStream_t)args->stream;
+    if (mixed && !ka.pos_fixed)
+        {
+        // fixed-point positions packed into the launchers' scratch
+        int4* pf = nullptr;
+        AZP_TRY(launcher_heap_scratch(p.N, &pf));
+        hipLaunchKernelGGL(k_pack_fixed_aniso, dim3(grid_for(p.N, 256)), dim3(256), 0, s, ka.pos, p.N, ka.box, pf);
+        AZP_LAUNCH_CHECK();
+        ka.pos_fixed = pf;
+        }
     return mixed ? launch_aniso<true>(ka, args->compute_virial != 0, block, s)
                  : launch_aniso<false>(ka, args->compute_virial != 0, block, s);
     }

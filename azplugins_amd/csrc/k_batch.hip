@@ -478,7 +478,7 @@ __global__ void __launch_bounds__(512) k_b_forces(const BatchForceArgs a)
             AnisoAcc acc;
             const uint32_t nn = active ? a.n_neigh[i] : 0u;
             const uint32_t trips = __builtin_amdgcn_readfirstlane((tpm_wave_max(nn) + 3u) & ~3u);
-            tpm_list<MIXED, VIRIAL, SINGLE, MIXED>(ie, nn, trips, a.nlist, a.pitch, a.pos, a.pos_fixed, a.orientation,
+            tpm_list<MIXED, VIRIAL, SINGLE, MIXED>(ie, nn, trips, a.nlist + ie, a.pitch, a.pos, a.pos_fixed, a.orientation,
                                                    a.dirs, me, box, nt, q0, s_q, acc);
             if (active)
                 aniso_store<VIRIAL>(a.force, a.torque, a.virial, a.vpitch, i, acc);

@@ -28,6 +28,7 @@ struct DPDKernelArgs
     const uint32_t* n_neigh;
     const uint32_t* nlist;
     uint32_t nlist_pitch;
+    const uint64_t* head_list; // HOOMD's layout when non-null
     const azp_param_dpd_general_weight* params;
     const double* rcutsq;
     uint32_t ntypes;
@@ -84,24 +85,34 @@ __device__ __forceinline__ void dpd_eval(const DPDPair& q,
 // takes the queued neighbors in order; the wave evaluates as many pairs as
 // the lane with the most in-cut entries has, not as many as its longest
 // row.  Each particle still sums its pair terms in row order.
-template<bool VIRIAL>
+__device__ __forceinline__ DPDPair make_dpd_pair(const DPDKernelArgs& a, uint32_t t)
+    {
+    DPDPair q;
+    q.A = a.params[t].A;
+    q.gamma = a.params[t].gamma;
+    q.half_s = 0.5 * a.params[t].s;
+    q.rcutsq = a.rcutsq[t];
+    q.rcutinv = 1.0 / sqrt(q.rcutsq);
+    q.rcut = 1.0 / q.rcutinv;
+    q.rand_pre = 1.0 / sqrt(a.dt / (a.kT * q.gamma * 6.0));
+    return q;
+    }
+
+// LDS_TABLE: the type-pair table is staged in LDS (up to 16 KiB, 17 types);
+// otherwise each in-cut pair derives its entry from the global parameter
+// arrays by the same operations (bit-identical, slower).
+template<bool VIRIAL, bool LDS_TABLE>
 __global__ void __launch_bounds__(512) k_dpd(const DPDKernelArgs a)
     {
     extern __shared__ double smem[];
     DPDPair* s_p = reinterpret_cast<DPDPair*>(smem);
     const uint32_t nt = a.ntypes;
-    uint32_t* s_q = reinterpret_cast<uint32_t*>(s_p + nt * nt);
-    for (uint32_t t = threadIdx.x; t < nt * nt; t += blockDim.x)
+    uint32_t* s_q = reinterpret_cast<uint32_t*>(LDS_TABLE ? reinterpret_cast<char*>(s_p + nt * nt)
+                                                          : reinterpret_cast<char*>(smem));
+    if (LDS_TABLE)
         {
-        DPDPair q;
-        q.A = a.params[t].A;
-        q.gamma = a.params[t].gamma;
-        q.half_s = 0.5 * a.params[t].s;
-        q.rcutsq = a.rcutsq[t];
-        q.rcutinv = 1.0 / sqrt(q.rcutsq);
-        q.rcut = 1.0 / q.rcutinv;
-        q.rand_pre = 1.0 / sqrt(a.dt / (a.kT * q.gamma * 6.0));
-        s_p[t] = q;
+        for (uint32_t t = threadIdx.x; t < nt * nt; t += blockDim.x)
+            s_p[t] = make_dpd_pair(a, t);
         }
     __syncthreads();
     const uint32_t i = xcd_logical_block() * blockDim.x + threadIdx.x;
@@ -114,9 +125,10 @@ __global__ void __launch_bounds__(512) k_dpd(const DPDKernelArgs a)
     const double4 vi = a.vel[i];
     const uint32_t tagi = a.tag[i];
     const uint32_t ti = (uint32_t)pi.w;
-    const double* rc_row = &s_p[ti * nt].rcutsq;
-    const uint32_t* row = a.nlist + i;
-    const size_t pitch = a.nlist_pitch;
+    const double* rc_row = LDS_TABLE ? &s_p[ti * nt].rcutsq : a.rcutsq + ti * nt;
+    constexpr uint32_t rc_stride = LDS_TABLE ? sizeof(DPDPair) / sizeof(double) : 1u;
+    const uint32_t* row = a.head_list ? a.nlist + a.head_list[i] : a.nlist + i;
+    const size_t pitch = a.head_list ? 1 : a.nlist_pitch;
     double fx = 0, fy = 0, fz = 0, pe = 0;
     double v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0, v5 = 0;
     const uint32_t nn = a.n_neigh[i];
@@ -125,7 +137,7 @@ __global__ void __launch_bounds__(512) k_dpd(const DPDKernelArgs a)
         const double dy = min_image_1d(pi.y - pj.y, a.box.L[1], a.box.Linv[1]);
         const double dz = min_image_1d(pi.z - pj.z, a.box.L[2], a.box.Linv[2]);
         const double rsq = dx * dx + dy * dy + dz * dz;
-        return rsq < rc_row[(uint32_t)pj.w * (sizeof(DPDPair) / sizeof(double))];
+        return rsq < rc_row[(uint32_t)pj.w * rc_stride];
     };
     uint32_t k = 0;
     while (k < nn)
@@ -171,7 +183,7 @@ __global__ void __launch_bounds__(512) k_dpd(const DPDKernelArgs a)
             const double dy = min_image_1d(pi.y - pj.y, a.box.L[1], a.box.Linv[1]);
             const double dz = min_image_1d(pi.z - pj.z, a.box.L[2], a.box.Linv[2]);
             const double rsq = dx * dx + dy * dy + dz * dz;
-            const DPDPair q = s_p[ti * nt + (uint32_t)pj.w];
+            const DPDPair q = LDS_TABLE ? s_p[ti * nt + (uint32_t)pj.w] : make_dpd_pair(a, ti * nt + (uint32_t)pj.w);
             const double rdotv = dx * (vi.x - vj.x) + dy * (vi.y - vj.y) + dz * (vi.z - vj.z);
             double fdivr, fcons, e;
             dpd_eval(q, rsq, a.seed, tagi, tagj, a.timestep, rdotv, fdivr, fcons, e);
@@ -242,6 +254,9 @@ extern "C" int azp_gpu_compute_dpd_forces(const azp_pair_args* args, const void*
     ka.n_neigh = args->d_n_neigh;
     ka.nlist = args->d_nlist;
     ka.nlist_pitch = args->nlist_pitch;
+    ka.head_list = args->d_head_list;
+    if (!args->d_nlist)
+        return set_error(AZP_ERR_INVALID, "the DPD kernel reads the plain list (d_nlist)");
     ka.params = static_cast<const azp_param_dpd_general_weight*>(d_params);
     ka.rcutsq = args->d_rcutsq;
     ka.ntypes = args->ntypes;
@@ -253,9 +268,11 @@ extern "C" int azp_gpu_compute_dpd_forces(const azp_pair_args* args, const void*
     ka.virial = args->out.d_virial;
     ka.vpitch = args->out.virial_pitch;
     const unsigned block = args->block_size ? args->block_size : 256;
-    const size_t table = (size_t)args->ntypes * args->ntypes * sizeof(DPDPair);
-    if (table > 16 * 1024)
-        return set_error(AZP_ERR_UNSUPPORTED, "too many particle types for LDS staging");
+    // type-pair table in LDS up to 16 KiB (17 types); beyond that it is
+    // derived per pair from the global arrays
+    const size_t table_bytes = (size_t)args->ntypes * args->ntypes * sizeof(DPDPair);
+    const bool lds_table = table_bytes <= 16 * 1024;
+    const size_t table = lds_table ? table_bytes : 0;
     // queue of 32 indices per lane, fewer for blocks above 256 threads
     // (queue and table stay within 64 KiB of LDS)
     ka.queue_len = std::min<uint32_t>(32u, (uint32_t)((48 * 1024) / (block * sizeof(uint32_t))));
@@ -263,10 +280,21 @@ extern "C" int azp_gpu_compute_dpd_forces(const azp_pair_args* args, const void*
         return set_error(AZP_ERR_INVALID, "DPD block size too large");
     const size_t shmem = table + (size_t)ka.queue_len * block * sizeof(uint32_t);
     hipStream_t s = (hipStream_t)args->stream;
+    const dim3 grid(grid_for(p.N, block));
     if (args->compute_virial)
-        hipLaunchKernelGGL(k_dpd<true>, dim3(grid_for(p.N, block)), dim3(block), shmem, s, ka);
+        {
+        if (lds_table)
+            hipLaunchKernelGGL((k_dpd<true, true>), grid, dim3(block), shmem, s, ka);
+        else
+            hipLaunchKernelGGL((k_dpd<true, false>), grid, dim3(block), shmem, s, ka);
+        }
     else
-        hipLaunchKernelGGL(k_dpd<false>, dim3(grid_for(p.N, block)), dim3(block), shmem, s, ka);
+        {
+        if (lds_table)
+            hipLaunchKernelGGL((k_dpd<false, true>), grid, dim3(block), shmem, s, ka);
+        else
+            hipLaunchKernelGGL((k_dpd<false, false>), grid, dim3(block), shmem, s, ka);
+        }
     AZP_LAUNCH_CHECK();
     return AZP_OK;
     }

@@ -443,6 +443,12 @@ struct azp_nlist
 
 static int nlist_build(azp_nlist* nl, const azp_particles* p, hipStream_t s)
     {
+    // until this build succeeds there is no list: an error return leaves the
+    // list invalid, so the next update rebuilds it instead of comparing the
+    // displacement against the failed build's positions
+    nl->have_list = false;
+    nl->built_N = 0;
+    nl->tiles.valid = false;
     const uint32_t N = p->N;
     const DevBox box = make_box(p->box);
     const double rmax = nl->r_list_max();

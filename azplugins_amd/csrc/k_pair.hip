@@ -19,514 +19,14 @@
 //                  accumulators.  Pairs whose FP32 rsq lies within a relative
 //                  band of 1e-5 around rcutsq are re-tested on the FP64
 //                  positions, so the in/out cutoff decision is the FP64 one.
+#include <algorithm>
+#include <mutex>
 #include <type_traits>
 
-#include "evaluators.h"
-#include "tiles.h"
+#include "pair_kernel.h"
 
 namespace azp
 {
-
-constexpr float kCutBand = 1e-5f;
-// Mixed form: list entries summed in FP32 between additions to the FP64
-// accumulators (a multiple of the batch of 8).
-constexpr uint32_t kFlushEntries = 32;
-
-struct PairKernelArgs
-    {
-    const double4* pos;
-    const int4* pos_fixed;
-    uint32_t N;
-    DevBox box;
-    float fx_scale[3];    // L / 2^32 per axis
-    double fx_scale64[3]; // the same in FP64
-    float fx_scale2[3];   // (L / 2^32)^2 per axis
-    const uint32_t* n_neigh;
-    const uint32_t* nlist;
-    uint32_t nlist_pitch;
-    const void* params;
-    const double* rcutsq;
-    const double* ronsq;
-    uint32_t ntypes;
-    int shift_mode;
-    double4* force;
-    double* virial;
-    uint32_t vpitch;
-    // tiled list (mixed form): per-tile neighbor sets and 16-bit entries;
-    // for tiles of cell-list slots, column s is particle tile_order[s] with
-    // tile_nn[s] entries (both null for tiles of consecutive particles)
-    const uint32_t* tile_count;
-    const uint32_t* tile_jset;
-    const uint32_t* tile_list;
-    const uint32_t* tile_order;
-    const uint32_t* tile_nn;
-    uint32_t tile_slots; // LDS position slots of the tiled kernel (the tables follow them)
-    };
-
-// Position (double4 x, y, z, type) -> wrapped 32-bit fixed point of the box.
-__device__ __forceinline__ int4 fixed4(const double4& p, const DevBox& box)
-    {
-    return make_int4(to_fixed(p.x, box.Linv[0]), to_fixed(p.y, box.Linv[1]), to_fixed(p.z, box.Linv[2]), (int)p.w);
-    }
-
-template<typename R> struct TypePairExtra
-    {
-    double rcsq64;
-    R rcsq;
-    R rcsq_lo; // rcsq (1 - band)
-    R rcsq_hi; // rcsq (1 + band)
-    R ronsq;
-    };
-
-template<class Eval, typename R>
-__device__ __forceinline__ void prepare_pair(const PairKernelArgs& a,
-                                             uint32_t t,
-                                             typename Eval::template Prep<R>& q,
-                                             TypePairExtra<R>& x)
-    {
-    const auto* params = static_cast<const typename Eval::param_type*>(a.params);
-    const double rc = a.rcutsq[t];
-    const double ron = a.ronsq ? a.ronsq[t] : 0.0;
-    const bool shift
-        = a.shift_mode == AZP_SHIFT_SHIFT || (a.shift_mode == AZP_SHIFT_XPLOR && ron > rc);
-    q = Eval::template prepare<R>(params[t], rc, shift);
-    x.rcsq64 = rc;
-    x.rcsq = (R)rc;
-    x.rcsq_lo = (R)(rc * (1.0 - (double)kCutBand));
-    x.rcsq_hi = (R)(rc * (1.0 + (double)kCutBand));
-    x.ronsq = (R)ron;
-    }
-
-// xplor smoothing (upstream PotentialPair, restated in the oracle):
-// S = (rc2 - r2)^2 (rc2 + 2 r2 - 3 ron2) / (rc2 - ron2)^3 for ron2 <= r2 < rc2.
-template<typename R>
-__device__ __forceinline__ void apply_xplor(R rsq, R rcutsq, R ronsq, R& fdivr, R& e)
-    {
-    if (rsq >= ronsq && rsq < rcutsq)
-        {
-        const R old_e = e, old_f = fdivr;
-        const R denom = (rcutsq - ronsq) * (rcutsq - ronsq) * (rcutsq - ronsq);
-        const R denom_inv = f_rcp(denom);
-        const R d = rsq - rcutsq;
-        const R S = d * d * (rcutsq + R(2) * rsq - R(3) * ronsq) * denom_inv;
-        const R dS_dr_divr = R(12) * (rsq - ronsq) * d * denom_inv;
-        e = old_e * S;
-        fdivr = S * old_f - dS_dr_divr * old_e;
-        }
-    }
-
-struct Acc
-    {
-    double fx = 0, fy = 0, fz = 0, e = 0;
-    double v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0, v5 = 0;
-    };
-
-template<bool VIRIAL>
-__device__ __forceinline__ void accumulate(Acc& acc, double fd, double dx, double dy, double dz, double e)
-    {
-    const double px = fd * dx, py = fd * dy, pz = fd * dz;
-    acc.fx += px;
-    acc.fy += py;
-    acc.fz += pz;
-    acc.e += e;
-    if (VIRIAL)
-        {
-        acc.v0 = fma(px, dx, acc.v0);
-        acc.v1 = fma(px, dy, acc.v1);
-        acc.v2 = fma(px, dz, acc.v2);
-        acc.v3 = fma(py, dy, acc.v3);
-        acc.v4 = fma(py, dz, acc.v4);
-        acc.v5 = fma(pz, dz, acc.v5);
-        }
-    }
-
-// Mixed form: the terms of two batches of list entries are summed in FP32 (at
-// most 16 terms), then each such sum is added to the FP64 accumulators.
-struct Acc32
-    {
-    float fx = 0, fy = 0, fz = 0, e = 0;
-    float v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0, v5 = 0;
-    };
-
-template<bool VIRIAL>
-__device__ __forceinline__ void accumulate32(Acc32& acc, float fd, float dx, float dy, float dz, float e)
-    {
-    const float px = fd * dx, py = fd * dy, pz = fd * dz;
-    acc.fx += px;
-    acc.fy += py;
-    acc.fz += pz;
-    acc.e += e;
-    if (VIRIAL)
-        {
-        acc.v0 = fmaf(px, dx, acc.v0);
-        acc.v1 = fmaf(px, dy, acc.v1);
-        acc.v2 = fmaf(px, dz, acc.v2);
-        acc.v3 = fmaf(py, dy, acc.v3);
-        acc.v4 = fmaf(py, dz, acc.v4);
-        acc.v5 = fmaf(pz, dz, acc.v5);
-        }
-    }
-
-template<bool VIRIAL>
-__device__ __forceinline__ void flush32(Acc& acc, const Acc32& b)
-    {
-    acc.fx += (double)b.fx;
-    acc.fy += (double)b.fy;
-    acc.fz += (double)b.fz;
-    acc.e += (double)b.e;
-    if (VIRIAL)
-        {
-        acc.v0 += (double)b.v0;
-        acc.v1 += (double)b.v1;
-        acc.v2 += (double)b.v2;
-        acc.v3 += (double)b.v3;
-        acc.v4 += (double)b.v4;
-        acc.v5 += (double)b.v5;
-        }
-    }
-
-template<bool VIRIAL>
-__device__ __forceinline__ void store(const PairKernelArgs& a, uint32_t i, const Acc& acc)
-    {
-    a.force[i] = make_double4(acc.fx, acc.fy, acc.fz, 0.5 * acc.e);
-    if (VIRIAL)
-        {
-        const size_t p = a.vpitch;
-        a.virial[0 * p + i] = 0.5 * acc.v0;
-        a.virial[1 * p + i] = 0.5 * acc.v1;
-        a.virial[2 * p + i] = 0.5 * acc.v2;
-        a.virial[3 * p + i] = 0.5 * acc.v3;
-        a.virial[4 * p + i] = 0.5 * acc.v4;
-        a.virial[5 * p + i] = 0.5 * acc.v5;
-        }
-    }
-
-// FP64 displacement with minimum image, no contraction (matches the oracle).
-__device__ __forceinline__ double exact_rsq(const PairKernelArgs& a, uint32_t i, uint32_t j)
-    {
-    const double4 pi = a.pos[i];
-    const double4 pj = a.pos[j];
-    const double dx = min_image_1d(pi.x - pj.x, a.box.L[0], a.box.Linv[0]);
-    const double dy = min_image_1d(pi.y - pj.y, a.box.L[1], a.box.Linv[1]);
-    const double dz = min_image_1d(pi.z - pj.z, a.box.L[2], a.box.Linv[2]);
-    return rsq3_nofma(dx, dy, dz);
-    }
-
-// Neighbor entries processed per batch: the batch's list words are loaded
-// first, then all its gathers are issued, then the pair terms are evaluated,
-// so each lane keeps U gathers in flight.  The neighbor list pads every
-// wavefront's columns with the particle's own index to a multiple of 8, so the
-// batch loop is wave-uniform and padding entries are masked by j == i.
-template<bool MIXED> struct Unroll
-    {
-    static constexpr uint32_t value = MIXED ? 8 : 4;
-    };
-
-__device__ __forceinline__ uint32_t wave_max(uint32_t v)
-    {
-    for (int off = 32; off > 0; off >>= 1)
-        v = max(v, (uint32_t)__shfl_xor((int)v, off));
-    return v;
-    }
-
-// Accumulators of the mixed form are kept in integer-grid units of the
-// fixed-point positions (dx = n * L / 2^32): sum f n and sum f n_a n_b are
-// formed from the integer displacements (as FP32) and scaled once at the end.
-template<bool VIRIAL>
-__device__ __forceinline__ void store_scaled(const PairKernelArgs& a, uint32_t i, const Acc& acc)
-    {
-    const double sx = a.fx_scale64[0], sy = a.fx_scale64[1], sz = a.fx_scale64[2];
-    a.force[i] = make_double4(acc.fx * sx, acc.fy * sy, acc.fz * sz, 0.5 * acc.e);
-    if (VIRIAL)
-        {
-        const size_t p = a.vpitch;
-        a.virial[0 * p + i] = 0.5 * acc.v0 * (sx * sx);
-        a.virial[1 * p + i] = 0.5 * acc.v1 * (sx * sy);
-        a.virial[2 * p + i] = 0.5 * acc.v2 * (sx * sz);
-        a.virial[3 * p + i] = 0.5 * acc.v3 * (sy * sy);
-        a.virial[4 * p + i] = 0.5 * acc.v4 * (sy * sz);
-        a.virial[5 * p + i] = 0.5 * acc.v5 * (sz * sz);
-        }
-    }
-
-// Branchless evaluation of one list entry (FP64 form): padding entries and
-// pairs beyond the cutoff are evaluated at a safe distance and contribute zero.
-template<class Eval, bool VIRIAL, bool XPLOR>
-__device__ __forceinline__ void term_fp64(const typename Eval::template Prep<double>& q,
-                                          const TypePairExtra<double>& x,
-                                          bool valid,
-                                          double dx,
-                                          double dy,
-                                          double dz,
-                                          Acc& acc)
-    {
-    const double rsq = dx * dx + dy * dy + dz * dz;
-    const bool in = valid && rsq < x.rcsq64 && Eval::active(q);
-    const double rs = in ? rsq : x.rcsq64;
-    double fdivr, e;
-    Eval::compute(q, rs, fdivr, e);
-    if constexpr (XPLOR)
-        apply_xplor<double>(rs, x.rcsq64, x.ronsq, fdivr, e);
-    fdivr = in ? fdivr : 0.0;
-    e = in ? e : 0.0;
-    accumulate<VIRIAL>(acc, fdivr, dx, dy, dz, e);
-    }
-
-// One list entry of particle i in the mixed form.  `valid` is false for
-// padding; `jkey` is the neighbor's index, or its rank in the tile's neighbor
-// set `jmap` (tiled list), needed only for the FP64 re-test near the cutoff.
-template<class Eval, bool VIRIAL, bool XPLOR, bool CUBIC>
-__device__ __forceinline__ void term_mixed(const PairKernelArgs& a,
-                                           const typename Eval::template Prep<float>& q,
-                                           const TypePairExtra<float>& x,
-                                           bool valid,
-                                           uint32_t i,
-                                           uint32_t jkey,
-                                           const uint32_t* jmap,
-                                           int32_t ix,
-                                           int32_t iy,
-                                           int32_t iz,
-                                           Acc32& acc)
-    {
-    const float fx = (float)ix, fy = (float)iy, fz = (float)iz;
-    float rsq;
-    if constexpr (CUBIC)
-        rsq = (fx * fx + fy * fy + fz * fz) * a.fx_scale2[0];
-    else
-        rsq = fx * fx * a.fx_scale2[0] + fy * fy * a.fx_scale2[1] + fz * fz * a.fx_scale2[2];
-    bool in = valid && rsq < x.rcsq_lo;
-    double r64 = -1.0;
-    if (valid && !in && rsq < x.rcsq_hi)
-        {
-        // FP32 cannot decide this one: apply the FP64 cutoff test
-        const uint32_t j = jmap ? jmap[jkey] : jkey;
-        r64 = exact_rsq(a, i, j);
-        in = r64 < x.rcsq64;
-        rsq = (float)r64;
-        }
-    in = in && Eval::active(q);
-    float fdivr, e;
-    if constexpr (HasMasked<Eval>::value && !XPLOR)
-        {
-        // masked entries at r^2 = +inf (the evaluator's terms vanish there)
-        Eval::compute_masked(q, in ? rsq : __builtin_huge_valf(), in, fdivr, e);
-        accumulate32<VIRIAL>(acc, fdivr, fx, fy, fz, e);
-        return;
-        }
-    const float rs = in ? rsq : x.rcsq;
-    if constexpr (NeedsRsq64<Eval>::value)
-        {
-        if (r64 < 0.0)
-            {
-            const double ex = (double)ix * a.fx_scale64[0];
-            const double ey = (double)iy * a.fx_scale64[1];
-            const double ez = (double)iz * a.fx_scale64[2];
-            r64 = ex * ex + ey * ey + ez * ez;
-            }
-        Eval::compute_precise(q, rs, in ? r64 : x.rcsq64, fdivr, e);
-        }
-    else
-        {
-        Eval::compute(q, rs, fdivr, e);
-        }
-    if constexpr (XPLOR)
-        apply_xplor<float>(rs, x.rcsq, x.ronsq, fdivr, e);
-    fdivr = in ? fdivr : 0.0f;
-    e = in ? e : 0.0f;
-    accumulate32<VIRIAL>(acc, fdivr, fx, fy, fz, e);
-    }
-
-// LDS bytes of the per-type-pair tables, rounded up to 16 (the tile's
-// positions follow them).
-template<class Eval, typename R> __host__ __device__ constexpr size_t table_bytes(uint32_t nt2)
-    {
-    return (nt2 * (sizeof(typename Eval::template Prep<R>) + sizeof(TypePairExtra<R>)) + 15) & ~(size_t)15;
-    }
-
-// LDS the tiled kernel may use for its type-pair tables and neighbor slots.
-constexpr size_t kTileLds = 64 * 1024;
-
-template<class Eval> static bool tile_fits(uint32_t ntypes, uint32_t slots)
-    {
-    const size_t tables = ntypes == 1 ? 0 : table_bytes<Eval, float>(ntypes * ntypes);
-    return tables + (size_t)slots * sizeof(int4) <= kTileLds;
-    }
-
-template<class Eval, bool MIXED, bool VIRIAL, bool SINGLE, bool XPLOR, bool CUBIC, bool TILED>
-__global__ void __launch_bounds__(512) k_pair(const PairKernelArgs a)
-    {
-    using R = std::conditional_t<MIXED, float, double>;
-    using Prep = typename Eval::template Prep<R>;
-    constexpr uint32_t U = Unroll<MIXED>::value;
-    extern __shared__ double smem[];
-    const uint32_t nt = a.ntypes;
-    const uint32_t nt2 = nt * nt;
-    // tiled: the neighbor positions sit at LDS offset 0 (so a list entry,
-    // 16 x rank, is their byte address) and the type-pair tables after them
-    Prep* s_prep = reinterpret_cast<Prep*>(reinterpret_cast<char*>(smem) + (TILED ? a.tile_slots * sizeof(int4) : 0));
-    TypePairExtra<R>* s_x = reinterpret_cast<TypePairExtra<R>*>(s_prep + nt2);
-
-    Prep q0;
-    TypePairExtra<R> x0;
-    if (SINGLE)
-        {
-        prepare_pair<Eval, R>(a, 0, q0, x0);
-        }
-    else
-        {
-        for (uint32_t t = threadIdx.x; t < nt2; t += blockDim.x)
-            prepare_pair<Eval, R>(a, t, s_prep[t], s_x[t]);
-        }
-
-    const uint32_t lb = xcd_logical_block();
-    int4* s_pos = nullptr;
-    const uint32_t* tile_js = nullptr;
-    uint32_t tile_m = 0;
-    if constexpr (TILED)
-        {
-        // stage the tile's neighbor positions, converted to fixed point here
-        // (block lb is tile lb); every lane of the workgroup takes part before
-        // any wavefront may leave.  Slot M is the padding entries' dummy.
-        s_pos = reinterpret_cast<int4*>(smem);
-        tile_js = a.tile_jset + (size_t)lb * kTileCap;
-        tile_m = a.tile_count[lb];
-        for (uint32_t m = threadIdx.x; m < tile_m; m += blockDim.x)
-            s_pos[m] = fixed4(a.pos[tile_js[m]], a.box);
-        if (threadIdx.x == 0)
-            s_pos[tile_m] = make_int4(0, 0, 0, 0);
-        }
-    if (!SINGLE || TILED)
-        __syncthreads();
-
-    const uint32_t i = lb * blockDim.x + threadIdx.x;
-    // wavefronts entirely past N leave together; lanes past N in the last
-    // wavefront shadow particle N-1 (same column, results not stored)
-    if (lb * blockDim.x + (threadIdx.x & ~63u) >= a.N)
-        return;
-    const uint32_t ie = min(i, a.N - 1);
-    // slot-ordered tiles: column ie belongs to particle pe
-    const bool by_slot = TILED && a.tile_order != nullptr;
-    const uint32_t pe = by_slot ? a.tile_order[ie] : ie;
-    const uint32_t nn = by_slot ? a.tile_nn[ie] : a.n_neigh[ie];
-    // wave-uniform trip count, held in a scalar register
-    const uint32_t P = __builtin_amdgcn_readfirstlane((wave_max(nn) + U - 1) & ~(U - 1));
-    const size_t stride = a.nlist_pitch;
-
-    Acc acc;
-    if constexpr (!MIXED)
-        {
-        const double4 pi = a.pos[ie];
-        const uint32_t ti = (uint32_t)pi.w;
-        for (uint32_t k = 0; k < P; k += U)
-            {
-            uint32_t jj[U];
-#pragma unroll
-            for (uint32_t u = 0; u < U; ++u)
-                jj[u] = (a.nlist + (size_t)(k + u) * stride)[ie];
-            double4 pj[U];
-#pragma unroll
-            for (uint32_t u = 0; u < U; ++u)
-                pj[u] = a.pos[jj[u]];
-#pragma unroll
-            for (uint32_t u = 0; u < U; ++u)
-                {
-                const double dx = min_image_1d(pi.x - pj[u].x, a.box.L[0], a.box.Linv[0]);
-                const double dy = min_image_1d(pi.y - pj[u].y, a.box.L[1], a.box.Linv[1]);
-                const double dz = min_image_1d(pi.z - pj[u].z, a.box.L[2], a.box.Linv[2]);
-                const uint32_t tp = SINGLE ? 0u : ti * nt + (uint32_t)pj[u].w;
-                term_fp64<Eval, VIRIAL, XPLOR>(SINGLE ? q0 : s_prep[tp], SINGLE ? x0 : s_x[tp],
-                                               jj[u] != ie, dx, dy, dz, acc);
-                }
-            }
-        if (i < a.N)
-            store<VIRIAL>(a, i, acc);
-        }
-    else if constexpr (TILED)
-        {
-        const int4 pi = fixed4(a.pos[pe], a.box);
-        const uint32_t ti = (uint32_t)pi.w;
-        const uint32_t col_off = ie * 4u;
-        const uint32_t tile_m16 = tile_m << 4; // the padding entry
-        Acc32 part;
-        for (uint32_t k = 0; k < P; k += U)
-            {
-            uint32_t w[U / 2];
-#pragma unroll
-            for (uint32_t u = 0; u < U / 2; ++u)
-                {
-                const char* row = reinterpret_cast<const char*>(a.tile_list + (size_t)((k >> 1) + u) * stride);
-                w[u] = *reinterpret_cast<const uint32_t*>(row + col_off);
-                }
-            uint32_t e[U];
-#pragma unroll
-            for (uint32_t u = 0; u < U / 2; ++u)
-                {
-                e[2 * u] = w[u] & 0xffffu;
-                e[2 * u + 1] = w[u] >> 16;
-                }
-            int4 pj[U];
-#pragma unroll
-            for (uint32_t u = 0; u < U; ++u)
-                pj[u] = *reinterpret_cast<const int4*>(reinterpret_cast<const char*>(s_pos) + e[u]);
-#pragma unroll
-            for (uint32_t u = 0; u < U; ++u)
-                {
-                const uint32_t tp = SINGLE ? 0u : ti * nt + (uint32_t)pj[u].w;
-                term_mixed<Eval, VIRIAL, XPLOR, CUBIC>(a, SINGLE ? q0 : s_prep[tp], SINGLE ? x0 : s_x[tp],
-                                                       e[u] != tile_m16, pe, e[u] >> 4, tile_js,
-                                                       fixed_delta(pi.x, pj[u].x), fixed_delta(pi.y, pj[u].y),
-                                                       fixed_delta(pi.z, pj[u].z), part);
-                }
-            if (((k + U) & (kFlushEntries - 1)) == 0 || k + U >= P)
-                {
-                flush32<VIRIAL>(acc, part);
-                part = Acc32();
-                }
-            }
-        if (i < a.N)
-            store_scaled<VIRIAL>(a, pe, acc);
-        }
-    else
-        {
-        const int4 pi = a.pos_fixed[ie];
-        const uint32_t ti = (uint32_t)pi.w;
-        // 32-bit byte offsets from wave-uniform bases (global loads with an
-        // SGPR base and a VGPR offset: no per-entry 64-bit address math)
-        const uint32_t col_off = ie * 4u;
-        const char* pf_base = reinterpret_cast<const char*>(a.pos_fixed);
-        Acc32 part;
-        for (uint32_t k = 0; k < P; k += U)
-            {
-            uint32_t jj[U];
-#pragma unroll
-            for (uint32_t u = 0; u < U; ++u)
-                {
-                const char* row = reinterpret_cast<const char*>(a.nlist + (size_t)(k + u) * stride);
-                jj[u] = *reinterpret_cast<const uint32_t*>(row + col_off);
-                }
-            int4 pj[U];
-#pragma unroll
-            for (uint32_t u = 0; u < U; ++u)
-                pj[u] = *reinterpret_cast<const int4*>(pf_base + (jj[u] << 4));
-#pragma unroll
-            for (uint32_t u = 0; u < U; ++u)
-                {
-                const uint32_t tp = SINGLE ? 0u : ti * nt + (uint32_t)pj[u].w;
-                term_mixed<Eval, VIRIAL, XPLOR, CUBIC>(a, SINGLE ? q0 : s_prep[tp], SINGLE ? x0 : s_x[tp],
-                                                       jj[u] != ie, ie, jj[u], nullptr, fixed_delta(pi.x, pj[u].x),
-                                                       fixed_delta(pi.y, pj[u].y), fixed_delta(pi.z, pj[u].z),
-                                                       part);
-                }
-            if (((k + U) & (kFlushEntries - 1)) == 0 || k + U >= P)
-                {
-                flush32<VIRIAL>(acc, part);
-                part = Acc32();
-                }
-            }
-        if (i < a.N)
-            store_scaled<VIRIAL>(a, i, acc);
-        }
-    }
 
 __global__ void k_pack_fixed(const double4* __restrict__ pos, uint32_t N, DevBox box, int4* __restrict__ out)
     {
@@ -536,81 +36,6 @@ __global__ void k_pack_fixed(const double4* __restrict__ pos, uint32_t N, DevBox
     out[i] = fixed4(pos[i], box);
     }
 
-template<class Eval, bool MIXED, bool VIRIAL, bool SINGLE, bool XPLOR, bool CUBIC, bool TILED>
-static int launch_t(const PairKernelArgs& ka, unsigned block, unsigned slots, hipStream_t stream)
-    {
-    using R = std::conditional_t<MIXED, float, double>;
-    const size_t nt2 = (size_t)ka.ntypes * ka.ntypes;
-    size_t shmem = SINGLE ? 0 : table_bytes<Eval, R>((uint32_t)nt2);
-    PairKernelArgs kb = ka;
-    kb.tile_slots = TILED ? slots : 0u;
-    if (TILED)
-        shmem += (size_t)slots * sizeof(int4);
-    if (shmem > kTileLds)
-        return set_error(AZP_ERR_UNSUPPORTED, "too many particle types (%u) for LDS staging", ka.ntypes);
-    const unsigned grid = grid_for(ka.N, block);
-    hipLaunchKernelGGL((k_pair<Eval, MIXED, VIRIAL, SINGLE, XPLOR, CUBIC, TILED>), dim3(grid), dim3(block), shmem,
-                       stream, kb);
-    AZP_LAUNCH_CHECK();
-    return AZP_OK;
-    }
-
-template<class Eval, bool MIXED, bool VIRIAL, bool SINGLE, bool TILED>
-static int launch_mode(const PairKernelArgs& ka, unsigned block, unsigned slots, hipStream_t s)
-    {
-    const bool xplor = ka.shift_mode == AZP_SHIFT_XPLOR;
-    if constexpr (MIXED)
-        {
-        const bool cubic = ka.fx_scale2[0] == ka.fx_scale2[1] && ka.fx_scale2[0] == ka.fx_scale2[2];
-        if (xplor)
-            return cubic ? launch_t<Eval, true, VIRIAL, SINGLE, true, true, TILED>(ka, block, slots, s)
-                         : launch_t<Eval, true, VIRIAL, SINGLE, true, false, TILED>(ka, block, slots, s);
-        return cubic ? launch_t<Eval, true, VIRIAL, SINGLE, false, true, TILED>(ka, block, slots, s)
-                     : launch_t<Eval, true, VIRIAL, SINGLE, false, false, TILED>(ka, block, slots, s);
-        }
-    else
-        {
-        return xplor ? launch_t<Eval, false, VIRIAL, SINGLE, true, false, false>(ka, block, slots, s)
-                     : launch_t<Eval, false, VIRIAL, SINGLE, false, false, false>(ka, block, slots, s);
-        }
-    }
-
-template<class Eval, bool MIXED, bool TILED>
-static int launch_virial(const PairKernelArgs& ka, bool virial, unsigned block, unsigned slots, hipStream_t s)
-    {
-    const bool single = ka.ntypes == 1;
-    if (virial)
-        return single ? launch_mode<Eval, MIXED, true, true, TILED>(ka, block, slots, s)
-                      : launch_mode<Eval, MIXED, true, false, TILED>(ka, block, slots, s);
-    return single ? launch_mode<Eval, MIXED, false, true, TILED>(ka, block, slots, s)
-                  : launch_mode<Eval, MIXED, false, false, TILED>(ka, block, slots, s);
-    }
-
-// The tiled list serves the mixed form when the tile tables fit next to the
-// type-pair tables in 64 KB of LDS; otherwise the plain list is used.
-template<class Eval, bool MIXED>
-static int launch_eval(const PairKernelArgs& ka,
-                       bool virial,
-                       unsigned block,
-                       unsigned tile_size,
-                       unsigned tile_slots,
-                       hipStream_t s)
-    {
-    if constexpr (MIXED)
-        {
-        if (tile_size && ka.tile_list && tile_fits<Eval>(ka.ntypes, tile_slots))
-            return launch_virial<Eval, true, true>(ka, virial, tile_size, tile_slots, s);
-        if (!ka.nlist)
-            return set_error(AZP_ERR_INVALID,
-                             "the tiled list's %u neighbor slots and the type tables exceed %u bytes of LDS; "
-                             "pass the plain list",
-                             tile_slots,
-                             (unsigned)kTileLds);
-        if (!ka.pos_fixed)
-            return set_error(AZP_ERR_INVALID, "mixed precision without a tiled list needs d_pos_fixed");
-        }
-    return launch_virial<Eval, MIXED, false>(ka, virial, block, 0, s);
-    }
 
 int pack_fixed(const double* d_pos, uint32_t N, const azp_box& box, int32_t* d_out, hipStream_t stream)
     {
@@ -664,7 +89,7 @@ int validate_common(const azp_pair_args* args)
         return set_error(AZP_ERR_BOX, "box lengths must be positive");
     if (args->ntypes == 0)
         return set_error(AZP_ERR_INVALID, "ntypes must be >= 1");
-    if (args->nlist_pitch < p.N)
+    if (!args->d_head_list && args->nlist_pitch < p.N)
         return set_error(AZP_ERR_INVALID, "nlist pitch %u < N %u", args->nlist_pitch, p.N);
     if (args->compute_virial && (!args->out.d_virial || args->out.virial_pitch < p.N))
         return set_error(AZP_ERR_INVALID, "virial requested without a virial array of pitch >= N");
@@ -691,7 +116,54 @@ extern "C" int azp_gpu_pack_fixed(const double* d_pos,
     return pack_fixed(d_pos, N, *box, d_pos_fixed, (hipStream_t)stream);
     }
 
-extern "C" int azp_gpu_compute_pair_forces(const azp_pair_args* args, int evaluator, const void* d_params)
+namespace azp
+{
+
+// Library-owned scratch of the launcher: the fixed-point positions of the
+// mixed form over the plain list when the caller passes no d_pos_fixed.  One
+// buffer per device, grown on demand (after a device synchronization) and
+// kept until the process ends.  Launches that overlap on one device share it.
+struct LauncherScratch
+    {
+    int4* ptr = nullptr;
+    size_t n = 0;
+    };
+constexpr int kScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static LauncherScratch g_scratch_heap[kScratchDevices];
+
+int launcher_heap_scratch(uint32_t N, int4** out)
+    {
+    int dev = 0;
+    AZP_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kScratchDevices)
+        return set_error(AZP_ERR_INVALID, "device ordinal %d is outside the launcher's table", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    LauncherScratch& b = g_scratch_heap[dev];
+    if (b.n < N)
+        {
+        if (b.ptr)
+            {
+            AZP_HIP(hipDeviceSynchronize());
+            AZP_HIP(hipFree(b.ptr));
+            b.ptr = nullptr;
+            b.n = 0;
+            }
+        const size_t want = std::max<size_t>((size_t)N + N / 8, 4096);
+        AZP_HIP(hipMalloc((void**)&b.ptr, want * sizeof(int4)));
+        b.n = want;
+        }
+    *out = b.ptr;
+    return AZP_OK;
+    }
+
+// Where the launcher puts fixed-point positions it has to pack itself.
+enum class PackTarget
+    {
+    Heap,
+    };
+
+static int compute_pair_forces(const azp_pair_args* args, int evaluator, const void* d_params, PackTarget target)
     {
     AZP_TRY(validate_common(args));
     const azp_particles& p = args->particles;
@@ -700,8 +172,6 @@ extern "C" int azp_gpu_compute_pair_forces(const azp_pair_args* args, int evalua
     if (!d_params)
         return set_error(AZP_ERR_INVALID, "null params");
     const bool mixed = args->precision == AZP_PREC_MIXED;
-    if (mixed && !p.d_pos_fixed && !args->d_tile_list)
-        return set_error(AZP_ERR_INVALID, "mixed precision needs d_pos_fixed (azp_gpu_pack_fixed)");
     if (args->shift_mode == AZP_SHIFT_XPLOR && !args->d_ronsq)
         return set_error(AZP_ERR_INVALID, "xplor needs d_ronsq");
 
@@ -732,6 +202,9 @@ extern "C" int azp_gpu_compute_pair_forces(const azp_pair_args* args, int evalua
     ka.tile_list = args->d_tile_list;
     ka.tile_order = args->d_tile_list ? args->d_tile_order : nullptr;
     ka.tile_nn = args->d_tile_list ? args->d_tile_nn : nullptr;
+    ka.head_list = args->d_head_list;
+    if (args->d_head_list && (args->d_tile_list || !args->d_nlist))
+        return set_error(AZP_ERR_INVALID, "HOOMD's head list goes with d_nlist and without a tiled list");
     unsigned tile = 0, slots = 0;
     if (args->d_tile_list)
         {
@@ -753,6 +226,23 @@ extern "C" int azp_gpu_compute_pair_forces(const azp_pair_args* args, int evalua
         return set_error(AZP_ERR_INVALID, "block size %u must be a multiple of 64 in [64, 512]", block);
     const bool vir = args->compute_virial != 0;
     hipStream_t s = (hipStream_t)args->stream;
+    // the mixed form over the plain list reads fixed-point positions: packed
+    // here into the launcher's scratch when the caller brings none
+    const bool tiled_ok = tile != 0 && pair_tile_fits(evaluator, args->ntypes, slots);
+    if (mixed && !p.d_pos_fixed && !tiled_ok && args->d_nlist && evaluator != AZP_EVAL_COLLOID)
+        {
+        int4* pf = nullptr;
+        switch (target)
+            {
+        case PackTarget::Heap:
+            AZP_TRY(launcher_heap_scratch(p.N, &pf));
+            break;
+            }
+        AZP_TRY(pack_fixed(p.d_pos, p.N, p.box, reinterpret_cast<int32_t*>(pf), s));
+        ka.pos_fixed = pf;
+        }
+    if (args->d_head_list)
+        return launch_pair_head(ka, evaluator, mixed, vir, block, s);
 
     switch (evaluator)
         {
@@ -779,3 +269,11 @@ extern "C" int azp_gpu_compute_pair_forces(const azp_pair_args* args, int evalua
         return set_error(AZP_ERR_INVALID, "evaluator %d is not an isotropic pair evaluator", evaluator);
         }
     }
+
+} // namespace azp
+
+extern "C" int azp_gpu_compute_pair_forces(const azp_pair_args* args, int evaluator, const void* d_params)
+    {
+    return compute_pair_forces(args, evaluator, d_params, PackTarget::Heap);
+    }
+

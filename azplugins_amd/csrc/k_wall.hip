@@ -225,7 +225,8 @@ struct azp_wall
     DeviceArray<double> d_params, d_geom;
     DeviceArray<uint32_t> d_flags;
     uint32_t* h_flags = nullptr;
-    bool check_pending = false; // a type-error flag copy was queued by the last compute
+    bool check_pending = false; // a type-error flag copy is queued and not yet read
+    bool flag_ready = false;    // the device flag was cleared once after allocation
 
     ~azp_wall()
         {
@@ -395,6 +396,33 @@ extern "C" int azp_wall_set_geometry(azp_wall* w,
     return AZP_OK;
     }
 
+// Reports the type error the sticky device flag holds and clears the flag, so
+// that calls made after the caller fixed the types run normally.
+static int wall_report_type_error(azp_wall* w, hipStream_t s)
+    {
+    hipLaunchKernelGGL(k_wall_clear_flag, dim3(1), dim3(1), 0, s, w->d_flags.ptr);
+    AZP_LAUNCH_CHECK();
+    return set_error(AZP_ERR_INVALID, "a particle type index is outside [0, %u) (pos.w must hold the type)",
+                     w->ntypes);
+    }
+
+extern "C" int azp_wall_check(azp_wall* w, void* stream)
+    {
+    if (!w)
+        return set_error(AZP_ERR_INVALID, "null argument");
+    if (!w->flag_ready)
+        return AZP_OK; // nothing computed yet
+    hipStream_t s = (hipStream_t)stream;
+    *reinterpret_cast<volatile uint32_t*>(w->h_flags) = kWallFlagPending;
+    std::atomic_thread_fence(std::memory_order_seq_cst);
+    AZP_HIP(hipMemcpyAsync(w->h_flags, w->d_flags.ptr, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    AZP_HIP(hipStreamSynchronize(s));
+    w->check_pending = false;
+    if (*reinterpret_cast<volatile uint32_t*>(w->h_flags) & 1u)
+        return wall_report_type_error(w, s);
+    return AZP_OK;
+    }
+
 extern "C" int azp_wall_compute(azp_wall* w, const azp_particles* p, const azp_outputs* out, int compute_virial,
                                 void* stream)
     {
@@ -411,8 +439,10 @@ extern "C" int azp_wall_compute(azp_wall* w, const azp_particles* p, const azp_o
     if (compute_virial && out->virial_pitch < N)
         return set_error(AZP_ERR_INVALID, "virial pitch %u is smaller than N %u", out->virial_pitch, N);
     hipStream_t s = (hipStream_t)stream;
-    // The type-error flag of the previous call, read without waiting: its
-    // copy has landed once the stream passed it (a sentinel until then).
+    // The type-error flag, read without waiting: its copy has landed once the
+    // stream passed it (a sentinel until then).  The device flag is sticky
+    // (kernels only OR into it), so a copy queued by any earlier call sees
+    // every error up to its place in the stream.
     if (w->check_pending)
         {
         const uint32_t v = *reinterpret_cast<volatile uint32_t*>(w->h_flags);
@@ -420,8 +450,7 @@ extern "C" int azp_wall_compute(azp_wall* w, const azp_particles* p, const azp_o
             {
             w->check_pending = false;
             if (v & 1u)
-                return set_error(AZP_ERR_INVALID,
-                                 "a particle type index is outside [0, %u) (pos.w must hold the type)", w->ntypes);
+                return wall_report_type_error(w, s);
             }
         }
     if (!w->h_flags && hipHostMalloc((void**)&w->h_flags, sizeof(uint32_t), hipHostMallocCoherent) != hipSuccess)
@@ -442,7 +471,11 @@ extern "C" int azp_wall_compute(azp_wall* w, const azp_particles* p, const azp_o
         AZP_HIP(hipStreamSynchronize(s)); // the host vectors may change after return
         w->dirty = false;
         }
-    hipLaunchKernelGGL(k_wall_clear_flag, dim3(1), dim3(1), 0, s, w->d_flags.ptr);
+    if (!w->flag_ready)
+        {
+        hipLaunchKernelGGL(k_wall_clear_flag, dim3(1), dim3(1), 0, s, w->d_flags.ptr);
+        w->flag_ready = true;
+        }
     const unsigned B = 256;
     auto launch = [&](auto kernel) {
         hipLaunchKernelGGL(kernel, dim3(grid_for(N, B)), dim3(B), 0, s, reinterpret_cast<const double4*>(p->d_pos), N,

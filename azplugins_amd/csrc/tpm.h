@@ -246,8 +246,8 @@ template<bool MIXED, bool VIRIAL, bool SINGLE, bool DIRS>
 __device__ __forceinline__ void tpm_list(uint32_t ie,
                                          uint32_t nn,
                                          uint32_t trips,
-                                         const uint32_t* nlist,
-                                         uint32_t pitch,
+                                         const uint32_t* row,
+                                         size_t stride,
                                          const double4* pos,
                                          const int4* pos_fixed,
                                          const double4* orientation,
@@ -267,7 +267,7 @@ __device__ __forceinline__ void tpm_list(uint32_t ie,
         uint32_t jj[U];
 #pragma unroll
         for (uint32_t u = 0; u < U; ++u)
-            jj[u] = k + u < nn ? nlist[(size_t)(k + u) * pitch + ie] : ie;
+            jj[u] = k + u < nn ? row[(size_t)(k + u) * stride] : ie;
         PosT pj[U];
         OriT oj[U];
 #pragma unroll

@@ -163,6 +163,8 @@ class Simulation:
                         w._act(nxt)
             for w in writers:
                 w.flush()
+            if integ is not None:
+                integ._finish_run()
         finally:
             self.state._thaw_view()
 

@@ -73,6 +73,11 @@ class Integrator:
             m._step_two(self, timestep, dt=0.0)
         self._one_done = False
 
+    def _finish_run(self):
+        """End of run(): forces that report errors late (walls) check now."""
+        for f in self.forces:
+            f._finish_run()
+
     # Steps whose second half runs straight into the next step's first half
     # (nothing acts between them) fuse the two in one kernel per particle.
     fuse_half_steps = True

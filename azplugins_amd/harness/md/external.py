@@ -81,5 +81,9 @@ class WallPotential(Force):
         self._cpp.compute(self._sim.state.particles(), self._outputs(), compute_virial=virial)
         self._virial_current = bool(virial)
 
+    def _finish_run(self):
+        if self._cpp is not None:
+            self._cpp.check()
+
 
 wall = _types.SimpleNamespace(WallPotential=WallPotential)

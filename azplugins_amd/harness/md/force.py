@@ -51,6 +51,10 @@ class Force:
         if nl is not None:
             nl._invalidate()
 
+    def _finish_run(self):
+        """Called at the end of every run(); errors a compute flagged on the
+        device without waiting are reported here at the latest."""
+
     def _require(self):
         if not self._attached:
             raise RuntimeError("the force is not attached to a running simulation")

@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define AZP_ABI_VERSION 2
+#define AZP_ABI_VERSION 3
 
 enum azp_status
     {
@@ -147,7 +147,11 @@ typedef struct
     uint32_t N;
     azp_box box;
     const double* d_pos;
-    const int32_t* d_pos_fixed; /* optional (mixed); filled internally when NULL */
+    /* optional (mixed form over the plain or head list): when NULL the
+     * launchers pack it into library-owned device scratch (one buffer per
+     * device, grown on demand, kept until the process ends; launches that
+     * overlap on one device share it).  The tiled list does not read it. */
+    const int32_t* d_pos_fixed;
     const double* d_vel;
     const uint32_t* d_tag;
     const double* d_orientation;
@@ -197,6 +201,13 @@ typedef struct
     uint32_t tile_slots;
     const uint32_t* d_tile_order;
     const uint32_t* d_tile_nn;
+    /* optional: HOOMD's own list layout (pair_args_t of
+     * PotentialPairGPUKernel.cu.inc:25-28).  When non-NULL, particle i's
+     * neighbors are d_nlist[d_head_list[i] + k] for k < d_n_neigh[i] (rows
+     * unpadded); nlist_pitch is not read and no tiled list may be given.
+     * Results equal the column layout's bit for bit (same entries in the
+     * same order). */
+    const uint64_t* d_head_list;
     } azp_pair_args;
 
 /* ---- library ---- */
@@ -429,9 +440,11 @@ int azp_gather_rows(const void* d_src, void* d_dst, const uint32_t* d_order, uin
  * open}; inside / open as 0 or 1 (HOOMD's hoomd.wall attributes).  compute
  * writes force.xyz, the energy (shifted to zero at r_cut, all of it on the
  * particle) in force.w and, when asked, the virial 1/2 (s n) x F.  It does
- * not wait for the stream: a particle type outside [0, ntypes) is reported
- * by a later call, once the flag that the kernel raised has reached the
- * host. ---- */
+ * not wait for the stream: a particle type outside [0, ntypes) raises a
+ * sticky device flag, reported by a later compute once its copy has reached
+ * the host, or by azp_wall_check, which waits for the stream (callers run it
+ * at the end of a run so that an error of the last step is not lost).  The
+ * flag is cleared when an error is reported. ---- */
 typedef struct azp_wall azp_wall;
 int azp_wall_create(azp_wall** out, const char* cpp_class_name, uint32_t ntypes);
 int azp_wall_destroy(azp_wall* w);
@@ -446,6 +459,7 @@ int azp_wall_set_geometry(azp_wall* w,
                           uint32_t n_planes,
                           const double* planes);
 int azp_wall_compute(azp_wall* w, const azp_particles* p, const azp_outputs* out, int compute_virial, void* stream);
+int azp_wall_check(azp_wall* w, void* stream);
 
 /* ---- pair potential objects (replace PotentialPair<Name>GPU,
  * PotentialPairDPDThermo<Name>GPU, AnisoPotentialPair<Name>GPU) ---- */

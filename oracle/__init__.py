@@ -226,7 +226,11 @@ def nlist(pos4, L, ntypes, r_listsq, method="cell"):
     rl = _f64(r_listsq)
     N = pos4.shape[0]
     fn = lib().orc_nlist_cell if method == "cell" else lib().orc_nlist_brute
-    cap = max(1024, N * 32)
+    # half-list size of a uniform fluid at this density and the largest list
+    # radius, with 30 % headroom, so that one walk usually suffices
+    rmax = float(np.sqrt(np.max(rl))) if rl.size else 0.0
+    est = 0.5 * N * (N / float(np.prod(L))) * 4.0 / 3.0 * np.pi * rmax**3
+    cap = int(max(1024, min(1.3 * est + 16 * N, N * (N - 1) / 2 + 1)))
     while True:
         pi = np.zeros(cap, dtype=np.uint32)
         pj = np.zeros(cap, dtype=np.uint32)

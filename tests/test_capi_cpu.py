@@ -30,7 +30,7 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(L, n), n
     bound = {s[0] for s in _azp.SIGNATURES}
     assert set(names) == bound, set(names) ^ bound
-    assert L.azp_abi_version() == _azp.ABI_VERSION == 2
+    assert L.azp_abi_version() == _azp.ABI_VERSION == 3
 
 
 KEYS = oracle.PARAM_KEYS

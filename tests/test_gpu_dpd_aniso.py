@@ -57,25 +57,61 @@ def _isolated_pairs(npairs, L, rc, seed):
     return pos4, vel4, tags
 
 
-def test_dpd_isolated_pairs_bit_exact():
+# the DPD workload's parameters (2M particles, density 3): A 25 / 40 / 25,
+# gamma 4.5, s 0.5 for every type pair
+DPD_C3 = {(0, 0): {"A": 25.0, "gamma": 4.5, "s": 0.5},
+          (0, 1): {"A": 40.0, "gamma": 4.5, "s": 0.5},
+          (1, 1): {"A": 25.0, "gamma": 4.5, "s": 0.5}}
+
+
+@pytest.mark.parametrize("which", ["mixed_s", "workload"])
+def test_dpd_isolated_pairs_bit_exact(which):
     """Each particle feels exactly one pair force: GPU == oracle bit for bit."""
+    params = DPD_PARAMS if which == "mixed_s" else DPD_C3
     L = 60.0
     pos4, vel4, tags = _isolated_pairs(3000, L, 1.0, 11)
     Lv = np.array([L, L, L])
-    seed, t, dt, kT = 4321, 98765, 0.02, 1.3
-    say("[dpd isolated]")
+    seed, t, dt, kT = (4321, 98765, 0.02, 1.3) if which == "mixed_s" else (3, 123457, 0.02, 1.0)
+    say(f"[dpd isolated] {which}")
     r = gpu_pair("PotentialPairDPDThermoGeneralWeight", GpuSystem(pos4, Lv, vel4, tag=tags), 2,
-                 DPD_PARAMS, 1.0, thermostat=(seed, dt, kT), timestep=t)
+                 params, 1.0, thermostat=(seed, dt, kT), timestep=t)
     pairs = oracle.nlist(pos4, Lv, 2, oracle.r_listsq_matrix([1.0] * 4, 0.4), "brute")
     assert len(pairs[0]) == 3000
-    fo, vo = oracle.dpd_forces(pos4, vel4, tags, Lv, pairs, 2, oracle.pack_matrix(4, DPD_PARAMS, 2),
+    fo, vo = oracle.dpd_forces(pos4, vel4, tags, Lv, pairs, 2, oracle.pack_matrix(4, params, 2),
                                np.ones(4), seed, t, dt, kT, portable_pow=True)
     assert np.array_equal(r.force.view(np.uint64), fo.view(np.uint64))
     assert np.array_equal(r.virial.view(np.uint64), vo.view(np.uint64))
     # the reference's libm pow differs from the shared portable pow by ulps only
-    fl, _ = oracle.dpd_forces(pos4, vel4, tags, Lv, pairs, 2, oracle.pack_matrix(4, DPD_PARAMS, 2),
+    fl, _ = oracle.dpd_forces(pos4, vel4, tags, Lv, pairs, 2, oracle.pack_matrix(4, params, 2),
                               np.ones(4), seed, t, dt, kT, portable_pow=False)
     assert rel_err(r.force, fl) < 1e-13
+
+
+def test_dpd_many_types_bit_exact():
+    """20 particle types: the type-pair table (22 KB) exceeds the LDS budget of
+    the staged table, so the kernel derives each pair's entry from the global
+    arrays; isolated pairs still equal the oracle bit for bit."""
+    L = 60.0
+    nt = 20
+    pos4, vel4, tags = _isolated_pairs(3000, L, 1.0, 13)
+    rng = np.random.default_rng(17)
+    pos4[:, 3] = rng.integers(0, nt, size=pos4.shape[0])
+    params = {}
+    for a in range(nt):
+        for b in range(a, nt):
+            params[(a, b)] = {"A": float(rng.uniform(10, 40)), "gamma": float(rng.uniform(1, 5)),
+                              "s": float(rng.choice([0.5, 1.0, 2.0]))}
+    Lv = np.array([L, L, L])
+    seed, t, dt, kT = 99, 31337, 0.02, 1.1
+    say(f"[dpd many types] ntypes={nt}")
+    r = gpu_pair("PotentialPairDPDThermoGeneralWeight", GpuSystem(pos4, Lv, vel4, tag=tags), nt,
+                 params, 1.0, thermostat=(seed, dt, kT), timestep=t)
+    pairs = oracle.nlist(pos4, Lv, nt, oracle.r_listsq_matrix([1.0] * nt * nt, 0.4), "brute")
+    assert len(pairs[0]) == 3000
+    fo, vo = oracle.dpd_forces(pos4, vel4, tags, Lv, pairs, nt, oracle.pack_matrix(4, params, nt),
+                               np.ones(nt * nt), seed, t, dt, kT, portable_pow=True)
+    assert np.array_equal(r.force.view(np.uint64), fo.view(np.uint64))
+    assert np.array_equal(r.virial.view(np.uint64), vo.view(np.uint64))
 
 
 def test_dpd_many_body():

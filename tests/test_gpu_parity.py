@@ -193,3 +193,49 @@ def test_invalid_inputs_are_errors():
     sysm.pos[3, 0] = float("nan")
     with pytest.raises(_azp.AzpError, match="not finite"):
         r.nl.update(sysm.particles())
+
+
+@pytest.mark.parametrize("tiled", [False, True])
+def test_failed_build_forces_a_rebuild(tiled):
+    """A build that fails (a type index without parameters) leaves no list.
+    The particles move by more than r_buff / 2 after a good build, the rebuild
+    fails on a bad type; once the type is valid again, the next update must
+    rebuild although the positions did not move since the failed attempt, and
+    the forces equal the oracle's."""
+    from azplugins_amd import _azp, _azplugins
+
+    L = 12.0
+    pos4, rng = jittered_lattice(12, L, 0.1, 1, seed=5)
+    Lv = np.array([L, L, L])
+    params = {(0, 0): ISO_PARAMS[2][(0, 0)]}
+    sysm = GpuSystem(pos4, Lv)
+    nl = _azplugins.NeighborListGPU(0.4)
+    pp = _azplugins.PotentialPairPerturbedLennardJonesGPU(1)
+    pp.setParams(0, 0, [1.0, 1.0, 0.5])
+    pp.setRCut(0, 0, 3.0)
+    nl.set_r_cut(1, pp.getRCutMatrix())
+    if tiled:
+        nl.set_tiled(True)
+    nl.update(sysm.particles())
+    assert nl.num_builds() == 1
+    moved = pos4.copy()
+    moved[:, :3] += rng.uniform(-0.35, 0.35, size=(pos4.shape[0], 3))
+    moved[:, :3] -= Lv * np.floor(moved[:, :3] / Lv + 0.5)
+    bad = moved.copy()
+    bad[7, 3] = 3.0
+    sysm.pos.copy_(_azp.torch.tensor(bad, device="cuda"))
+    with pytest.raises(_azp.AzpError, match="type index"):
+        nl.update(sysm.particles())
+    sysm.pos[7, 3] = 0.0
+    nl.update(sysm.particles())
+    assert nl.num_builds() == 2
+    N = pos4.shape[0]
+    pitch = (N + 63) // 64 * 64
+    force = _azp.torch.zeros((N, 4), dtype=_azp.torch.float64, device="cuda")
+    vir = _azp.torch.zeros((6, pitch), dtype=_azp.torch.float64, device="cuda")
+    pp.compute(nl.handle, sysm.particles(), _azp.Outputs(force.data_ptr(), vir.data_ptr(), pitch, None), 0)
+    _azp.torch.cuda.synchronize()
+    fo, _, _ = _oracle_iso(2, moved, Lv, 1, params, 3.0, "none")
+    err = rel_err(force.cpu().numpy()[:, :3], fo[:, :3])
+    say(f"[rebuild after error] tiled={tiled} fp64 rel err {err:.2e}")
+    assert err < FP64_TOL

@@ -172,3 +172,32 @@ def test_bad_type_is_reported_by_a_later_call():
     with pytest.raises(_azp.AzpError):
         w.compute(sysm.particles(), out, compute_virial=False)
     assert force[1].abs().sum().item() == 0.0 and force[0, 2].item() != 0.0
+    # the report clears the flag: with the type fixed the calls run normally
+    sysm.pos[1, 3] = 0.0
+    w.compute(sysm.particles(), out, compute_virial=False)
+    w.compute(sysm.particles(), out, compute_virial=False)
+    w.check()
+    assert force[1, 2].item() != 0.0
+
+
+def test_bad_type_on_the_last_call_is_reported_by_check():
+    """Calls that run while an earlier flag copy is in flight still raise the
+    sticky flag; the end-of-run check waits for the stream and reports it."""
+    from azplugins_amd import _azp, _azplugins
+
+    torch = _azp.torch
+    pos4 = np.array([[0.0, 0.0, 1.0, 0.0], [0.0, 0.0, 1.5, 0.0]])
+    w = _azplugins.WallsPotentialLJ93GPU(1)
+    w.setParams(0, [1.0, 1.0, 3.0, 0.0])
+    w.setGeometry([], [], [[0, 0, 0, 0, 0, 1, 1]])
+    sysm = GpuSystem(pos4, np.array([10.0, 10.0, 10.0]))
+    force = torch.zeros((2, 4), dtype=torch.float64, device="cuda")
+    out = _azp.Outputs()
+    out.d_force, out.d_virial, out.virial_pitch, out.d_torque = force.data_ptr(), None, 64, None
+    w.compute(sysm.particles(), out, compute_virial=False)
+    w.check()
+    sysm.pos[1, 3] = 3.0
+    w.compute(sysm.particles(), out, compute_virial=False)
+    with pytest.raises(_azp.AzpError, match="type index"):
+        w.check()
+    w.check()  # reported once, then cleared

@@ -113,6 +113,58 @@ def test_xplor_switches_smoothly_and_falls_back_to_shift(orc):
     assert (e3, f3) == (e4, f4)
 
 
+def _xplor_cases():
+    cases = [(ev, params, rc, 0.6 * rc) for ev, (params, rc) in ISO.items()]
+    cases += [(3, params, rc, r0 + 0.45 * (r1 - r0)) for params, rc, r0, r1 in COLLOID_CASES]
+    return cases
+
+
+@pytest.mark.parametrize("case", range(6))
+def test_xplor_is_the_unique_cubic_switch(orc, case):
+    """xplor multiplies E by a switch S(r) on [r_on, r_cut) and sets
+    F = S f - (dS/dr) E.  The reference has no test of it.  Four conditions
+    fix a polynomial of degree 3 in r^2 uniquely: S(r_on) = 1 and
+    S'(r_on) = 0 (E and F continuous across r_on), S(r_cut) = 0 and
+    S'(r_cut) = 0 (E and F vanish at r_cut).  The ratio E_xplor / E_none
+    sampled inside the switching range must be such a cubic (its fourth
+    divided difference in r^2 vanishes); with FD consistency of F (tested
+    above) that pins the restated switch for every evaluator."""
+    ev, params, rc, ron = _xplor_cases()[case]
+    p = orc.pack(ev, params)
+    rcsq, ronsq = rc * rc, ron * ron
+    eps = 1e-9
+    # continuity across r_on
+    e_in, f_in = pair_energy(orc, ev, p, (ron * (1 - eps)) ** 2, rcsq, ronsq, "xplor")
+    e_out, f_out = pair_energy(orc, ev, p, (ron * (1 + eps)) ** 2, rcsq, ronsq, "xplor")
+    scale_e = max(abs(e_in), 1e-12)
+    scale_f = max(abs(f_in), 1e-12)
+    assert abs(e_out - e_in) < 1e-6 * scale_e + 1e-12
+    assert abs(f_out - f_in) < 1e-5 * scale_f + 1e-10
+    # below r_on the switch is off
+    e_none, f_none = pair_energy(orc, ev, p, (ron * (1 - eps)) ** 2, rcsq, ronsq, "none")
+    assert (e_in, f_in) == (e_none, f_none)
+    # E and F vanish at r_cut
+    e_c, f_c = pair_energy(orc, ev, p, (rc * (1 - 1e-8)) ** 2, rcsq, ronsq, "xplor")
+    e_mid, f_mid = pair_energy(orc, ev, p, (0.5 * (ron + rc)) ** 2, rcsq, ronsq, "xplor")
+    assert abs(e_c) < 1e-6 * abs(e_mid) + 1e-12
+    assert abs(f_c) < 1e-4 * abs(f_mid) + 1e-9
+    # the switch is a cubic in r^2: fourth divided difference of S(r^2) is 0
+    xs = np.linspace(ronsq, rcsq, 7)[1:-1]
+    S = []
+    for x in xs:
+        ex, _ = pair_energy(orc, ev, p, x, rcsq, ronsq, "xplor")
+        en, _ = pair_energy(orc, ev, p, x, rcsq, ronsq, "none")
+        assert en != 0.0
+        S.append(ex / en)
+    S = np.array(S)
+    assert np.all((S > 0.0) & (S < 1.0))
+    d = S.copy()
+    for order in range(1, 5):
+        d = (d[1:] - d[:-1]) / (xs[order:] - xs[:-order])
+    coef3 = ((S[1:] - S[:-1]) / (xs[1:] - xs[:-1]))  # first divided differences, for scale
+    assert abs(d[0]) < 1e-9 * np.abs(coef3).max() / (rcsq - ronsq) ** 3
+
+
 def _many_body(orc, ev, params_by_pair, rc, N=160, L=7.0, ntypes=2, mode="none", seed=1):
     pos = random_fluid(N, L, ntypes, min_sep=0.85 if ev != 1 else 0.95, seed=seed)
     Lv = np.array([L, L, L * 1.1])
