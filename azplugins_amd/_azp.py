@@ -38,7 +38,7 @@ DRIVER_CONSERVATIVE = 2
 DRIVER_ANISO = 3
 
 SHIFT_MODES = {"none": 0, "shift": 1, "xplor": 2}
-PRECISIONS = {"fp64": 0, "mixed": 1}
+PRECISIONS = {"fp64": 0, "mixed": 1, "mixed_acc64": 2}
 
 
 class AzpError(RuntimeError):

@@ -52,15 +52,17 @@ class PairForceField:
         self.evaluator, cls = _EVALUATORS[potential]
         self.ntypes = int(ntypes)
         self.box = _azp.Box(*[float(x) for x in np.broadcast_to(np.asarray(box, dtype=float), (3,))])
-        if precision not in ("fp64", "mixed") or adjoint_precision not in ("fp64", "mixed"):
-            raise ValueError("precision and adjoint_precision must be 'fp64' or 'mixed'")
+        if precision not in ("fp64", "mixed", "mixed_acc64") or adjoint_precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64', 'mixed' or 'mixed_acc64', adjoint_precision 'fp64' "
+                             "or 'mixed'")
         self.mode = mode
         self.precision = precision
-        # The backward pass evaluates the per-pair partials in FP64 by default
-        # also when the forward runs the mixed form: at the 262,144-particle
-        # workload FP32 partials leave the parameter gradients ~1.7e-6 from
-        # FP64 finite differences; FP64 partials are the default so that the
-        # gradients meet 1e-6.
+        # The backward pass evaluates the per-pair partials in FP64 by default.
+        # The forward's own FP32 error still reaches the gradients of a force-
+        # matching loss, scaled by the inverse relative force residual: at the
+        # 262,144-particle workload with targets from 5 % parameter changes the
+        # mixed forward leaves the parameter gradients ~1.7e-6 from FP64 finite
+        # differences, the FP64 forward ~2e-10.
         self.adjoint_precision = adjoint_precision
         self.pair_keys = [(a, b) for a in range(self.ntypes) for b in range(a, self.ntypes)]
         rc = np.zeros(self.ntypes * self.ntypes)
@@ -131,8 +133,8 @@ class _PairForceFunction(torch.autograd.Function):
         ff._nl.update(parts)
         d_params = ff._pack(theta)
         force = torch.zeros((N, 4), dtype=torch.float64, device=pos.device)
-        args = ff._args(pos4, force, "fp64" if ff.precision == "fp64" else "mixed")
-        if ff.precision == "mixed":
+        args = ff._args(pos4, force, ff.precision)
+        if ff.precision != "fp64":
             pos_fixed = torch.empty((N, 4), dtype=torch.int32, device=pos.device)
             _azp.call("azp_gpu_pack_fixed", pos4.data_ptr(), N, ctypes.byref(ff.box), pos_fixed.data_ptr(),
                       _azp.stream_ptr())

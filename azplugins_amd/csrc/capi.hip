@@ -32,7 +32,7 @@ void clear_error()
     }
 
 int pack_fixed(const double* d_pos, uint32_t N, const azp_box& box, int32_t* d_out, hipStream_t stream);
-bool pair_tile_fits(int evaluator, uint32_t ntypes, uint32_t tile_slots);
+bool pair_tile_fits(int evaluator, uint32_t ntypes, uint32_t tile_slots, bool mixed);
 int nlist_get_on_stream(const azp_nlist* nl,
                         hipStream_t s,
                         bool need_plain,
@@ -404,11 +404,13 @@ extern "C" int azp_pair_set_shift_mode(azp_pair* pp, int mode)
 
 extern "C" int azp_pair_set_precision(azp_pair* pp, int precision)
     {
-    if (!pp || (precision != AZP_PREC_FP64 && precision != AZP_PREC_MIXED))
+    if (!pp || (precision != AZP_PREC_FP64 && precision != AZP_PREC_MIXED && precision != AZP_PREC_MIXED_ACC64))
         return set_error(AZP_ERR_INVALID, "bad precision");
-    if (precision == AZP_PREC_MIXED
+    if (precision != AZP_PREC_FP64
         && (pp->evaluator == AZP_EVAL_COLLOID || pp->driver == AZP_DRIVER_DPD_THERMO))
         return set_error(AZP_ERR_UNSUPPORTED, "this potential runs in the plain FP64 form only");
+    if (precision == AZP_PREC_MIXED_ACC64 && pp->driver == AZP_DRIVER_ANISO)
+        return set_error(AZP_ERR_UNSUPPORTED, "the anisotropic pair has the FP64 and mixed forms");
     pp->precision = precision;
     return AZP_OK;
     }
@@ -466,15 +468,15 @@ extern "C" int azp_pair_compute(azp_pair* pp,
     std::memset(&a, 0, sizeof(a));
     a.particles = *p;
     a.out = *out;
-    // the mixed isotropic drivers run over the tiled list when it is there
-    // and its neighbor positions fit in LDS; everything else reads the plain
-    // list (expanded from the tiled one on first use)
+    // the isotropic drivers run over the tiled list when it is there and
+    // their neighbor positions fit in LDS (16-byte slots in the mixed form,
+    // 32-byte FP64 slots in the plain form); DPD and the anisotropic driver
+    // read the plain list (expanded from the tiled one on first use)
     uint32_t tiled = 0, slots = 0;
-    if (pp->precision == AZP_PREC_MIXED
-        && (pp->driver == AZP_DRIVER_PAIR || pp->driver == AZP_DRIVER_CONSERVATIVE))
+    if (pp->driver == AZP_DRIVER_PAIR || pp->driver == AZP_DRIVER_CONSERVATIVE)
         {
         AZP_TRY(azp_nlist_get_tiles(nl, nullptr, nullptr, nullptr, &tiled, &slots));
-        if (tiled != 0 && !pair_tile_fits(pp->evaluator, pp->ntypes, slots))
+        if (tiled != 0 && !pair_tile_fits(pp->evaluator, pp->ntypes, slots, pp->precision != AZP_PREC_FP64))
             tiled = 0;
         }
     AZP_TRY(nlist_get_on_stream(nl, s, tiled == 0, &a.d_n_neigh, &a.d_nlist, &a.nlist_pitch, &a.max_neighbors));
@@ -490,7 +492,7 @@ extern "C" int azp_pair_compute(azp_pair* pp,
     a.kT = pp->kT;
     a.stream = stream;
     // the tiled kernel converts positions itself
-    if (pp->precision == AZP_PREC_MIXED && !p->d_pos_fixed && p->N > 0 && tiled == 0)
+    if (pp->precision != AZP_PREC_FP64 && !p->d_pos_fixed && p->N > 0 && tiled == 0)
         {
         AZP_TRY(pp->d_pos_fixed.reserve((size_t)p->N * 4));
         AZP_TRY(pack_fixed(p->d_pos, p->N, p->box, pp->d_pos_fixed.ptr, s));

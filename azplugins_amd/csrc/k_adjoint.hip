@@ -411,6 +411,8 @@ extern "C" int azp_gpu_pair_adjoint(const azp_pair_args* args,
     ka.grad_pos = reinterpret_cast<double4*>(d_grad_pos);
     ka.partial = d_work;
     hipStream_t s = (hipStream_t)args->stream;
+    if (args->precision == AZP_PREC_MIXED_ACC64)
+        return set_error(AZP_ERR_UNSUPPORTED, "the adjoint has the FP64 and mixed forms");
     const bool mixed = args->precision == AZP_PREC_MIXED;
     int st;
     if (evaluator == AZP_EVAL_EXPANDED_YUKAWA)

@@ -127,6 +127,8 @@ extern "C" int azp_gpu_compute_pair_aniso_forces(const azp_pair_args* args, int 
         return set_error(AZP_ERR_INVALID, "evaluator %d is not anisotropic", evaluator);
     if (!d_params || !p.d_orientation || !args->out.d_torque)
         return set_error(AZP_ERR_INVALID, "aniso needs params, orientations and a torque array");
+    if (args->precision == AZP_PREC_MIXED_ACC64)
+        return set_error(AZP_ERR_UNSUPPORTED, "the anisotropic kernel has the FP64 and mixed forms");
     const bool mixed = args->precision == AZP_PREC_MIXED;
     if (!args->d_nlist)
         return set_error(AZP_ERR_INVALID, "the anisotropic kernel reads the plain list (d_nlist)");

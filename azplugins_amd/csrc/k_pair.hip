@@ -55,20 +55,26 @@ int pack_fixed(const double* d_pos, uint32_t N, const azp_box& box, int32_t* d_o
     return AZP_OK;
     }
 
-// Whether the mixed kernel of `evaluator` can run over a tiled list with
-// `tile_slots` neighbor slots for `ntypes` types.
-bool pair_tile_fits(int evaluator, uint32_t ntypes, uint32_t tile_slots)
+// Whether the kernel of `evaluator` in the given form can run over a tiled
+// list with `tile_slots` neighbor slots for `ntypes` types (its positions and
+// type tables fit in LDS).
+bool pair_tile_fits(int evaluator, uint32_t ntypes, uint32_t tile_slots, bool mixed)
     {
     switch (evaluator)
         {
     case AZP_EVAL_HERTZ:
-        return tile_fits<EvalHertz>(ntypes, tile_slots);
+        return mixed ? tile_fits<EvalHertz>(ntypes, tile_slots) : tile_fits64<EvalHertz>(ntypes, tile_slots);
     case AZP_EVAL_EXPANDED_YUKAWA:
-        return tile_fits<EvalExpandedYukawa>(ntypes, tile_slots);
+        return mixed ? tile_fits<EvalExpandedYukawa>(ntypes, tile_slots)
+                     : tile_fits64<EvalExpandedYukawa>(ntypes, tile_slots);
     case AZP_EVAL_PERTURBED_LJ:
-        return tile_fits<EvalPerturbedLJ>(ntypes, tile_slots);
+        return mixed ? tile_fits<EvalPerturbedLJ>(ntypes, tile_slots)
+                     : tile_fits64<EvalPerturbedLJ>(ntypes, tile_slots);
+    case AZP_EVAL_COLLOID:
+        return !mixed && tile_fits64<EvalColloid>(ntypes, tile_slots);
     case AZP_EVAL_DPD_GENERAL_WEIGHT:
-        return tile_fits<EvalDPDConservative>(ntypes, tile_slots);
+        return mixed ? tile_fits<EvalDPDConservative>(ntypes, tile_slots)
+                     : tile_fits64<EvalDPDConservative>(ntypes, tile_slots);
     default:
         return false;
         }
@@ -160,7 +166,7 @@ int launcher_heap_scratch(uint32_t N, int4** out)
 // Where the launcher puts fixed-point positions it has to pack itself.
 enum class PackTarget
     {
-    Heap,
+    Heap, // library-owned device buffer, grown with hipMalloc
     };
 
 static int compute_pair_forces(const azp_pair_args* args, int evaluator, const void* d_params, PackTarget target)
@@ -171,7 +177,10 @@ static int compute_pair_forces(const azp_pair_args* args, int evaluator, const v
         return AZP_OK;
     if (!d_params)
         return set_error(AZP_ERR_INVALID, "null params");
-    const bool mixed = args->precision == AZP_PREC_MIXED;
+    if (args->precision != AZP_PREC_FP64 && args->precision != AZP_PREC_MIXED
+        && args->precision != AZP_PREC_MIXED_ACC64)
+        return set_error(AZP_ERR_INVALID, "unknown precision %d", args->precision);
+    const bool mixed = args->precision != AZP_PREC_FP64;
     if (args->shift_mode == AZP_SHIFT_XPLOR && !args->d_ronsq)
         return set_error(AZP_ERR_INVALID, "xplor needs d_ronsq");
 
@@ -203,6 +212,7 @@ static int compute_pair_forces(const azp_pair_args* args, int evaluator, const v
     ka.tile_order = args->d_tile_list ? args->d_tile_order : nullptr;
     ka.tile_nn = args->d_tile_list ? args->d_tile_nn : nullptr;
     ka.head_list = args->d_head_list;
+    ka.acc64 = args->precision == AZP_PREC_MIXED_ACC64;
     if (args->d_head_list && (args->d_tile_list || !args->d_nlist))
         return set_error(AZP_ERR_INVALID, "HOOMD's head list goes with d_nlist and without a tiled list");
     unsigned tile = 0, slots = 0;
@@ -228,7 +238,7 @@ static int compute_pair_forces(const azp_pair_args* args, int evaluator, const v
     hipStream_t s = (hipStream_t)args->stream;
     // the mixed form over the plain list reads fixed-point positions: packed
     // here into the launcher's scratch when the caller brings none
-    const bool tiled_ok = tile != 0 && pair_tile_fits(evaluator, args->ntypes, slots);
+    const bool tiled_ok = tile != 0 && pair_tile_fits(evaluator, args->ntypes, slots, mixed);
     if (mixed && !p.d_pos_fixed && !tiled_ok && args->d_nlist && evaluator != AZP_EVAL_COLLOID)
         {
         int4* pf = nullptr;
@@ -243,31 +253,20 @@ static int compute_pair_forces(const azp_pair_args* args, int evaluator, const v
         }
     if (args->d_head_list)
         return launch_pair_head(ka, evaluator, mixed, vir, block, s);
+    if (tiled_ok)
+        return launch_pair_tiled(ka, evaluator, mixed, vir, tile, slots, s);
+    if (!args->d_nlist)
+        return set_error(AZP_ERR_INVALID,
+                         "the tiled list's %u neighbor slots and the type tables exceed the LDS of a workgroup; "
+                         "pass the plain list",
+                         slots);
+    return launch_pair_column(ka, evaluator, mixed, vir, block, s);
+    }
 
-    switch (evaluator)
-        {
-    case AZP_EVAL_HERTZ:
-        return mixed ? launch_eval<EvalHertz, true>(ka, vir, block, tile, slots, s)
-                     : launch_eval<EvalHertz, false>(ka, vir, block, 0, 0, s);
-    case AZP_EVAL_EXPANDED_YUKAWA:
-        return mixed ? launch_eval<EvalExpandedYukawa, true>(ka, vir, block, tile, slots, s)
-                     : launch_eval<EvalExpandedYukawa, false>(ka, vir, block, 0, 0, s);
-    case AZP_EVAL_PERTURBED_LJ:
-        return mixed ? launch_eval<EvalPerturbedLJ, true>(ka, vir, block, tile, slots, s)
-                     : launch_eval<EvalPerturbedLJ, false>(ka, vir, block, 0, 0, s);
-    case AZP_EVAL_COLLOID:
-        if (mixed)
-            return set_error(AZP_ERR_UNSUPPORTED,
-                             "Colloid runs in the plain FP64 form only (its colloid-colloid "
-                             "branch cancels catastrophically in FP32)");
-        return launch_eval<EvalColloid, false>(ka, vir, block, 0, 0, s);
-    case AZP_EVAL_DPD_GENERAL_WEIGHT:
-        // PotentialPairConservative<DPDPairEvaluatorGeneralWeight>
-        return mixed ? launch_eval<EvalDPDConservative, true>(ka, vir, block, tile, slots, s)
-                     : launch_eval<EvalDPDConservative, false>(ka, vir, block, 0, 0, s);
-    default:
-        return set_error(AZP_ERR_INVALID, "evaluator %d is not an isotropic pair evaluator", evaluator);
-        }
+int launch_pair_column(const PairKernelArgs& ka, int evaluator, bool mixed, bool virial, unsigned block,
+                       hipStream_t s)
+    {
+    return launch_pair_evaluator<kLayoutColumn>(ka, evaluator, mixed, virial, block, 0, s);
     }
 
 } // namespace azp
@@ -276,4 +275,3 @@ extern "C" int azp_gpu_compute_pair_forces(const azp_pair_args* args, int evalua
     {
     return compute_pair_forces(args, evaluator, d_params, PackTarget::Heap);
     }
-

@@ -17,29 +17,7 @@ namespace azp
 int launch_pair_head(const PairKernelArgs& ka, int evaluator, bool mixed, bool virial, unsigned block,
                      hipStream_t s)
     {
-    switch (evaluator)
-        {
-    case AZP_EVAL_HERTZ:
-        return mixed ? launch_eval_head<EvalHertz, true>(ka, virial, block, s)
-                     : launch_eval_head<EvalHertz, false>(ka, virial, block, s);
-    case AZP_EVAL_EXPANDED_YUKAWA:
-        return mixed ? launch_eval_head<EvalExpandedYukawa, true>(ka, virial, block, s)
-                     : launch_eval_head<EvalExpandedYukawa, false>(ka, virial, block, s);
-    case AZP_EVAL_PERTURBED_LJ:
-        return mixed ? launch_eval_head<EvalPerturbedLJ, true>(ka, virial, block, s)
-                     : launch_eval_head<EvalPerturbedLJ, false>(ka, virial, block, s);
-    case AZP_EVAL_COLLOID:
-        if (mixed)
-            return set_error(AZP_ERR_UNSUPPORTED,
-                             "Colloid runs in the plain FP64 form only (its colloid-colloid "
-                             "branch cancels catastrophically in FP32)");
-        return launch_eval_head<EvalColloid, false>(ka, virial, block, s);
-    case AZP_EVAL_DPD_GENERAL_WEIGHT:
-        return mixed ? launch_eval_head<EvalDPDConservative, true>(ka, virial, block, s)
-                     : launch_eval_head<EvalDPDConservative, false>(ka, virial, block, s);
-    default:
-        return set_error(AZP_ERR_INVALID, "evaluator %d is not an isotropic pair evaluator", evaluator);
-        }
+    return launch_pair_evaluator<kLayoutHead>(ka, evaluator, mixed, virial, block, 0, s);
     }
 
 } // namespace azp

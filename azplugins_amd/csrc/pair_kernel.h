@@ -69,6 +69,8 @@ struct PairKernelArgs
     uint32_t tile_slots; // LDS position slots of the tiled kernel (the tables follow them)
     // head-list layout: particle i's entries at nlist[head_list[i] + k], k < n_neigh[i]
     const uint64_t* head_list;
+    // mixed form: add every entry's terms to the FP64 accumulators
+    bool acc64;
     };
 
 // Position (double4 x, y, z, type) -> wrapped 32-bit fixed point of the box.
@@ -354,10 +356,20 @@ template<class Eval, typename R> __host__ __device__ constexpr size_t table_byte
 // LDS the tiled kernel may use for its type-pair tables and neighbor slots.
 constexpr size_t kTileLds = 64 * 1024;
 
+// LDS of the FP64 form over the tiled list: 32-byte position slots, up to the
+// 160 KiB a gfx950 workgroup may use.
+constexpr size_t kTileLds64 = 160 * 1024;
+
 template<class Eval> static bool tile_fits(uint32_t ntypes, uint32_t slots)
     {
     const size_t tables = ntypes == 1 ? 0 : table_bytes<Eval, float>(ntypes * ntypes);
     return tables + (size_t)slots * sizeof(int4) <= kTileLds;
+    }
+
+template<class Eval> static bool tile_fits64(uint32_t ntypes, uint32_t slots)
+    {
+    const size_t tables = ntypes == 1 ? 0 : table_bytes<Eval, double>(ntypes * ntypes);
+    return tables + (size_t)slots * sizeof(double4) <= kTileLds64;
     }
 
 // Neighbor-list layouts a kernel instance reads.
@@ -365,7 +377,10 @@ constexpr int kLayoutColumn = 0; // nlist[k * pitch + i], padded with i
 constexpr int kLayoutTiled = 1;  // per-tile 16-bit entries (mixed form)
 constexpr int kLayoutHead = 2;   // HOOMD's head list: nlist[head_list[i] + k], k < n_neigh[i]
 
-template<class Eval, bool MIXED, bool VIRIAL, bool SINGLE, bool XPLOR, bool CUBIC, int LAYOUT>
+// ACC64 (mixed form only): every list entry's FP32 terms are added to the
+// FP64 accumulators as they come, instead of being summed in FP32 over 32
+// entries first.
+template<class Eval, bool MIXED, bool VIRIAL, bool SINGLE, bool XPLOR, bool CUBIC, int LAYOUT, bool ACC64>
 __global__ void __launch_bounds__(512) k_pair(const PairKernelArgs a)
     {
     constexpr bool TILED = LAYOUT == kLayoutTiled;
@@ -376,9 +391,12 @@ __global__ void __launch_bounds__(512) k_pair(const PairKernelArgs a)
     extern __shared__ double smem[];
     const uint32_t nt = a.ntypes;
     const uint32_t nt2 = nt * nt;
-    // tiled: the neighbor positions sit at LDS offset 0 (so a list entry,
-    // 16 x rank, is their byte address) and the type-pair tables after them
-    Prep* s_prep = reinterpret_cast<Prep*>(reinterpret_cast<char*>(smem) + (TILED ? a.tile_slots * sizeof(int4) : 0));
+    // tiled: the neighbor positions sit at LDS offset 0 (a list entry,
+    // 16 x rank, is their byte address in the mixed form's 16-byte slots and
+    // half of it in the FP64 form's 32-byte slots) and the type-pair tables
+    // after them
+    using SlotT = std::conditional_t<MIXED, int4, double4>;
+    Prep* s_prep = reinterpret_cast<Prep*>(reinterpret_cast<char*>(smem) + (TILED ? a.tile_slots * sizeof(SlotT) : 0));
     TypePairExtra<R>* s_x = reinterpret_cast<TypePairExtra<R>*>(s_prep + nt2);
 
     Prep q0;
@@ -397,7 +415,7 @@ __global__ void __launch_bounds__(512) k_pair(const PairKernelArgs a)
     int4* s_pos = nullptr;
     const uint32_t* tile_js = nullptr;
     uint32_t tile_m = 0;
-    if constexpr (TILED)
+    if constexpr (TILED && MIXED)
         {
         // stage the tile's neighbor positions, converted to fixed point here
         // (block lb is tile lb); every lane of the workgroup takes part before
@@ -409,6 +427,17 @@ __global__ void __launch_bounds__(512) k_pair(const PairKernelArgs a)
             s_pos[m] = fixed4(a.pos[tile_js[m]], a.box);
         if (threadIdx.x == 0)
             s_pos[tile_m] = make_int4(0, 0, 0, 0);
+        }
+    if constexpr (TILED && !MIXED)
+        {
+        // FP64 form: the FP64 positions themselves (32-byte slots)
+        double4* s_pos64 = reinterpret_cast<double4*>(smem);
+        tile_js = a.tile_jset + (size_t)lb * kTileCap;
+        tile_m = a.tile_count[lb];
+        for (uint32_t m = threadIdx.x; m < tile_m; m += blockDim.x)
+            s_pos64[m] = a.pos[tile_js[m]];
+        if (threadIdx.x == 0)
+            s_pos64[tile_m] = make_double4(0.0, 0.0, 0.0, 0.0);
         }
     if (!SINGLE || TILED)
         __syncthreads();
@@ -428,7 +457,51 @@ __global__ void __launch_bounds__(512) k_pair(const PairKernelArgs a)
     const size_t stride = a.nlist_pitch;
 
     Acc acc;
-    if constexpr (!MIXED)
+    if constexpr (!MIXED && TILED)
+        {
+        // FP64 over the tiled list: the FP64 positions from LDS; the same
+        // terms in the same order as over the plain list, so the results are
+        // bit-identical to the plain form over the same entries
+        const double4 pi = a.pos[pe];
+        const uint32_t ti = (uint32_t)pi.w;
+        const uint32_t col_off = ie * 4u;
+        const uint32_t tile_m16 = tile_m << 4; // the padding entry
+        const char* s_base = reinterpret_cast<const char*>(smem);
+        for (uint32_t k = 0; k < P; k += U)
+            {
+            uint32_t w[U / 2];
+#pragma unroll
+            for (uint32_t u = 0; u < U / 2; ++u)
+                {
+                const char* row = reinterpret_cast<const char*>(a.tile_list + (size_t)((k >> 1) + u) * stride);
+                w[u] = *reinterpret_cast<const uint32_t*>(row + col_off);
+                }
+            uint32_t e[U];
+#pragma unroll
+            for (uint32_t u = 0; u < U / 2; ++u)
+                {
+                e[2 * u] = w[u] & 0xffffu;
+                e[2 * u + 1] = w[u] >> 16;
+                }
+            double4 pj[U];
+#pragma unroll
+            for (uint32_t u = 0; u < U; ++u)
+                pj[u] = *reinterpret_cast<const double4*>(s_base + 2u * e[u]);
+#pragma unroll
+            for (uint32_t u = 0; u < U; ++u)
+                {
+                const double dx = min_image_1d(pi.x - pj[u].x, a.box.L[0], a.box.Linv[0]);
+                const double dy = min_image_1d(pi.y - pj[u].y, a.box.L[1], a.box.Linv[1]);
+                const double dz = min_image_1d(pi.z - pj[u].z, a.box.L[2], a.box.Linv[2]);
+                const uint32_t tp = SINGLE ? 0u : ti * nt + (uint32_t)pj[u].w;
+                term_fp64<Eval, VIRIAL, XPLOR>(SINGLE ? q0 : s_prep[tp], SINGLE ? x0 : s_x[tp],
+                                               e[u] != tile_m16, dx, dy, dz, acc);
+                }
+            }
+        if (i < a.N)
+            store<VIRIAL>(a, pe, acc);
+        }
+    else if constexpr (!MIXED)
         {
         const double4 pi = a.pos[ie];
         const uint32_t ti = (uint32_t)pi.w;
@@ -497,8 +570,13 @@ __global__ void __launch_bounds__(512) k_pair(const PairKernelArgs a)
                                                        e[u] != tile_m16, pe, e[u] >> 4, tile_js,
                                                        fixed_delta(pi.x, pj[u].x), fixed_delta(pi.y, pj[u].y),
                                                        fixed_delta(pi.z, pj[u].z), part);
+                if (ACC64)
+                    {
+                    flush32<VIRIAL>(acc, part);
+                    part = Acc32();
+                    }
                 }
-            if (((k + U) & (kFlushEntries - 1)) == 0 || k + U >= P)
+            if (!ACC64 && (((k + U) & (kFlushEntries - 1)) == 0 || k + U >= P))
                 {
                 flush32<VIRIAL>(acc, part);
                 part = Acc32();
@@ -545,8 +623,13 @@ __global__ void __launch_bounds__(512) k_pair(const PairKernelArgs a)
                                                        jj[u] != ie, ie, jj[u], nullptr, fixed_delta(pi.x, pj[u].x),
                                                        fixed_delta(pi.y, pj[u].y), fixed_delta(pi.z, pj[u].z),
                                                        part);
+                if (ACC64)
+                    {
+                    flush32<VIRIAL>(acc, part);
+                    part = Acc32();
+                    }
                 }
-            if (((k + U) & (kFlushEntries - 1)) == 0 || k + U >= P)
+            if (!ACC64 && (((k + U) & (kFlushEntries - 1)) == 0 || k + U >= P))
                 {
                 flush32<VIRIAL>(acc, part);
                 part = Acc32();
@@ -568,12 +651,33 @@ static int launch_t(const PairKernelArgs& ka, unsigned block, unsigned slots, hi
     PairKernelArgs kb = ka;
     kb.tile_slots = TILED ? slots : 0u;
     if (TILED)
-        shmem += (size_t)slots * sizeof(int4);
-    if (shmem > kTileLds)
+        shmem += (size_t)slots * (MIXED ? sizeof(int4) : sizeof(double4));
+    const size_t cap = (TILED && !MIXED) ? kTileLds64 : kTileLds;
+    if (shmem > cap)
         return set_error(AZP_ERR_UNSUPPORTED, "too many particle types (%u) for LDS staging", ka.ntypes);
+    if (shmem > 64 * 1024)
+        {
+        // more than 64 KiB of dynamic LDS (gfx950 gives a workgroup 160 KiB)
+        static bool raised = false;
+        if (!raised)
+            {
+            // only the FP64 tiled form (ACC64 false) stages more than 64 KiB
+            const hipError_t e = hipFuncSetAttribute(
+                reinterpret_cast<const void*>(&k_pair<Eval, MIXED, VIRIAL, SINGLE, XPLOR, CUBIC, LAYOUT, false>),
+                hipFuncAttributeMaxDynamicSharedMemorySize, (int)cap);
+            if (e != hipSuccess)
+                return set_error(AZP_ERR_HIP, "hipFuncSetAttribute (dynamic LDS %zu bytes) failed: %s", cap,
+                                 hipGetErrorString(e));
+            raised = true;
+            }
+        }
     const unsigned grid = grid_for(ka.N, block);
-    hipLaunchKernelGGL((k_pair<Eval, MIXED, VIRIAL, SINGLE, XPLOR, CUBIC, LAYOUT>), dim3(grid), dim3(block), shmem,
-                       stream, kb);
+    if (MIXED && ka.acc64)
+        hipLaunchKernelGGL((k_pair<Eval, MIXED, VIRIAL, SINGLE, XPLOR, CUBIC, LAYOUT, MIXED>), dim3(grid), dim3(block),
+                           shmem, stream, kb);
+    else
+        hipLaunchKernelGGL((k_pair<Eval, MIXED, VIRIAL, SINGLE, XPLOR, CUBIC, LAYOUT, false>), dim3(grid), dim3(block),
+                           shmem, stream, kb);
     AZP_LAUNCH_CHECK();
     return AZP_OK;
     }
@@ -609,44 +713,55 @@ static int launch_virial(const PairKernelArgs& ka, bool virial, unsigned block, 
                   : launch_mode<Eval, MIXED, false, false, LAYOUT>(ka, block, slots, s);
     }
 
-// The tiled list serves the mixed form when the tile tables fit next to the
-// type-pair tables in 64 KB of LDS; otherwise the plain list is used.
-template<class Eval, bool MIXED>
-static int launch_eval(const PairKernelArgs& ka,
-                       bool virial,
-                       unsigned block,
-                       unsigned tile_size,
-                       unsigned tile_slots,
-                       hipStream_t s)
+// One layout's kernels for the evaluator code: the per-layout translation
+// units (k_pair.hip column, k_pair_tiled.hip tiled, k_pair_head.hip head list)
+// each instantiate this for their layout.
+template<int LAYOUT>
+static int launch_pair_evaluator(const PairKernelArgs& ka,
+                                 int evaluator,
+                                 bool mixed,
+                                 bool virial,
+                                 unsigned block,
+                                 unsigned slots,
+                                 hipStream_t s)
     {
-    if constexpr (MIXED)
+    if (mixed && LAYOUT != kLayoutTiled && !ka.pos_fixed)
+        return set_error(AZP_ERR_INVALID, "the mixed form over the plain list needs fixed-point positions");
+    switch (evaluator)
         {
-        if (tile_size && ka.tile_list && tile_fits<Eval>(ka.ntypes, tile_slots))
-            return launch_virial<Eval, true, kLayoutTiled>(ka, virial, tile_size, tile_slots, s);
-        if (!ka.nlist)
-            return set_error(AZP_ERR_INVALID,
-                             "the tiled list's %u neighbor slots and the type tables exceed %u bytes of LDS; "
-                             "pass the plain list",
-                             tile_slots,
-                             (unsigned)kTileLds);
-        if (!ka.pos_fixed)
-            return set_error(AZP_ERR_INVALID, "mixed precision without a tiled list needs d_pos_fixed");
+    case AZP_EVAL_HERTZ:
+        return mixed ? launch_virial<EvalHertz, true, LAYOUT>(ka, virial, block, slots, s)
+                     : launch_virial<EvalHertz, false, LAYOUT>(ka, virial, block, slots, s);
+    case AZP_EVAL_EXPANDED_YUKAWA:
+        return mixed ? launch_virial<EvalExpandedYukawa, true, LAYOUT>(ka, virial, block, slots, s)
+                     : launch_virial<EvalExpandedYukawa, false, LAYOUT>(ka, virial, block, slots, s);
+    case AZP_EVAL_PERTURBED_LJ:
+        return mixed ? launch_virial<EvalPerturbedLJ, true, LAYOUT>(ka, virial, block, slots, s)
+                     : launch_virial<EvalPerturbedLJ, false, LAYOUT>(ka, virial, block, slots, s);
+    case AZP_EVAL_COLLOID:
+        if (mixed)
+            return set_error(AZP_ERR_UNSUPPORTED,
+                             "Colloid runs in the plain FP64 form only (its colloid-colloid "
+                             "branch cancels catastrophically in FP32)");
+        return launch_virial<EvalColloid, false, LAYOUT>(ka, virial, block, slots, s);
+    case AZP_EVAL_DPD_GENERAL_WEIGHT:
+        // PotentialPairConservative<DPDPairEvaluatorGeneralWeight>
+        return mixed ? launch_virial<EvalDPDConservative, true, LAYOUT>(ka, virial, block, slots, s)
+                     : launch_virial<EvalDPDConservative, false, LAYOUT>(ka, virial, block, slots, s);
+    default:
+        return set_error(AZP_ERR_INVALID, "evaluator %d is not an isotropic pair evaluator", evaluator);
         }
-    return launch_virial<Eval, MIXED, kLayoutColumn>(ka, virial, block, 0, s);
     }
 
-// Head-list form: every instance reads HOOMD's unpadded rows; the mixed form
-// reads fixed-point positions from pos_fixed.
-template<class Eval, bool MIXED>
-static int launch_eval_head(const PairKernelArgs& ka, bool virial, unsigned block, hipStream_t s)
-    {
-    if (MIXED && !ka.pos_fixed)
-        return set_error(AZP_ERR_INVALID, "the mixed form needs fixed-point positions");
-    return launch_virial<Eval, MIXED, kLayoutHead>(ka, virial, block, 0, s);
-    }
-
-// Launches the head-list form (d_head_list given): defined in k_pair_head.hip.
+// Defined in k_pair.hip, k_pair_tiled.hip and k_pair_head.hip.  The tiled
+// form runs workgroups of `block` = the tile size over `slots` LDS slots.
+int launch_pair_column(const PairKernelArgs& ka, int evaluator, bool mixed, bool virial, unsigned block,
+                       hipStream_t s);
+int launch_pair_tiled(const PairKernelArgs& ka, int evaluator, bool mixed, bool virial, unsigned tile,
+                      unsigned slots, hipStream_t s);
 int launch_pair_head(const PairKernelArgs& ka, int evaluator, bool mixed, bool virial, unsigned block,
                      hipStream_t s);
+// Whether the tiled kernel of the evaluator fits its positions and tables in LDS.
+bool pair_tile_fits(int evaluator, uint32_t ntypes, uint32_t tile_slots, bool mixed);
 
 } // namespace azp

@@ -59,11 +59,11 @@ class NeighborList:
         return nt, m
 
     def _wants_tiles(self):
-        """The tiled list serves isotropic pair forces in the mixed form."""
+        """The tiled list serves isotropic pair forces in both forms."""
         from ... import _azp
 
-        return any(getattr(u, "_precision", "fp64") == "mixed" and u._cpp is not None
-                   and u._cpp.driver in (_azp.DRIVER_PAIR, _azp.DRIVER_CONSERVATIVE) for u in self._users)
+        return any(u._cpp is not None and u._cpp.driver in (_azp.DRIVER_PAIR, _azp.DRIVER_CONSERVATIVE)
+                   for u in self._users)
 
     def _needs_update(self, timestep):
         """Whether this step's update (rebuild check) is still due; pushes the

@@ -10,7 +10,7 @@ registers the r_cut matrix with the neighbor list.
 from ...data import RCutDict, TypeParameter, TypeParameterDict  # noqa: F401
 from ..force import Force
 
-_PRECISIONS = ("fp64", "mixed")
+_PRECISIONS = ("fp64", "mixed", "mixed_acc64")
 
 
 class Pair(Force):
@@ -47,7 +47,9 @@ class Pair(Force):
 
     @property
     def precision(self):
-        """'fp64' (plain form) or 'mixed' (FP32 pair math, FP64 accumulation)."""
+        """'fp64' (plain form), 'mixed' (FP32 pair math, FP32 sums of 32 entries
+        added to FP64 accumulators) or 'mixed_acc64' (FP32 pair math, every
+        entry added to FP64 accumulators; isotropic potentials)."""
         return self._precision
 
     @precision.setter

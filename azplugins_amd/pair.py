@@ -3,8 +3,11 @@
 Same class names, constructor signatures, parameter keys, accepted modes and
 ``_cpp_class_name`` strings as the reference; the C++ objects they select are
 the gfx950 implementations in ``_azplugins``.  One addition: every class has a
-``precision`` attribute, ``"fp64"`` (default, the plain form) or ``"mixed"``
-(FP32 pair arithmetic with FP64 accumulation).
+``precision`` attribute: ``"fp64"`` (default, the plain form), ``"mixed"``
+(FP32 pair arithmetic; the terms of 32 list entries are summed in FP32 and
+each such sum is added to FP64 accumulators) or, for the isotropic
+potentials, ``"mixed_acc64"`` (FP32 pair arithmetic, every entry's terms added
+to FP64 accumulators).
 """
 
 from . import _azplugins

@@ -79,8 +79,11 @@ enum azp_shift_mode
 
 enum azp_precision
     {
-    AZP_PREC_FP64 = 0, /* plain form: FP64 everywhere */
-    AZP_PREC_MIXED = 1 /* FP32 pair arithmetic, FP64 accumulation */
+    AZP_PREC_FP64 = 0,       /* plain form: FP64 everywhere */
+    AZP_PREC_MIXED = 1,      /* FP32 pair arithmetic; the terms of 32 list entries are summed in FP32,
+                              * each such sum is added to FP64 accumulators */
+    AZP_PREC_MIXED_ACC64 = 2 /* FP32 pair arithmetic, every entry's terms added to FP64 accumulators
+                              * (isotropic pair launcher only) */
     };
 
 typedef struct

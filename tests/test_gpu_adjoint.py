@@ -187,7 +187,13 @@ def test_yukawa_262k_gradients_vs_gpu_fd():
         fdp.append(d / (2 * h))
     fdp = np.array(fdp)
     say("[adjoint 262k] backward")
-    bounds = {("fp64", "fp64"): (1e-6, 1e-6), ("mixed", "fp64"): (1e-6, 2e-5), ("mixed", "mixed"): (5e-6, 2e-5)}
+    # Only the FP64 forward meets 1e-6: the loss weighs each force error by
+    # the force residual, about 5 % of the force here, so the mixed form's
+    # FP32 force error (~1e-7 per pair) reaches the gradient ~20 times larger
+    # (1.75e-6 measured with the FP64 adjoint); bounds are a few times the
+    # measured errors
+    bounds = {("fp64", "fp64"): (1e-6, 1e-6), ("mixed_acc64", "fp64"): (5e-6, 2e-5),
+              ("mixed", "fp64"): (5e-6, 2e-5), ("mixed", "mixed"): (5e-6, 2e-5)}
     for (precision, adjoint), (pbound, xbound) in bounds.items():
         ffp = PairForceField("ExpandedYukawa", 2, Lv, 4.0, precision=precision, adjoint_precision=adjoint)
         th = torch.tensor(theta, device="cuda", requires_grad=True)

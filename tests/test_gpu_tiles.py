@@ -332,3 +332,87 @@ def test_fused_walk_fallbacks():
     say(f"[fused] nine types: walk kind {tiled.nl.walk_kind()}, tile size {tiled.nl.get_tiles()[3]}")
     assert tiled.nl.walk_kind() == 0
     assert np.array_equal(plain.force, tiled.force)
+
+
+def _fp64_pair(cls, sysm, nt, params, rc, **kw):
+    plain = gpu_pair(cls, sysm, nt, params, rc, **kw)
+    tiled = gpu_pair(cls, sysm, nt, params, rc, tiled=True, **kw)
+    return plain, tiled
+
+
+@pytest.mark.parametrize("mode", ["none", "shift", "xplor"])
+def test_fp64_tiled_equals_plain_and_oracle(mode):
+    """The FP64 form over the tiled list reads FP64 positions from LDS (32-byte
+    slots): the same terms in the same order as over the plain list, so the
+    results are bit-identical, and within 1e-12 of the oracle."""
+    L = 16.0
+    pos4, _ = jittered_lattice(16, L, 0.15, 2, seed=43)
+    Lv = np.array([L, L * 1.02, L])
+    pos4[:, :3] -= Lv * np.floor(pos4[:, :3] / Lv + 0.5)
+    sysm = GpuSystem(pos4, Lv)
+    yk = {(0, 0): {"epsilon": 1.0, "kappa": 2.0, "delta": 0.2}, (0, 1): {"epsilon": 0.7, "kappa": 1.5, "delta": 0.3},
+          (1, 1): {"epsilon": 1.3, "kappa": 2.5, "delta": 0.1}}
+    plj = {(0, 0): {"epsilon": 1.0, "sigma": 1.0, "attraction_scale_factor": 0.5},
+           (0, 1): {"epsilon": 0.8, "sigma": 0.9, "attraction_scale_factor": 0.3},
+           (1, 1): {"epsilon": 1.2, "sigma": 0.95, "attraction_scale_factor": 0.7}}
+    hz = {(0, 0): {"epsilon": 1.0}, (0, 1): {"epsilon": 2.0}, (1, 1): {"epsilon": 0.5}}
+    cases = (("PotentialPairExpandedYukawa", oracle.EXPANDED_YUKAWA, yk, 3.0),
+             ("PotentialPairPerturbedLennardJones", oracle.PERTURBED_LJ, plj, 2.5),
+             ("PotentialPairHertz", oracle.HERTZ, hz, 1.3))
+    for cls, ev, params, rc in cases:
+        if cls == "PotentialPairHertz" and mode != "none":
+            continue
+        r_on = 0.7 * rc if mode == "xplor" else None
+        plain, tiled = _fp64_pair(cls, sysm, 2, params, rc, mode=mode, r_on=r_on)
+        size = tiled.nl.get_tiles()[3]
+        say(f"[tiles fp64] {cls} {mode}: tile size {size}")
+        assert size in (256, 512)
+        assert np.array_equal(plain.force, tiled.force), cls
+        assert np.array_equal(plain.virial, tiled.virial), cls
+        rcm = np.full(4, rc)
+        pairs = oracle.nlist(pos4, Lv, 2, oracle.r_listsq_matrix(rcm, 0.4), "cell")
+        fo, vo = oracle.pair_forces(ev, pos4, Lv, pairs, 2, oracle.pack_matrix(ev, params, 2), rcm * rcm,
+                                    (0.7 * rcm) ** 2, mode)
+        err = max(rel_err(tiled.force[:, :3], fo[:, :3]), rel_err(tiled.force[:, 3], fo[:, 3]),
+                  rel_err(tiled.virial, vo))
+        say(f"  fp64 tiled vs oracle {err:.2e}")
+        assert err < 1e-12
+
+
+def test_fp64_tiled_colloid():
+    """Colloid (FP64 only) over the tiled list equals the plain list."""
+    from test_gpu_parity import ISO_PARAMS, ISO_RCUT, _colloid_positions
+
+    L = 26.0
+    pos4 = _colloid_positions(12, L, seed=13)
+    Lv = np.array([L, L, L])
+    sysm = GpuSystem(pos4, Lv)
+    plain, tiled = _fp64_pair("PotentialPairColloid", sysm, 2, ISO_PARAMS[3], ISO_RCUT[3])
+    say(f"[tiles fp64] colloid tile size {tiled.nl.get_tiles()[3]}")
+    assert np.array_equal(plain.force, tiled.force)
+    assert np.array_equal(plain.virial, tiled.virial)
+
+
+def test_mixed_acc64_tiled_equals_plain():
+    """FP32 pair arithmetic with every entry's terms added to the FP64
+    accumulators: tiled and plain lists give the same bits; the force error
+    against the oracle is within the mixed bound and not larger than the
+    FP32-partial-sum form's."""
+    L = 20.0
+    pos4, _ = jittered_lattice(20, L, 0.1, 1, seed=21)
+    Lv = np.array([L, L, L])
+    sysm = GpuSystem(pos4, Lv)
+    plain = gpu_pair("PotentialPairPerturbedLennardJones", sysm, 1, PLJ, 3.0, precision="mixed_acc64")
+    tiled = gpu_pair("PotentialPairPerturbedLennardJones", sysm, 1, PLJ, 3.0, precision="mixed_acc64", tiled=True)
+    part = gpu_pair("PotentialPairPerturbedLennardJones", sysm, 1, PLJ, 3.0, precision="mixed", tiled=True)
+    assert tiled.nl.get_tiles()[3] in (256, 512)
+    assert np.array_equal(plain.force, tiled.force)
+    assert np.array_equal(plain.virial, tiled.virial)
+    pairs = oracle.nlist(pos4, Lv, 1, oracle.r_listsq_matrix([3.0], 0.4), "cell")
+    fo, vo = oracle.pair_forces(oracle.PERTURBED_LJ, pos4, Lv, pairs, 1,
+                                oracle.pack_matrix(oracle.PERTURBED_LJ, PLJ, 1), np.array([9.0]))
+    e64 = rel_err(tiled.force[:, :3], fo[:, :3])
+    e32 = rel_err(part.force[:, :3], fo[:, :3])
+    say(f"[tiles] mixed_acc64 force rel err {e64:.2e}, mixed (32-entry FP32 sums) {e32:.2e}")
+    assert e64 < 1e-5 and rel_err(tiled.virial, vo) < 1e-5
+    assert e64 <= 1.5 * e32
