@@ -159,6 +159,7 @@ SIGNATURES = [
     ("azp_param_unpack", _i, [_i, _vp, _pd]),
     ("azp_class_lookup", _i, [ctypes.c_char_p, ctypes.POINTER(_i), ctypes.POINTER(_i)]),
     ("azp_gpu_compute_pair_forces", _i, [ctypes.POINTER(PairArgs), _i, _vp]),
+    ("azp_gpu_compute_pair_forces_noalloc", _i, [ctypes.POINTER(PairArgs), _i, _vp]),
     ("azp_gpu_compute_dpd_forces", _i, [ctypes.POINTER(PairArgs), _vp]),
     ("azp_gpu_compute_pair_aniso_forces", _i, [ctypes.POINTER(PairArgs), _i, _vp]),
     ("azp_gpu_pack_fixed", _i, [_vp, _u32, ctypes.POINTER(Box), _vp, _vp]),

@@ -163,10 +163,48 @@ int launcher_heap_scratch(uint32_t N, int4** out)
     return AZP_OK;
     }
 
+// Scratch of azp_gpu_compute_pair_forces_noalloc: a file-scope device array
+// of kNoAllocMaxN int4 (16 MiB on every device the library runs on, held
+// until the process ends).  In host code `scratch` names a host-side
+// stand-in, not device memory: its device address comes from
+// hipGetSymbolAddress, once per device with that device current, and only
+// that address is passed to kernels.  Calls that overlap on one device share
+// the array.  Nothing in it has to start at zero: the pack kernel writes the
+// N entries that the pair kernel then reads.
+constexpr uint32_t kNoAllocMaxN = 1u << 20;
+static __device__ __align__(256) int4 scratch[kNoAllocMaxN];
+static int4* g_scratch_static[kScratchDevices];
+
+static int noalloc_scratch(uint32_t N, int4** out)
+    {
+    if (N > kNoAllocMaxN)
+        return set_error(AZP_ERR_INVALID,
+                         "the allocation-free launcher packs at most %u particles for the mixed form over the "
+                         "plain list (N = %u): pass d_pos_fixed, a tiled list, or use azp_gpu_compute_pair_forces",
+                         kNoAllocMaxN, N);
+    int dev = 0;
+    AZP_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kScratchDevices)
+        return set_error(AZP_ERR_INVALID, "device ordinal %d is outside the launcher's table", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_static[dev])
+        {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return set_error(AZP_ERR_HIP, "hipGetSymbolAddress of the launcher scratch failed: %s",
+                             hipGetErrorString(e));
+        g_scratch_static[dev] = static_cast<int4*>(ptr);
+        }
+    *out = g_scratch_static[dev];
+    return AZP_OK;
+    }
+
 // Where the launcher puts fixed-point positions it has to pack itself.
 enum class PackTarget
     {
-    Heap, // library-owned device buffer, grown with hipMalloc
+    Heap,   // library-owned device buffer, grown with hipMalloc
+    Static, // the file-scope array above (no allocation)
     };
 
 static int compute_pair_forces(const azp_pair_args* args, int evaluator, const void* d_params, PackTarget target)
@@ -247,6 +285,9 @@ static int compute_pair_forces(const azp_pair_args* args, int evaluator, const v
         case PackTarget::Heap:
             AZP_TRY(launcher_heap_scratch(p.N, &pf));
             break;
+        case PackTarget::Static:
+            AZP_TRY(noalloc_scratch(p.N, &pf));
+            break;
             }
         AZP_TRY(pack_fixed(p.d_pos, p.N, p.box, reinterpret_cast<int32_t*>(pf), s));
         ka.pos_fixed = pf;
@@ -275,3 +316,9 @@ extern "C" int azp_gpu_compute_pair_forces(const azp_pair_args* args, int evalua
     {
     return compute_pair_forces(args, evaluator, d_params, PackTarget::Heap);
     }
+
+extern "C" int azp_gpu_compute_pair_forces_noalloc(const azp_pair_args* args, int evaluator, const void* d_params)
+    {
+    return compute_pair_forces(args, evaluator, d_params, PackTarget::Static);
+    }
+

@@ -234,6 +234,16 @@ int azp_class_lookup(const char* cpp_class_name, int* evaluator, int* driver);
  * gpu_compute_pair_aniso_forces<Eval>  (AnisoPotentialPairGPUKernel.cu.inc:21-25).
  * d_params points to ntypes^2 PODs of the evaluator's param type in device memory. */
 int azp_gpu_compute_pair_forces(const azp_pair_args* args, int evaluator, const void* d_params);
+/* No counterpart in the reference.  The same launch as
+ * azp_gpu_compute_pair_forces without any device allocation, for callers
+ * that cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Where the launcher must pack
+ * fixed-point positions itself (mixed form over the plain or head list with
+ * d_pos_fixed NULL) it uses a file-scope device array of 2^20 int4 (16 MiB
+ * per device, held until the process ends) and rejects N > 2^20 with
+ * AZP_ERR_INVALID; the other forms use no scratch.  Every call on a device
+ * shares that array: calls that overlap on one device race. */
+int azp_gpu_compute_pair_forces_noalloc(const azp_pair_args* args, int evaluator, const void* d_params);
 int azp_gpu_compute_dpd_forces(const azp_pair_args* args, const void* d_params);
 int azp_gpu_compute_pair_aniso_forces(const azp_pair_args* args, int evaluator, const void* d_params);
 /* pos (double4) -> pos_fixed (int4) for the mixed-precision form */
