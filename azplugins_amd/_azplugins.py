@@ -255,14 +255,17 @@ class NeighborListGPU:
         return n.value, r.value
 
     def download(self, N):
-        """(n_neigh[N], list[max_neighbors, pitch]) as numpy arrays."""
+        """(n_neigh[N], list[max_neighbors, pitch]) as numpy arrays.  They view
+        pinned host memory, so each array arrives in one DMA copy."""
         import numpy as np
 
+        torch = _azp.torch
         _, _, pitch, mx = self.get()
-        nn = np.zeros(max(N, 1), dtype=np.uint32)
-        lst = np.zeros(max(mx * pitch, 1), dtype=np.uint32)
-        _azp.call("azp_nlist_download", self._h, int(N), nn.ctypes.data_as(ctypes.c_void_p),
-                  lst.ctypes.data_as(ctypes.c_void_p), lst.size)
+        nn_t = torch.empty(max(N, 1), dtype=torch.int32, pin_memory=True)
+        lst_t = torch.empty(max(mx * pitch, 1), dtype=torch.int32, pin_memory=True)
+        _azp.call("azp_nlist_download", self._h, int(N), nn_t.data_ptr(), lst_t.data_ptr(), lst_t.numel())
+        nn = nn_t.numpy().view(np.uint32)
+        lst = lst_t.numpy().view(np.uint32)
         return nn[:N], lst[: mx * pitch].reshape(mx, pitch) if pitch else lst[:0].reshape(0, 0)
 
     def num_builds(self):

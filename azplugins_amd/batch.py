@@ -175,9 +175,11 @@ class ReplicaBatch:
                       one.ctypes.data_as(ctypes.c_void_p), 0, ctypes.byref(pitch), ctypes.byref(mx))
         except _azp.AzpError:
             pass  # capacity query
-        lst = np.zeros(int(mx.value) * int(pitch.value), dtype=np.uint32)
+        # pinned host memory: the list arrives in one DMA copy
+        lst_t = torch.empty(max(int(mx.value) * int(pitch.value), 1), dtype=torch.int32, pin_memory=True)
         _azp.call("azp_batch_nlist_download", self._h, nn.ctypes.data_as(ctypes.c_void_p),
-                  lst.ctypes.data_as(ctypes.c_void_p), lst.size, ctypes.byref(pitch), ctypes.byref(mx))
+                  lst_t.data_ptr(), int(mx.value) * int(pitch.value), ctypes.byref(pitch), ctypes.byref(mx))
+        lst = lst_t.numpy().view(np.uint32)[: int(mx.value) * int(pitch.value)]
         return nn, lst.reshape(int(mx.value), int(pitch.value))
 
 

@@ -34,7 +34,22 @@ METRIC = "pair interactions/s & TPS at 1M particles, mixed vs FP64 (force rel er
 HBM_PEAK_GBS = 8000.0
 # bytes per particle besides the list: pos 16 (fixed) + n_neigh 4 + head 4 + force 32 + virial 48
 MIXED_BYTES_PER_PARTICLE = 104
+MIXED_NO_VIRIAL_BYTES_PER_PARTICLE = 56
 FP64_BYTES_PER_PARTICLE = 124
+# non-packed VALU issue: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz
+VALU_PEAK_LANE_OPS = 256 * 4 * 16 * 2.4e9
+# the headline kernel as rocprofv3 names it: PerturbedLJ, mixed, virial, one
+# type, no xplor, cubic box, tiled list, FP32 partial sums
+HEADLINE_KERNEL = ("void azp::k_pair<azp::EvalPerturbedLJ, true, true, true, false, true, 1, false>"
+                   "(azp::PairKernelArgs)")
+# sources every pair kernel is compiled from; counters measured on other
+# sources are not reported
+KERNEL_SOURCES = ("azplugins_amd/csrc/pair_kernel.h", "azplugins_amd/csrc/evaluators.h",
+                  "azplugins_amd/csrc/device_common.h", "azplugins_amd/csrc/tiles.h",
+                  "azplugins_amd/csrc/k_pair_tiled.hip", "azplugins_amd/csrc/k_pair.hip",
+                  "azplugins_amd/csrc/k_dpd.hip", "azplugins_amd/csrc/k_adjoint.hip",
+                  "azplugins_amd/csrc/k_batch.hip", "azplugins_amd/csrc/tpm.h",
+                  "azplugins_amd/csrc/k_tile_walk.hip", "azplugins_amd/csrc/portable_math.h", "include/azp.h")
 
 
 def log(*a):
@@ -52,6 +67,12 @@ def parse():
                     help="tiled list for the mixed form: 0 off, 1 auto (512 else 256), or a fixed size")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the DPD, adjoint, batched-replica and Hertz workloads")
+    ap.add_argument("--extras", default="dpd_2m,yukawa_262k_adjoint,tpm_batch_512,hertz_4096,walls_1m,gsd_frame_1m",
+                    help="comma-separated workloads to run besides the headline")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="batched replicas: keep the default launch shape (short profiling passes)")
+    ap.add_argument("--no-headline", action="store_true",
+                    help="run only the workloads in --extras (profiling passes of those kernels)")
     ap.add_argument("--tps-steps", type=int, default=200)
     ap.add_argument("--cpu-cells", type=int, default=64, help="FCC cells per side of the CPU sample")
     ap.add_argument("--sweep-blocks", action="store_true", help="time the pair kernels per block size")
@@ -80,13 +101,17 @@ def main():
         torch.cuda.synchronize()
         log("bench: copy calibration, 1073741824 bytes read + written per launch")
         return
+    if args.no_headline:
+        print(json.dumps({"workloads": run_extras(args)}))
+        return
     c = configs.plj_1m()
     pos4, L = c["pos4"], c["L"]
     N = pos4.shape[0]
     log(f"bench: plj_1m N={N} L={L[0]:.4f}")
     pos = torch.tensor(pos4, device=dev)
-    pos_fixed = torch.zeros((N, 4), dtype=torch.int32, device=dev)
-    parts = _azp.Particles(N, _azp.Box(*L), pos.data_ptr(), pos_fixed.data_ptr(), None, None, None)
+    # no d_pos_fixed: the tiled kernel converts positions itself, and over the
+    # plain list the launcher packs them into its scratch
+    parts = _azp.Particles(N, _azp.Box(*L), pos.data_ptr(), None, None, None, None)
     box = _azp.Box(*L)
     stream = _azp.stream_ptr()
 
@@ -120,13 +145,13 @@ def main():
     rcsq = torch.tensor([9.0], dtype=torch.float64, device=dev)
     vpitch = (N + 63) // 64 * 64
     force = torch.zeros((N, 4), dtype=torch.float64, device=dev)
-    virial = torch.zeros((6, vpitch), dtype=torch.float64, device=dev)
+    virial_t = torch.zeros((6, vpitch), dtype=torch.float64, device=dev)
 
-    def make_args(precision):
+    def make_args(precision, virial=1, tiled=False):
         a = _azp.PairArgs()
         a.particles = parts
         a.out.d_force = force.data_ptr()
-        a.out.d_virial = virial.data_ptr()
+        a.out.d_virial = virial_t.data_ptr()
         a.out.virial_pitch = vpitch
         a.out.d_torque = None
         a.d_n_neigh = d_nn
@@ -137,28 +162,31 @@ def main():
         a.d_ronsq = None
         a.ntypes = 1
         a.shift_mode = 0
-        a.compute_virial = 1
+        a.compute_virial = virial
         a.precision = precision
         a.block_size = 0
         a.stream = stream
+        if tiled and tiles[3]:
+            a.d_tile_count, a.d_tile_jset, a.d_tile_list, a.tile_size, a.tile_slots = tiles
+            a.d_tile_order, a.d_tile_nn = tile_order
         return a
 
-    args_mixed = make_args(1)
-    args_mixed.d_tile_count, args_mixed.d_tile_jset, args_mixed.d_tile_list, args_mixed.tile_size, \
-        args_mixed.tile_slots = tiles
-    args_mixed.d_tile_order, args_mixed.d_tile_nn = tile_order
-    args_fp64 = make_args(0)
+    # the forms measured: the headline (mixed, tiled list, with the per-particle
+    # virial), the same without the virial (what a Langevin run without
+    # pressure logging computes), FP32 pair arithmetic with every entry added
+    # to FP64 accumulators, and the FP64 form over the plain and the tiled list
+    forms = {
+        "mixed": make_args(1, tiled=True),
+        "mixed_no_virial": make_args(1, virial=0, tiled=True),
+        "mixed_acc64": make_args(2, tiled=True),
+        "fp64": make_args(0),
+        "fp64_tiled": make_args(0, tiled=True),
+    }
 
-    def step_mixed():
-        # over the tiled list the pair kernel converts positions to fixed point itself
-        if not tiles[3]:
-            _azp.call("azp_gpu_pack_fixed", _azp.dptr(pos), N, ctypes.byref(box), _azp.dptr(pos_fixed), stream)
-        _azp.call("azp_gpu_compute_pair_forces", ctypes.byref(args_mixed), 2, _azp.dptr(d_params))
+    def launch(name):
+        _azp.call("azp_gpu_compute_pair_forces", ctypes.byref(forms[name]), 2, _azp.dptr(d_params))
 
-    def step_fp64():
-        _azp.call("azp_gpu_compute_pair_forces", ctypes.byref(args_fp64), 2, _azp.dptr(d_params))
-
-    def time_steps(fn, steps, warmup, kernel_events=False):
+    def time_steps(fn, steps, warmup):
         for _ in range(warmup):
             fn()
         torch.cuda.synchronize()
@@ -168,85 +196,111 @@ def main():
         torch.cuda.synchronize()
         return (time.perf_counter() - t0) / steps
 
-    def time_kernel(precision, reps):
-        """Average duration of the pair kernel alone: HIP events on its stream around
-        `reps` back-to-back launches (the sustained rate, as in a run of steps)."""
-        a = args_mixed if precision == 1 else args_fp64
+    def time_kernel(name, reps):
+        """Average duration of the pair kernel alone: HIP events on its stream
+        (the launches go to torch's current stream) around `reps` back-to-back
+        launches, the sustained rate of a run of steps."""
         for _ in range(3):
-            _azp.call("azp_gpu_compute_pair_forces", ctypes.byref(a), 2, _azp.dptr(d_params))
+            launch(name)
         start = torch.cuda.Event(enable_timing=True)
         end = torch.cuda.Event(enable_timing=True)
         start.record()
         for _ in range(reps):
-            _azp.call("azp_gpu_compute_pair_forces", ctypes.byref(a), 2, _azp.dptr(d_params))
+            launch(name)
         end.record()
         torch.cuda.synchronize()
         return start.elapsed_time(end) / 1e3 / reps
 
-    # reference results (FP64 form) for the error of the mixed form
-    step_fp64()
-    torch.cuda.synchronize()
-    f64 = force.clone()
-    v64 = virial.clone()
-    step_mixed()
-    torch.cuda.synchronize()
-    df = (force[:, :3] - f64[:, :3]).abs().max().item() / f64[:, :3].abs().max().item()
-    de = (force[:, 3] - f64[:, 3]).abs().max().item() / f64[:, 3].abs().max().item()
-    dv = (virial - v64).abs().max().item() / v64.abs().max().item()
-    l2 = ((force[:, :3] - f64[:, :3]).norm() / f64[:, :3].norm()).item()
-    log(f"bench: mixed vs fp64 force rel err {df:.2e} (l2 {l2:.2e}) energy {de:.2e} virial {dv:.2e}")
+    def results(name):
+        launch(name)
+        torch.cuda.synchronize()
+        return force.clone(), virial_t.clone()
+
+    # errors of each form against the FP64 form over the plain list
+    f64, v64 = results("fp64")
+    errors = {}
+    for name in ("mixed", "mixed_acc64", "fp64_tiled"):
+        f, v = results(name)
+        errors[name] = {
+            "max_rel_force": (f[:, :3] - f64[:, :3]).abs().max().item() / f64[:, :3].abs().max().item(),
+            "l2_rel_force": ((f[:, :3] - f64[:, :3]).norm() / f64[:, :3].norm()).item(),
+            "max_rel_energy": (f[:, 3] - f64[:, 3]).abs().max().item() / f64[:, 3].abs().max().item(),
+            "max_rel_virial": (v - v64).abs().max().item() / v64.abs().max().item(),
+        }
+        log(f"bench: {name} vs fp64: force rel err {errors[name]['max_rel_force']:.2e} "
+            f"(l2 {errors[name]['l2_rel_force']:.2e}) energy {errors[name]['max_rel_energy']:.2e} "
+            f"virial {errors[name]['max_rel_virial']:.2e}")
 
     # count of unordered pairs inside r_cut (the unit of work)
-    p64 = pos
-    pairs_in_cut = count_pairs_in_cut(torch, p64, nn_host, nl, N, L, 9.0)
+    pairs_in_cut = count_pairs_in_cut(torch, pos, nn_host, nl, N, L, 9.0)
     log(f"bench: nlist entries {nlist_entries} ({nlist_entries / N:.1f}/particle), "
         f"pairs in cut {pairs_in_cut} ({2 * pairs_in_cut / N:.1f}/particle), nlist build {t_nlist * 1e3:.1f} ms")
 
     if args.sweep_blocks:
-        for prec in (1, 0):
-            a = args_mixed if prec == 1 else args_fp64
+        for name in ("mixed", "fp64"):
+            if name == "mixed" and tiles[3]:
+                continue  # the tiled kernel's workgroup is its tile
             for b in (64, 128, 256, 512):
-                a.block_size = b
-                time_kernel(prec, 3)
-                log(f"bench: sweep precision={'mixed' if prec else 'fp64'} block={b} "
-                    f"kernel {time_kernel(prec, 10) * 1e6:.1f} us")
-            a.block_size = 0
+                forms[name].block_size = b
+                time_kernel(name, 3)
+                log(f"bench: sweep {name} block={b} kernel {time_kernel(name, 10) * 1e6:.1f} us")
+            forms[name].block_size = 0
 
-    t_mixed = time_steps(step_mixed, args.steps, args.warmup)
-    t_fp64 = time_steps(step_fp64, args.steps, args.warmup)
-    k_mixed = time_kernel(1, max(10, args.steps))
-    k_fp64 = time_kernel(0, max(10, args.steps))
-    log(f"bench: step mixed {t_mixed * 1e6:.1f} us, fp64 {t_fp64 * 1e6:.1f} us; "
-        f"kernel mixed {k_mixed * 1e6:.1f} us, fp64 {k_fp64 * 1e6:.1f} us")
+    reps = max(10, args.steps)
+    t_step = {}
+    for name in ("mixed", "fp64", "fp64_tiled"):
+        t_step[name] = time_steps(lambda: launch(name), args.steps, args.warmup)
+    k_us = {name: time_kernel(name, reps) for name in forms}
+    log("bench: step " + ", ".join(f"{k} {v * 1e6:.1f} us" for k, v in t_step.items()) + "; kernel "
+        + ", ".join(f"{k} {v * 1e6:.1f} us" for k, v in k_us.items()))
+    t_mixed = t_step["mixed"]
 
+    # algorithmic bytes per launch: per-particle streams + 4 bytes per list entry
     bytes_mixed = N * MIXED_BYTES_PER_PARTICLE + 4 * nlist_entries
+    bytes_mixed_nv = N * MIXED_NO_VIRIAL_BYTES_PER_PARTICLE + 4 * nlist_entries
     bytes_fp64 = N * FP64_BYTES_PER_PARTICLE + 4 * nlist_entries
-    achieved = bytes_mixed / k_mixed / 1e9
-    traffic = load_traffic()
+    achieved = bytes_mixed / k_us["mixed"] / 1e9
+    kernel_name = HEADLINE_KERNEL if tiles[3] else "k_pair<EvalPerturbedLJ, mixed, virial, one type, column list>"
+    counters = load_counters("pmc_plj_1m.json", HEADLINE_KERNEL) if tiles[3] else None
     roofline = {
         "bound": "hbm",
         "achieved": round(achieved, 1),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": traffic,
-        "kernel": "k_pair<EvalPerturbedLJ, MIXED, VIRIAL, SINGLE%s>" % (", TILED" if tiles[3] else ""),
-        "kernel_us": round(k_mixed * 1e6, 2),
+        "traffic": counters["traffic_bytes"] if counters else None,
+        "kernel": kernel_name,
+        "kernel_us": round(k_us["mixed"] * 1e6, 2),
         "algorithmic_bytes": bytes_mixed,
+        "counters": counters["provenance"] if counters else "none measured on these kernel sources",
     }
     # the kernel is bound by VALU issue: its wavefront VALU instructions per
     # launch (rocprofv3 counters) over the live kernel time, against the
     # non-packed VALU issue peak
-    valu = load_valu()
-    if valu is not None and tiles[3]:
-        lane_ops = valu["insts_valu"] * 64
+    if counters and counters.get("insts_valu"):
+        lane_ops = counters["insts_valu"] * 64
         roofline["valu"] = {
-            "instructions_per_launch": valu["insts_valu"],
-            "achieved_lane_ops_per_s": lane_ops / k_mixed,
-            "peak_lane_ops_per_s": valu["peak"],
-            "frac": round(lane_ops / k_mixed / valu["peak"], 4),
+            "instructions_per_launch": counters["insts_valu"],
+            "lane_instructions_per_list_entry": round(lane_ops / nlist_entries, 2),
+            "achieved_lane_ops_per_s": lane_ops / k_us["mixed"],
+            "peak_lane_ops_per_s": VALU_PEAK_LANE_OPS,
+            "frac": round(lane_ops / k_us["mixed"] / VALU_PEAK_LANE_OPS, 4),
         }
 
+    def form_line(name, nbytes):
+        return {"kernel_us": round(k_us[name] * 1e6, 2), "value": pairs_in_cut / k_us[name],
+                "unit": "pair interactions/s (kernel time)", "algorithmic_bytes": nbytes,
+                "achieved_GBs": round(nbytes / k_us[name] / 1e9, 1),
+                "frac": round(nbytes / k_us[name] / 1e9 / HBM_PEAK_GBS, 4)}
+
+    variants = {
+        "mixed_no_virial": dict(form_line("mixed_no_virial", bytes_mixed_nv),
+                                note="forces and energies only, as the Langevin run computes them"),
+        "mixed_acc64": dict(form_line("mixed_acc64", bytes_mixed), errors=errors["mixed_acc64"],
+                            dtype="f32 pair arithmetic, every entry added to f64 accumulators"),
+        "fp64_tiled": dict(form_line("fp64_tiled", bytes_fp64), ms_per_step=t_step["fp64_tiled"] * 1e3,
+                           errors=errors["fp64_tiled"]),
+    }
     out = {
         "metric": METRIC,
         "value": pairs_in_cut / t_mixed,
@@ -258,7 +312,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 pair arithmetic, f64 accumulation",
+        "dtype": "f32 pair arithmetic; f32 sums of 32 list entries added to f64 accumulators",
         "data": "synthetic",
         "config": {
             "workload": "plj_1m",
@@ -270,17 +324,23 @@ def main():
             "pairs_in_cut": pairs_in_cut,
             "nlist_entries": nlist_entries,
             "precision": "mixed",
+            "virial": True,
+            "list": "tiled" if tiles[3] else "column",
         },
         "roofline": roofline,
         "fp64": {
-            "value": pairs_in_cut / t_fp64,
-            "ms_per_step": t_fp64 * 1e3,
-            "kernel_us": round(k_fp64 * 1e6, 2),
-            "achieved_GBs": round(bytes_fp64 / k_fp64 / 1e9, 1),
+            "value": pairs_in_cut / t_step["fp64"],
+            "ms_per_step": t_step["fp64"] * 1e3,
+            "kernel_us": round(k_us["fp64"] * 1e6, 2),
+            "achieved_GBs": round(bytes_fp64 / k_us["fp64"] / 1e9, 1),
+            "list": "column (plain)",
         },
-        "mixed_over_fp64": t_fp64 / t_mixed,
-        "max_rel_force_err_vs_fp64": df,
-        "l2_rel_force_err_vs_fp64": l2,
+        "mixed_over_fp64": t_step["fp64"] / t_mixed,
+        "mixed_over_fp64_tiled": t_step["fp64_tiled"] / t_mixed,
+        "max_rel_force_err_vs_fp64": errors["mixed"]["max_rel_force"],
+        "l2_rel_force_err_vs_fp64": errors["mixed"]["l2_rel_force"],
+        "errors_vs_fp64": errors["mixed"],
+        "variants": variants,
         "nlist_build_ms": t_nlist * 1e3,
         "nlist_build_with_tiles_ms": t_nlist_tiled * 1e3,
         "tile_size": tiles[3],
@@ -290,19 +350,28 @@ def main():
         out["tps"] = run_tps(c, args.tps_steps)
         out["tps_fp64"] = run_tps(c, args.tps_steps, "fp64")
     if not args.no_extras:
-        extras = {}
-        for name, fn in (("dpd_2m", bench_dpd), ("yukawa_262k_adjoint", bench_adjoint),
-                         ("tpm_batch_512", bench_batch), ("hertz_4096", bench_hertz),
-                         ("walls_1m", bench_walls), ("gsd_frame_1m", bench_gsd)):
-            try:
-                extras[name] = fn(args)
-            except Exception as exc:  # a failing extra must not hide the headline line
-                log(f"bench: {name} failed: {exc!r}")
-                extras[name] = {"error": repr(exc)}
-        out["workloads"] = extras
+        out["workloads"] = run_extras(args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_cells)
     print(json.dumps(out))
+
+
+def run_extras(args):
+    """The other workloads (DPD, adjoint, batched replicas, Hertz NVE, walls,
+    GSD frame) selected by --extras."""
+    extras = {}
+    wanted = set(args.extras.split(","))
+    for name, fn in (("dpd_2m", bench_dpd), ("yukawa_262k_adjoint", bench_adjoint),
+                     ("tpm_batch_512", bench_batch), ("hertz_4096", bench_hertz),
+                     ("walls_1m", bench_walls), ("gsd_frame_1m", bench_gsd)):
+        if name not in wanted:
+            continue
+        try:
+            extras[name] = fn(args)
+        except Exception as exc:  # a failing extra must not hide the headline line
+            log(f"bench: {name} failed: {exc!r}")
+            extras[name] = {"error": repr(exc)}
+    return extras
 
 
 def count_pairs_in_cut(torch, pos, nn_host, nl, N, L, rcutsq):
@@ -312,15 +381,18 @@ def count_pairs_in_cut(torch, pos, nn_host, nl, N, L, rcutsq):
 
 
 def count_list_pairs(torch, pos, nn_host, lst, N, L, rcutsq, typeid=None, ntypes=1):
-    """Unordered pairs of a full list with r^2 < rcutsq[ti, tj] (L: (3,) or per-particle (N, 3))."""
+    """Unordered pairs of a full list with r^2 < rcutsq[ti, tj] (L: (3,) or
+    per-particle (N, 3)).  The list goes to the device in one copy (it views
+    pinned memory after a download) and is counted there in row chunks."""
     dev = pos.device
     total = 0
     chunk = 16
-    nn_t = torch.tensor(nn_host.astype(np.int64), device=dev)
+    nn_t = torch.from_numpy(np.ascontiguousarray(nn_host).view(np.int32)).to(dev).long()
     Lt = torch.tensor(np.asarray(L, dtype=np.float64), device=dev)
     rc = torch.tensor(np.atleast_1d(np.asarray(rcutsq, dtype=np.float64)), device=dev)
-    for k0 in range(0, lst.shape[0], chunk):
-        rows = torch.tensor(lst[k0:k0 + chunk, :N].astype(np.int64), device=dev)
+    lst_d = torch.from_numpy(np.ascontiguousarray(lst).view(np.int32)).to(dev)
+    for k0 in range(0, lst_d.shape[0], chunk):
+        rows = lst_d[k0:k0 + chunk, :N].long()
         valid = (torch.arange(k0, k0 + rows.shape[0], device=dev)[:, None] < nn_t[None, :])
         jj = rows.clamp(max=N - 1)
         pj = pos[jj, :3]
@@ -332,6 +404,7 @@ def count_list_pairs(torch, pos, nn_host, lst, N, L, rcutsq, typeid=None, ntypes
         else:
             lim = rc[(typeid[None, :] * ntypes + typeid[jj]).long()]
         total += int(((rsq < lim) & valid).sum().item())
+    del lst_d
     return total // 2
 
 
@@ -347,6 +420,27 @@ def _events(torch, fn, reps, warmup=2):
     e.record()
     torch.cuda.synchronize()
     return s.elapsed_time(e) / 1e3 / reps
+
+
+def roofline_line(nbytes, seconds, kernel, counters_file, entries, note=None):
+    """roofline object of one workload kernel: algorithmic bytes over the
+    measured time against the HBM peak, with counters when measured on the
+    current sources."""
+    r = {"bound": "hbm", "algorithmic_bytes": nbytes, "us": round(seconds * 1e6, 2),
+         "achieved": round(nbytes / seconds / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+         "frac": round(nbytes / seconds / 1e9 / HBM_PEAK_GBS, 4)}
+    if note:
+        r["note"] = note
+    c = load_counters(counters_file, kernel) if counters_file else None
+    if c:
+        r["traffic"] = c["traffic_bytes"]
+        r["counters"] = c["provenance"]
+        if c.get("insts_valu"):
+            lane_ops = c["insts_valu"] * 64
+            r["valu"] = {"instructions_per_launch": c["insts_valu"],
+                         "lane_instructions_per_list_entry": round(lane_ops / entries, 2),
+                         "frac": round(lane_ops / seconds / VALU_PEAK_LANE_OPS, 4)}
+    return r
 
 
 def bench_dpd(args):
@@ -376,7 +470,7 @@ def bench_dpd(args):
     nl.update(parts, force_rebuild=True)
     vpitch = (N + 63) // 64 * 64
     force = torch.zeros((N, 4), dtype=torch.float64, device=dev)
-    virial = torch.zeros((6, vpitch), dtype=torch.float64, device=dev)
+    virial_t = torch.zeros((6, vpitch), dtype=torch.float64, device=dev)
     out = _azp.Outputs(force.data_ptr(), virial.data_ptr(), vpitch, None)
     step = [100]
 
@@ -387,8 +481,13 @@ def bench_dpd(args):
     t = _events(torch, fn, max(5, args.steps // 5))
     nn, lst = nl.download(N)
     pairs = count_list_pairs(torch, pos, nn, lst, N, L, [1.0])
+    entries = int(nn.sum())
+    # pos 32 + n_neigh 4 + head 8 + force 32 + virial 48 + vel 32 + tag 4 per particle, 4 per list entry
+    nbytes = N * 160 + 4 * entries
     res = {"value": pairs / t, "unit": "pair interactions/s", "ms_per_eval": t * 1e3, "N": N,
-           "pairs_in_cut": pairs, "nlist_entries": int(nn.sum()), "precision": "fp64",
+           "pairs_in_cut": pairs, "nlist_entries": entries, "precision": "fp64",
+           "roofline": roofline_line(nbytes, t, "void azp::k_dpd<true, true>(azp::DPDKernelArgs)", "pmc_dpd_2m.json",
+                                     entries),
            "note": "one DPD force evaluation (conservative + dissipative + Philox random) per step"}
     log(f"bench: dpd_2m {t * 1e3:.3f} ms/eval, {pairs / t:.3e} pairs/s")
     return res
@@ -425,8 +524,20 @@ def bench_adjoint(args):
         t = _events(torch, fwd_bwd, 10)
         with torch.no_grad():
             t_f = _events(torch, lambda: ff(pos, typeid, th), 10)
+        N = pos4.shape[0]
+        entries = int(ff._nl.download(N)[0].sum())
+        # forward: the pair kernel's model (104 B per particle mixed, 124 FP64,
+        # 4 per list entry); backward: pos 32 + dL/dF,dL/dE 32 + dL/dr 32 +
+        # n_neigh 4 per particle and 4 per list entry
+        fwd_bytes = N * (104 if precision == "mixed" else 124) + 4 * entries
+        bwd_bytes = N * 100 + 4 * entries
         res[precision] = {"value": 1.0 / t, "unit": "loss+gradient evaluations/s", "ms_forward_backward": t * 1e3,
-                          "ms_forward": t_f * 1e3}
+                          "ms_forward": t_f * 1e3, "nlist_entries": entries,
+                          "roofline_forward": roofline_line(fwd_bytes, t_f, None, None, entries),
+                          "roofline_backward": roofline_line(bwd_bytes, t - t_f, None, None, entries,
+                                                             note="backward = forward+backward minus forward: "
+                                                                  "the adjoint kernel, its final reduction "
+                                                                  "and the torch glue")}
         log(f"bench: yukawa_262k adjoint {precision}: forward {t_f * 1e3:.3f} ms, forward+backward {t * 1e3:.3f} ms")
     res["N"] = pos4.shape[0]
     return res
@@ -442,9 +553,9 @@ def bench_batch(args):
     b = from_config(cfg, precision="mixed")
     b.set_timestep(0)
     b.compute(force_rebuild=True)
-    tune = b.autotune()
+    tune = None if args.no_autotune else b.autotune()
     t_force = _events(torch, lambda: b.compute(), 10)
-    steps = 100
+    steps = 20 if args.no_autotune else 100
     b.run(20)
     torch.cuda.synchronize()
     st0 = b.stats()
@@ -456,10 +567,18 @@ def bench_batch(args):
     nn, lst = b.nlist()
     Lper = np.repeat(np.stack([r["L"] for r in cfg["replicas"]]), b.sizes.astype(np.int64), axis=0)
     pairs = count_list_pairs(torch, b.pos, nn, lst, b.N, Lper, [cfg["r_cut"] ** 2])
+    entries = int(nn.sum())
+    # the batched force launch: 104 + orientation 16 + torque 32 per particle, 4 per list entry
+    force_roofline = roofline_line(b.N * 152 + 4 * entries, t_force,
+                                   "void azp::k_b_forces<true, true, true>(azp::BatchForceArgs)",
+                                   "pmc_tpm_batch.json", entries,
+                                   note="time of azp_batch_compute: the fixed-point pack and the persistent "
+                                        "force launch")
     res = {"value": steps / dt, "unit": "timesteps/s (all replicas)", "replica_steps_per_s": b.R * steps / dt,
            "particle_steps_per_s": b.N * steps / dt, "pairs_in_cut": pairs,
            "pair_interactions_per_s_in_steps": pairs * steps / dt, "force_ms": t_force * 1e3,
-           "replicas": b.R, "N_total": b.N, "tuning": tune, "nlist_builds": st1["n_builds"] - st0["n_builds"],
+           "replicas": b.R, "N_total": b.N, "nlist_entries": entries, "roofline": force_roofline,
+           "tuning": tune, "nlist_builds": st1["n_builds"] - st0["n_builds"],
            "replica_rebuilds": st1["replica_builds"] - st0["replica_builds"],
            "graph_launches": st1["graph_launches"] - st0["graph_launches"], "precision": "mixed",
            "method": "Langevin (translation + rotation, I = 1), dt 0.005, per-replica kT"}
@@ -679,26 +798,36 @@ def host_cpu_model():
     return "unknown CPU"
 
 
-def load_valu():
-    """VALU instructions per launch of the tiled mixed kernel and the issue peak, if measured."""
-    p = os.path.join(ROOT, "profiles", "pmc_issue_plj_1m.json")
+def kernel_source_hash():
+    """SHA-256 of the kernel sources (first 16 hex digits): the provenance
+    recorded with every committed counter summary."""
+    import hashlib
+
+    h = hashlib.sha256()
+    for rel in KERNEL_SOURCES:
+        with open(os.path.join(ROOT, rel), "rb") as fh:
+            h.update(rel.encode() + b"\0" + fh.read())
+    return h.hexdigest()[:16]
+
+
+def load_counters(fname, kernel):
+    """Counters of `kernel` from profiles/<fname> (rocprofv3 --pmc summaries),
+    only when they were measured on the current kernel sources."""
+    p = os.path.join(ROOT, "profiles", fname)
     if not os.path.exists(p):
         return None
     with open(p) as fh:
         d = json.load(fh)
-    for name, v in d.get("kernels", {}).items():
-        if "k_pair<azp::EvalPerturbedLJ, true, true, true, false, true, true>" in name:
-            return {"insts_valu": v["SQ_INSTS_VALU"], "peak": d["peak_valu_lane_ops_per_s"]}
-    return None
-
-
-def load_traffic():
-    """HBM bytes per launch of the mixed pair kernel from rocprofv3 counters, if measured."""
-    p = os.path.join(ROOT, "profiles", "pmc_traffic_plj_1m.json")
-    if not os.path.exists(p):
+    src = kernel_source_hash()
+    if d.get("source_sha256") != src:
+        log(f"bench: profiles/{fname} was measured on sources {d.get('source_sha256')}, these are {src}: "
+            f"its counters are not reported")
         return None
-    with open(p) as fh:
-        return json.load(fh).get("traffic_bytes_per_launch")
+    k = d.get("kernels", {}).get(kernel)
+    if k is None:
+        return None
+    return {"traffic_bytes": k.get("traffic_bytes"), "insts_valu": k.get("SQ_INSTS_VALU"),
+            "provenance": f"profiles/{fname}, kernel sources {src}"}
 
 
 if __name__ == "__main__":
