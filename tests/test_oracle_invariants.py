@@ -394,3 +394,56 @@ def test_aniso_driver_third_law_and_virial_trace(orc):
         if ok:
             tr += d[k] @ F
     assert v[0].sum() + v[3].sum() + v[5].sum() == pytest.approx(tr, rel=1e-12)
+
+
+def test_aniso_virial_is_the_strain_derivative(orc):
+    """Every stored component of the anisotropic virial, off-diagonal ones
+    included, against the affine strain derivative of the total energy:
+    W_ab = -dU/d eps_ab where eps_ab moves coordinate a of each separation by
+    eps_ab times coordinate b (orientations held fixed).  With non-central
+    forces W is not symmetric, and this fixes which of sum F_a dx_b and
+    sum F_b dx_a the stored xy, xz and yz hold.  Mode none, where the force is
+    the energy's gradient (the shift term of mode shift depends on the angles
+    but is not differentiated, as in the reference's evaluator); the pairs
+    inside r_cut are evaluated without the cut for the differences, so no
+    pair crosses it."""
+    params = {"M_d": 1.0, "M_r": 0.2, "r_eq": 1.0, "omega": 20.0, "alpha": 0.4, "repulsion": True}
+    L = 7.0
+    pos = random_fluid(150, L, 1, min_sep=0.85, seed=9)
+    Lv = np.array([L, L, L])
+    rng = np.random.default_rng(31)
+    q = rng.normal(size=(150, 4))
+    q /= np.linalg.norm(q, axis=1)[:, None]
+    packed = orc.pack_matrix(5, {(0, 0): params}, 1)
+    pairs = orc.nlist(pos, Lv, 1, orc.r_listsq_matrix([2.5], 0.4), "brute")
+    f, t, v = orc.aniso_forces(pos, q, Lv, pairs, 1, packed, np.array([6.25]), "none")
+    W = v.sum(axis=1)
+    pi, pj = pairs
+    d0 = min_image(pos[pi, :3] - pos[pj, :3], Lv)
+    inside = (d0 * d0).sum(axis=1) <= 6.25
+
+    def energy(eps):
+        d = d0[inside] @ (np.eye(3) + eps).T
+        a, b = pi[inside], pj[inside]
+        U = 0.0
+        for k in range(d.shape[0]):
+            ok, F, e, _, _ = orc.eval_two_patch_morse(d[k], q[a[k]], q[b[k]], 100.0, packed[0], False)
+            assert ok
+            U += e
+        return U
+
+    comps = [(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]
+    h = 1e-6
+    asym = 0.0
+    for n, (a, b) in enumerate(comps):
+        eps = np.zeros((3, 3))
+        eps[a, b] = h
+        dU = (energy(eps) - energy(-eps)) / (2 * h)
+        assert W[n] == pytest.approx(-dU, rel=1e-6, abs=1e-7 * np.abs(W).max())
+        if a != b:
+            # the transposed strain gives another number: the test tells the orders apart
+            epsT = np.zeros((3, 3))
+            epsT[b, a] = h
+            dUT = (energy(epsT) - energy(-epsT)) / (2 * h)
+            asym = max(asym, abs(dU - dUT) / np.abs(W).max())
+    assert asym > 1e-2
