@@ -42,14 +42,20 @@ VALU_PEAK_LANE_OPS = 256 * 4 * 16 * 2.4e9
 # type, no xplor, cubic box, tiled list, FP32 partial sums
 HEADLINE_KERNEL = ("void azp::k_pair<azp::EvalPerturbedLJ, true, true, true, false, true, 1, false>"
                    "(azp::PairKernelArgs)")
-# sources every pair kernel is compiled from; counters measured on other
-# sources are not reported
-KERNEL_SOURCES = ("azplugins_amd/csrc/pair_kernel.h", "azplugins_amd/csrc/evaluators.h",
-                  "azplugins_amd/csrc/device_common.h", "azplugins_amd/csrc/tiles.h",
-                  "azplugins_amd/csrc/k_pair_tiled.hip", "azplugins_amd/csrc/k_pair.hip",
-                  "azplugins_amd/csrc/k_dpd.hip", "azplugins_amd/csrc/k_adjoint.hip",
-                  "azplugins_amd/csrc/k_batch.hip", "azplugins_amd/csrc/tpm.h",
-                  "azplugins_amd/csrc/k_tile_walk.hip", "azplugins_amd/csrc/portable_math.h", "include/azp.h")
+# the sources each counter summary's kernels are compiled from: a summary
+# measured on other sources is not reported
+KERNEL_SOURCES = {
+    "pmc_plj_1m.json": ("azplugins_amd/csrc/pair_kernel.h", "azplugins_amd/csrc/evaluators.h",
+                        "azplugins_amd/csrc/device_common.h", "azplugins_amd/csrc/tiles.h",
+                        "azplugins_amd/csrc/k_pair_tiled.hip", "include/azp.h"),
+    "pmc_dpd_2m.json": ("azplugins_amd/csrc/k_dpd.hip", "azplugins_amd/csrc/device_common.h",
+                        "azplugins_amd/csrc/portable_math.h", "include/azp.h"),
+    "pmc_tpm_batch.json": ("azplugins_amd/csrc/k_batch.hip", "azplugins_amd/csrc/tpm.h",
+                           "azplugins_amd/csrc/device_common.h", "include/azp.h"),
+    "pmc_yukawa_262k.json": ("azplugins_amd/csrc/k_adjoint.hip", "azplugins_amd/csrc/pair_kernel.h",
+                             "azplugins_amd/csrc/evaluators.h", "azplugins_amd/csrc/device_common.h",
+                             "include/azp.h"),
+}
 
 
 def log(*a):
@@ -470,7 +476,7 @@ def bench_dpd(args):
     nl.update(parts, force_rebuild=True)
     vpitch = (N + 63) // 64 * 64
     force = torch.zeros((N, 4), dtype=torch.float64, device=dev)
-    virial_t = torch.zeros((6, vpitch), dtype=torch.float64, device=dev)
+    virial = torch.zeros((6, vpitch), dtype=torch.float64, device=dev)
     out = _azp.Outputs(force.data_ptr(), virial.data_ptr(), vpitch, None)
     step = [100]
 
@@ -798,13 +804,13 @@ def host_cpu_model():
     return "unknown CPU"
 
 
-def kernel_source_hash():
-    """SHA-256 of the kernel sources (first 16 hex digits): the provenance
-    recorded with every committed counter summary."""
+def kernel_source_hash(fname):
+    """SHA-256 (first 16 hex digits) of the sources of the kernels that the
+    counter summary `fname` covers: the provenance recorded with it."""
     import hashlib
 
     h = hashlib.sha256()
-    for rel in KERNEL_SOURCES:
+    for rel in KERNEL_SOURCES[fname]:
         with open(os.path.join(ROOT, rel), "rb") as fh:
             h.update(rel.encode() + b"\0" + fh.read())
     return h.hexdigest()[:16]
@@ -818,7 +824,7 @@ def load_counters(fname, kernel):
         return None
     with open(p) as fh:
         d = json.load(fh)
-    src = kernel_source_hash()
+    src = kernel_source_hash(fname)
     if d.get("source_sha256") != src:
         log(f"bench: profiles/{fname} was measured on sources {d.get('source_sha256')}, these are {src}: "
             f"its counters are not reported")
