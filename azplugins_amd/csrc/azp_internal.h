@@ -18,6 +18,10 @@ namespace azp
 // kept until the process ends): fixed-point positions they pack themselves.
 int launcher_heap_scratch(uint32_t N, int4** out);
 
+// Whether the DPD kernel can stage the FP64 positions of a tiled list's
+// neighbor set of `slots` entries in one workgroup's LDS.
+bool dpd_tile_fits(uint32_t ntypes, uint32_t slots, uint32_t tile_size);
+
 // Records a message for azp_last_error() and returns `code`.
 int set_error(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
 void clear_error();

@@ -468,15 +468,18 @@ extern "C" int azp_pair_compute(azp_pair* pp,
     std::memset(&a, 0, sizeof(a));
     a.particles = *p;
     a.out = *out;
-    // the isotropic drivers run over the tiled list when it is there and
-    // their neighbor positions fit in LDS (16-byte slots in the mixed form,
-    // 32-byte FP64 slots in the plain form); DPD and the anisotropic driver
-    // read the plain list (expanded from the tiled one on first use)
+    // the isotropic drivers and DPD run over the tiled list when it is there
+    // and their neighbors fit in LDS (16-byte slots in the mixed form, 32-byte
+    // FP64 slots in the plain form and for DPD); the anisotropic driver reads
+    // the plain list (expanded from the tiled one on first use)
     uint32_t tiled = 0, slots = 0;
-    if (pp->driver == AZP_DRIVER_PAIR || pp->driver == AZP_DRIVER_CONSERVATIVE)
+    if (pp->driver != AZP_DRIVER_ANISO)
         {
         AZP_TRY(azp_nlist_get_tiles(nl, nullptr, nullptr, nullptr, &tiled, &slots));
-        if (tiled != 0 && !pair_tile_fits(pp->evaluator, pp->ntypes, slots, pp->precision != AZP_PREC_FP64))
+        const bool fits = pp->driver == AZP_DRIVER_DPD_THERMO
+                              ? dpd_tile_fits(pp->ntypes, slots, tiled)
+                              : pair_tile_fits(pp->evaluator, pp->ntypes, slots, pp->precision != AZP_PREC_FP64);
+        if (tiled != 0 && !fits)
             tiled = 0;
         }
     AZP_TRY(nlist_get_on_stream(nl, s, tiled == 0, &a.d_n_neigh, &a.d_nlist, &a.nlist_pitch, &a.max_neighbors));
@@ -498,16 +501,16 @@ extern "C" int azp_pair_compute(azp_pair* pp,
         AZP_TRY(pack_fixed(p->d_pos, p->N, p->box, pp->d_pos_fixed.ptr, s));
         a.particles.d_pos_fixed = pp->d_pos_fixed.ptr;
         }
+    if (tiled != 0)
+        {
+        AZP_TRY(azp_nlist_get_tiles(nl, &a.d_tile_count, &a.d_tile_jset, &a.d_tile_list, &a.tile_size,
+                                    &a.tile_slots));
+        AZP_TRY(azp_nlist_get_tile_order(nl, &a.d_tile_order, &a.d_tile_nn));
+        }
     switch (pp->driver)
         {
     case AZP_DRIVER_PAIR:
     case AZP_DRIVER_CONSERVATIVE:
-        if (tiled != 0)
-            {
-            AZP_TRY(azp_nlist_get_tiles(nl, &a.d_tile_count, &a.d_tile_jset, &a.d_tile_list, &a.tile_size,
-                                        &a.tile_slots));
-            AZP_TRY(azp_nlist_get_tile_order(nl, &a.d_tile_order, &a.d_tile_nn));
-            }
         return azp_gpu_compute_pair_forces(&a, pp->evaluator, pp->d_params.ptr);
     case AZP_DRIVER_DPD_THERMO:
         return azp_gpu_compute_dpd_forces(&a, pp->d_params.ptr);

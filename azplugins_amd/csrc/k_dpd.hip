@@ -14,6 +14,7 @@
 // same alpha and the pair force stays antisymmetric.
 #include "device_common.h"
 #include "portable_math.h"
+#include "tiles.h"
 
 namespace azp
 {
@@ -40,6 +41,15 @@ struct DPDKernelArgs
     double* virial;
     uint32_t vpitch;
     uint32_t queue_len; // in-cut neighbor indices per lane held in LDS
+    // tiled list (k_dpd_tiled): per tile the neighbor set and its count,
+    // 16-bit entries two per word; columns of cell-list slots when tile_order
+    // is set
+    const uint32_t* tile_count;
+    const uint32_t* tile_jset;
+    const uint32_t* tile_list;
+    const uint32_t* tile_order;
+    const uint32_t* tile_nn;
+    uint32_t tile_slots;
     };
 
 // One type pair's parameters with the quantities that depend on the pair
@@ -101,8 +111,11 @@ __device__ __forceinline__ DPDPair make_dpd_pair(const DPDKernelArgs& a, uint32_
 // LDS_TABLE: the type-pair table is staged in LDS (up to 16 KiB, 17 types);
 // otherwise each in-cut pair derives its entry from the global parameter
 // arrays by the same operations (bit-identical, slower).
+// At least four wavefronts per SIMD (at most 128 VGPRs, a few spilled): the
+// kernel waits on its gathers more than it issues, and the fourth wavefront
+// measured 4 % faster than three without the bound.
 template<bool VIRIAL, bool LDS_TABLE>
-__global__ void __launch_bounds__(512) k_dpd(const DPDKernelArgs a)
+__global__ void __launch_bounds__(512, 4) k_dpd(const DPDKernelArgs a)
     {
     extern __shared__ double smem[];
     DPDPair* s_p = reinterpret_cast<DPDPair*>(smem);
@@ -216,6 +229,171 @@ __global__ void __launch_bounds__(512) k_dpd(const DPDKernelArgs a)
         }
     }
 
+// DPD over the tiled list (tiles.h): one workgroup per tile stages the FP64
+// positions of the tile's neighbor set in LDS (32 bytes a neighbor), so the
+// scan over every list entry reads its neighbors from LDS instead of
+// gathering them from global memory; the velocities and tags of the in-cut
+// neighbors (about a third of the entries) are still gathered.  Staging the
+// velocities and tags too (64 bytes a neighbor) left room for one workgroup
+// per CU and measured slower.  A list entry is 16 x the neighbor's rank in
+// the set.  The scan, the in-cut queue (16-bit ranks) and the evaluation are
+// those of k_dpd, entry for entry: the tiled columns hold the plain list's
+// rows in the same order, so the results are bit-identical to k_dpd over the
+// plain list.
+// Four wavefronts per SIMD, i.e. two 512-lane workgroups per CU (their LDS
+// fits twice), so one workgroup's staging overlaps the other's work; with
+// three (the unbounded register count) it ran slower than the plain kernel.
+template<bool VIRIAL>
+__global__ void __launch_bounds__(512, 4) k_dpd_tiled(const DPDKernelArgs a)
+    {
+    extern __shared__ double smem[];
+    const uint32_t nt = a.ntypes;
+    const uint32_t slots = a.tile_slots;
+    double4* s_pos = reinterpret_cast<double4*>(smem);
+    DPDPair* s_p = reinterpret_cast<DPDPair*>(s_pos + slots);
+    uint16_t* s_q = reinterpret_cast<uint16_t*>(s_p + nt * nt);
+    const uint32_t lb = xcd_logical_block();
+    const uint32_t bs = blockDim.x;
+    const uint32_t* js = a.tile_jset + (size_t)lb * kTileCap;
+    {
+        const uint32_t m_count = a.tile_count[lb];
+        for (uint32_t m = threadIdx.x; m < m_count; m += bs)
+            s_pos[m] = a.pos[js[m]];
+        for (uint32_t t = threadIdx.x; t < nt * nt; t += bs)
+            s_p[t] = make_dpd_pair(a, t);
+    }
+    __syncthreads();
+    const uint32_t ie = lb * bs + threadIdx.x;
+    if (ie >= a.N)
+        return;
+    const uint32_t pe = a.tile_order ? a.tile_order[ie] : ie;
+    const uint32_t nn = a.tile_order ? a.tile_nn[ie] : a.n_neigh[ie];
+    uint16_t* q_col = s_q + threadIdx.x;
+    const uint32_t qlen = a.queue_len;
+    const double4 pi = a.pos[pe];
+    const double4 vi = a.vel[pe];
+    const uint32_t tagi = a.tag[pe];
+    const uint32_t ti = (uint32_t)pi.w;
+    const double* rc_row = &s_p[ti * nt].rcutsq;
+    constexpr uint32_t rc_stride = sizeof(DPDPair) / sizeof(double);
+    const uint32_t* col = a.tile_list + ie;
+    const size_t pitch = a.nlist_pitch;
+    double fx = 0, fy = 0, fz = 0, pe_sum = 0;
+    double v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0, v5 = 0;
+    auto in_cut = [&](const double4& pj) {
+        const double dx = min_image_1d(pi.x - pj.x, a.box.L[0], a.box.Linv[0]);
+        const double dy = min_image_1d(pi.y - pj.y, a.box.L[1], a.box.Linv[1]);
+        const double dz = min_image_1d(pi.z - pj.z, a.box.L[2], a.box.Linv[2]);
+        const double rsq = dx * dx + dy * dy + dz * dz;
+        return rsq < rc_row[(uint32_t)pj.w * rc_stride];
+    };
+    auto rank_at = [&](uint32_t kk) {
+        const uint32_t w = col[(size_t)(kk >> 1) * pitch];
+        return ((kk & 1u) ? (w >> 16) : (w & 0xffffu)) >> 4;
+    };
+    uint32_t k = 0;
+    while (k < nn)
+        {
+        // scan: fill the queue with the next in-cut neighbors.  A full queue
+        // can stop the scan at an odd entry; one single step then realigns k
+        // to a word, so the four-entry steps read entries k .. k + 3 as two
+        // whole words.
+        uint32_t cnt = 0;
+        if (k & 1u)
+            {
+            const uint32_t r = rank_at(k);
+            ++k;
+            if (in_cut(s_pos[r]))
+                q_col[(cnt++) * bs] = (uint16_t)r;
+            }
+        while (k + 4 <= nn && cnt + 4 <= qlen)
+            {
+            const uint32_t w0 = col[(size_t)(k >> 1) * pitch];
+            const uint32_t w1 = col[(size_t)((k >> 1) + 1) * pitch];
+            const uint32_t r0 = (w0 & 0xffffu) >> 4, r1 = w0 >> 20;
+            const uint32_t r2 = (w1 & 0xffffu) >> 4, r3 = w1 >> 20;
+            const bool c0 = in_cut(s_pos[r0]), c1 = in_cut(s_pos[r1]);
+            const bool c2 = in_cut(s_pos[r2]), c3 = in_cut(s_pos[r3]);
+            if (c0)
+                q_col[(cnt++) * bs] = (uint16_t)r0;
+            if (c1)
+                q_col[(cnt++) * bs] = (uint16_t)r1;
+            if (c2)
+                q_col[(cnt++) * bs] = (uint16_t)r2;
+            if (c3)
+                q_col[(cnt++) * bs] = (uint16_t)r3;
+            k += 4;
+            }
+        while (k < nn && cnt < qlen)
+            {
+            const uint32_t r = rank_at(k);
+            ++k;
+            if (in_cut(s_pos[r]))
+                q_col[(cnt++) * bs] = (uint16_t)r;
+            }
+        // evaluate the queued pairs in row order
+        for (uint32_t c = 0; c < cnt; ++c)
+            {
+            const uint32_t r = q_col[c * bs];
+            const uint32_t j = js[r];
+            const double4 pj = s_pos[r];
+            const double4 vj = a.vel[j];
+            const uint32_t tagj = a.tag[j];
+            const double dx = min_image_1d(pi.x - pj.x, a.box.L[0], a.box.Linv[0]);
+            const double dy = min_image_1d(pi.y - pj.y, a.box.L[1], a.box.Linv[1]);
+            const double dz = min_image_1d(pi.z - pj.z, a.box.L[2], a.box.Linv[2]);
+            const double rsq = dx * dx + dy * dy + dz * dz;
+            const DPDPair q = s_p[ti * nt + (uint32_t)pj.w];
+            const double rdotv = dx * (vi.x - vj.x) + dy * (vi.y - vj.y) + dz * (vi.z - vj.z);
+            double fdivr, fcons, e;
+            dpd_eval(q, rsq, a.seed, tagi, tagj, a.timestep, rdotv, fdivr, fcons, e);
+            fx += dx * fdivr;
+            fy += dy * fdivr;
+            fz += dz * fdivr;
+            pe_sum += e * 0.5;
+            if (VIRIAL)
+                {
+                const double fd2 = fcons * 0.5;
+                v0 += fd2 * dx * dx;
+                v1 += fd2 * dx * dy;
+                v2 += fd2 * dx * dz;
+                v3 += fd2 * dy * dy;
+                v4 += fd2 * dy * dz;
+                v5 += fd2 * dz * dz;
+                }
+            }
+        }
+    a.force[pe] = make_double4(fx, fy, fz, pe_sum);
+    if (VIRIAL)
+        {
+        const size_t p = a.vpitch;
+        a.virial[0 * p + pe] = v0;
+        a.virial[1 * p + pe] = v1;
+        a.virial[2 * p + pe] = v2;
+        a.virial[3 * p + pe] = v3;
+        a.virial[4 * p + pe] = v4;
+        a.virial[5 * p + pe] = v5;
+        }
+    }
+
+// LDS of the tiled kernel: the staged neighbors, the type-pair table and a
+// queue of `qlen` 16-bit ranks per lane.
+static size_t dpd_tile_lds(uint32_t ntypes, uint32_t slots, uint32_t tile_size, uint32_t qlen)
+    {
+    return (size_t)slots * sizeof(double4) + (size_t)ntypes * ntypes * sizeof(DPDPair)
+           + (size_t)qlen * tile_size * sizeof(uint16_t);
+    }
+
+constexpr size_t kDpdTileLds = 160 * 1024; // a gfx950 workgroup's LDS
+constexpr uint32_t kDpdTileQueue = 16;     // in-cut ranks per lane (12.6 on average at density 3)
+
+bool dpd_tile_fits(uint32_t ntypes, uint32_t slots, uint32_t tile_size)
+    {
+    return tile_size != 0 && slots != 0 && slots <= kTileCap
+           && dpd_tile_lds(ntypes, slots, tile_size, kDpdTileQueue) <= kDpdTileLds;
+    }
+
+
 __global__ void k_dpd_alpha(uint16_t seed,
                             uint32_t timestep,
                             uint64_t n,
@@ -245,7 +423,7 @@ extern "C" int azp_gpu_compute_dpd_forces(const azp_pair_args* args, const void*
         return set_error(AZP_ERR_INVALID, "DPD needs params, velocities and tags");
     if (args->precision != AZP_PREC_FP64)
         return set_error(AZP_ERR_UNSUPPORTED, "DPD runs in the plain FP64 form only");
-    DPDKernelArgs ka;
+    DPDKernelArgs ka{};
     ka.pos = reinterpret_cast<const double4*>(p.d_pos);
     ka.vel = reinterpret_cast<const double4*>(p.d_vel);
     ka.tag = p.d_tag;
@@ -255,8 +433,6 @@ extern "C" int azp_gpu_compute_dpd_forces(const azp_pair_args* args, const void*
     ka.nlist = args->d_nlist;
     ka.nlist_pitch = args->nlist_pitch;
     ka.head_list = args->d_head_list;
-    if (!args->d_nlist)
-        return set_error(AZP_ERR_INVALID, "the DPD kernel reads the plain list (d_nlist)");
     ka.params = static_cast<const azp_param_dpd_general_weight*>(d_params);
     ka.rcutsq = args->d_rcutsq;
     ka.ntypes = args->ntypes;
@@ -267,6 +443,48 @@ extern "C" int azp_gpu_compute_dpd_forces(const azp_pair_args* args, const void*
     ka.force = reinterpret_cast<double4*>(args->out.d_force);
     ka.virial = args->out.d_virial;
     ka.vpitch = args->out.virial_pitch;
+    hipStream_t s = (hipStream_t)args->stream;
+    if (args->d_tile_list && !args->d_head_list && dpd_tile_fits(args->ntypes, args->tile_slots, args->tile_size))
+        {
+        if (!args->d_tile_count || !args->d_tile_jset || (!args->d_tile_order && !args->d_n_neigh)
+            || (args->d_tile_order && !args->d_tile_nn) || (args->tile_size != 256 && args->tile_size != 512))
+            return set_error(AZP_ERR_INVALID, "incomplete tiled list");
+        ka.tile_count = args->d_tile_count;
+        ka.tile_jset = args->d_tile_jset;
+        ka.tile_list = args->d_tile_list;
+        ka.tile_order = args->d_tile_order;
+        ka.tile_nn = args->d_tile_nn;
+        ka.tile_slots = args->tile_slots;
+        ka.queue_len = kDpdTileQueue;
+        const size_t shmem = dpd_tile_lds(args->ntypes, args->tile_slots, args->tile_size, kDpdTileQueue);
+        if (shmem > 64 * 1024)
+            {
+            // more than 64 KiB of dynamic LDS (gfx950 gives a workgroup 160 KiB)
+            static bool raised = false;
+            if (!raised)
+                {
+                for (const void* f : {reinterpret_cast<const void*>(&k_dpd_tiled<true>),
+                                      reinterpret_cast<const void*>(&k_dpd_tiled<false>)})
+                    {
+                    const hipError_t e
+                        = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kDpdTileLds);
+                    if (e != hipSuccess)
+                        return set_error(AZP_ERR_HIP, "hipFuncSetAttribute (dynamic LDS) failed: %s",
+                                         hipGetErrorString(e));
+                    }
+                raised = true;
+                }
+            }
+        const dim3 grid(grid_for(p.N, args->tile_size));
+        if (args->compute_virial)
+            hipLaunchKernelGGL((k_dpd_tiled<true>), grid, dim3(args->tile_size), shmem, s, ka);
+        else
+            hipLaunchKernelGGL((k_dpd_tiled<false>), grid, dim3(args->tile_size), shmem, s, ka);
+        AZP_LAUNCH_CHECK();
+        return AZP_OK;
+        }
+    if (!args->d_nlist)
+        return set_error(AZP_ERR_INVALID, "the DPD kernel reads the plain list (d_nlist) when no tiled list fits");
     const unsigned block = args->block_size ? args->block_size : 256;
     // type-pair table in LDS up to 16 KiB (17 types); beyond that it is
     // derived per pair from the global arrays
@@ -279,7 +497,6 @@ extern "C" int azp_gpu_compute_dpd_forces(const azp_pair_args* args, const void*
     if (ka.queue_len < 4)
         return set_error(AZP_ERR_INVALID, "DPD block size too large");
     const size_t shmem = table + (size_t)ka.queue_len * block * sizeof(uint32_t);
-    hipStream_t s = (hipStream_t)args->stream;
     const dim3 grid(grid_for(p.N, block));
     if (args->compute_virial)
         {

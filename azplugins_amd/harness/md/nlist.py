@@ -59,10 +59,11 @@ class NeighborList:
         return nt, m
 
     def _wants_tiles(self):
-        """The tiled list serves isotropic pair forces in both forms."""
+        """The tiled list serves isotropic pair forces in both forms and DPD."""
         from ... import _azp
 
-        return any(u._cpp is not None and u._cpp.driver in (_azp.DRIVER_PAIR, _azp.DRIVER_CONSERVATIVE)
+        return any(u._cpp is not None
+                   and u._cpp.driver in (_azp.DRIVER_PAIR, _azp.DRIVER_CONSERVATIVE, _azp.DRIVER_DPD_THERMO)
                    for u in self._users)
 
     def _needs_update(self, timestep):

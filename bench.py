@@ -49,7 +49,7 @@ KERNEL_SOURCES = {
                         "azplugins_amd/csrc/device_common.h", "azplugins_amd/csrc/tiles.h",
                         "azplugins_amd/csrc/k_pair_tiled.hip", "include/azp.h"),
     "pmc_dpd_2m.json": ("azplugins_amd/csrc/k_dpd.hip", "azplugins_amd/csrc/device_common.h",
-                        "azplugins_amd/csrc/portable_math.h", "include/azp.h"),
+                        "azplugins_amd/csrc/portable_math.h", "azplugins_amd/csrc/tiles.h", "include/azp.h"),
     "pmc_tpm_batch.json": ("azplugins_amd/csrc/k_batch.hip", "azplugins_amd/csrc/tpm.h",
                            "azplugins_amd/csrc/device_common.h", "include/azp.h"),
     "pmc_yukawa_262k.json": ("azplugins_amd/csrc/k_adjoint.hip", "azplugins_amd/csrc/pair_kernel.h",
@@ -484,18 +484,36 @@ def bench_dpd(args):
         pp.compute(nl.handle, parts, out, step[0])
         step[0] += 1
 
+    def result():
+        pp.compute(nl.handle, parts, out, 99)
+        torch.cuda.synchronize()
+        return force.clone(), virial[:, :N].clone()
+
+    # the plain list (global gathers), then the tiled list (neighbors in LDS)
+    t_plain = _events(torch, fn, max(5, args.steps // 5))
+    f_plain, v_plain = result()
+    nl.set_tiled(True)
+    nl.update(parts, force_rebuild=True)
+    tiles = nl.get_tiles()
     t = _events(torch, fn, max(5, args.steps // 5))
+    f_tiled, v_tiled = result()
+    same = bool(torch.equal(f_plain, f_tiled) and torch.equal(v_plain, v_tiled))
+    del f_plain, v_plain, f_tiled, v_tiled
     nn, lst = nl.download(N)
     pairs = count_list_pairs(torch, pos, nn, lst, N, L, [1.0])
     entries = int(nn.sum())
     # pos 32 + n_neigh 4 + head 8 + force 32 + virial 48 + vel 32 + tag 4 per particle, 4 per list entry
     nbytes = N * 160 + 4 * entries
+    kernel = "void azp::k_dpd_tiled<true>(azp::DPDKernelArgs)" if tiles[3] else "void azp::k_dpd<true, true>(azp::DPDKernelArgs)"
     res = {"value": pairs / t, "unit": "pair interactions/s", "ms_per_eval": t * 1e3, "N": N,
            "pairs_in_cut": pairs, "nlist_entries": entries, "precision": "fp64",
-           "roofline": roofline_line(nbytes, t, "void azp::k_dpd<true, true>(azp::DPDKernelArgs)", "pmc_dpd_2m.json",
-                                     entries),
+           "list": "tiled" if tiles[3] else "column", "tile_size": tiles[3], "tile_slots": tiles[4],
+           "plain_list_ms_per_eval": t_plain * 1e3, "tiled_equals_plain_bitwise": same,
+           "roofline": roofline_line(nbytes, t, kernel, "pmc_dpd_2m.json", entries),
            "note": "one DPD force evaluation (conservative + dissipative + Philox random) per step"}
-    log(f"bench: dpd_2m {t * 1e3:.3f} ms/eval, {pairs / t:.3e} pairs/s")
+    log(f"bench: dpd_2m plain list {t_plain * 1e3:.3f} ms/eval; tiled (size {tiles[3]}, slots {tiles[4]}) "
+        f"{t * 1e3:.3f} ms/eval, bit-identical {same}")
+    log(f"bench: dpd_2m {pairs / t:.3e} pairs/s")
     return res
 
 

@@ -191,12 +191,14 @@ typedef struct
     double dt;
     double kT;
     void* stream;
-    /* optional tiled list for the mixed form (azp_nlist_get_tiles); NULL
-     * d_tile_list = use d_nlist (and d_pos_fixed) only.  The tiled kernel
-     * converts positions to fixed point itself (d_pos_fixed is not read).
-     * d_tile_order / d_tile_nn (azp_nlist_get_tile_order) for tiles of
-     * cell-list slots, NULL for tiles of consecutive particles; d_nlist may
-     * be NULL when the tiled list is given. */
+    /* optional tiled list (azp_nlist_get_tiles), read by
+     * azp_gpu_compute_pair_forces (every form) and azp_gpu_compute_dpd_forces
+     * when the tile's neighbors fit in LDS; NULL d_tile_list = use d_nlist
+     * (and d_pos_fixed) only.  The tiled mixed kernel converts positions to
+     * fixed point itself (d_pos_fixed is not read).  d_tile_order / d_tile_nn
+     * (azp_nlist_get_tile_order) for tiles of cell-list slots, NULL for tiles
+     * of consecutive particles; d_nlist may be NULL when the tiled list is
+     * given and fits. */
     const uint32_t* d_tile_count;
     const uint32_t* d_tile_jset;
     const uint32_t* d_tile_list;
@@ -244,6 +246,10 @@ int azp_gpu_compute_pair_forces(const azp_pair_args* args, int evaluator, const 
  * AZP_ERR_INVALID; the other forms use no scratch.  Every call on a device
  * shares that array: calls that overlap on one device race. */
 int azp_gpu_compute_pair_forces_noalloc(const azp_pair_args* args, int evaluator, const void* d_params);
+/* PotentialPairDPDThermoGPU's launch (PotentialPairDPDThermoGPUKernel.cu.inc:21-24):
+ * FP64 only.  Over the tiled list when given and its neighbor positions (32
+ * bytes a neighbor) fit in LDS, else over d_nlist (column or head-list
+ * layout); the results are the same bit for bit. */
 int azp_gpu_compute_dpd_forces(const azp_pair_args* args, const void* d_params);
 int azp_gpu_compute_pair_aniso_forces(const azp_pair_args* args, int evaluator, const void* d_params);
 /* pos (double4) -> pos_fixed (int4) for the mixed-precision form */

@@ -64,17 +64,20 @@ DPD_C3 = {(0, 0): {"A": 25.0, "gamma": 4.5, "s": 0.5},
           (1, 1): {"A": 25.0, "gamma": 4.5, "s": 0.5}}
 
 
+@pytest.mark.parametrize("tiled", [False, True])
 @pytest.mark.parametrize("which", ["mixed_s", "workload"])
-def test_dpd_isolated_pairs_bit_exact(which):
-    """Each particle feels exactly one pair force: GPU == oracle bit for bit."""
+def test_dpd_isolated_pairs_bit_exact(which, tiled):
+    """Each particle feels exactly one pair force: GPU == oracle bit for bit,
+    over the plain list and over the tiled list (neighbors staged in LDS)."""
     params = DPD_PARAMS if which == "mixed_s" else DPD_C3
     L = 60.0
     pos4, vel4, tags = _isolated_pairs(3000, L, 1.0, 11)
     Lv = np.array([L, L, L])
     seed, t, dt, kT = (4321, 98765, 0.02, 1.3) if which == "mixed_s" else (3, 123457, 0.02, 1.0)
-    say(f"[dpd isolated] {which}")
+    say(f"[dpd isolated] {which} tiled={tiled}")
     r = gpu_pair("PotentialPairDPDThermoGeneralWeight", GpuSystem(pos4, Lv, vel4, tag=tags), 2,
-                 params, 1.0, thermostat=(seed, dt, kT), timestep=t)
+                 params, 1.0, thermostat=(seed, dt, kT), timestep=t, tiled=tiled)
+    assert (r.nl.get_tiles()[3] != 0) == tiled
     pairs = oracle.nlist(pos4, Lv, 2, oracle.r_listsq_matrix([1.0] * 4, 0.4), "brute")
     assert len(pairs[0]) == 3000
     fo, vo = oracle.dpd_forces(pos4, vel4, tags, Lv, pairs, 2, oracle.pack_matrix(4, params, 2),
@@ -87,10 +90,12 @@ def test_dpd_isolated_pairs_bit_exact(which):
     assert rel_err(r.force, fl) < 1e-13
 
 
-def test_dpd_many_types_bit_exact():
-    """20 particle types: the type-pair table (22 KB) exceeds the LDS budget of
-    the staged table, so the kernel derives each pair's entry from the global
-    arrays; isolated pairs still equal the oracle bit for bit."""
+@pytest.mark.parametrize("tiled", [False, True])
+def test_dpd_many_types_bit_exact(tiled):
+    """20 particle types: over the plain list the type-pair table (22 KB)
+    exceeds the LDS budget of the staged table, so the kernel derives each
+    pair's entry from the global arrays; the tiled kernel stages it beside the
+    neighbors.  Isolated pairs equal the oracle bit for bit either way."""
     L = 60.0
     nt = 20
     pos4, vel4, tags = _isolated_pairs(3000, L, 1.0, 13)
@@ -103,9 +108,10 @@ def test_dpd_many_types_bit_exact():
                               "s": float(rng.choice([0.5, 1.0, 2.0]))}
     Lv = np.array([L, L, L])
     seed, t, dt, kT = 99, 31337, 0.02, 1.1
-    say(f"[dpd many types] ntypes={nt}")
+    say(f"[dpd many types] ntypes={nt} tiled={tiled}")
     r = gpu_pair("PotentialPairDPDThermoGeneralWeight", GpuSystem(pos4, Lv, vel4, tag=tags), nt,
-                 params, 1.0, thermostat=(seed, dt, kT), timestep=t)
+                 params, 1.0, thermostat=(seed, dt, kT), timestep=t, tiled=tiled)
+    assert (r.nl.get_tiles()[3] != 0) == tiled
     pairs = oracle.nlist(pos4, Lv, nt, oracle.r_listsq_matrix([1.0] * nt * nt, 0.4), "brute")
     assert len(pairs[0]) == 3000
     fo, vo = oracle.dpd_forces(pos4, vel4, tags, Lv, pairs, nt, oracle.pack_matrix(4, params, nt),
@@ -138,6 +144,11 @@ def test_dpd_many_body():
     assert rel_err(r.virial, vo) < 1e-12
     # random and dissipative pair forces cancel in the sum (Newton's third law)
     assert np.abs(r.force[:, :3].sum(axis=0)).max() < 1e-9 * np.abs(r.force[:, :3]).max() * N
+    # the tiled list holds the same entries in the same order: the same bits
+    rt = gpu_pair("PotentialPairDPDThermoGeneralWeight", GpuSystem(pos4, Lv, vel4), 2, DPD_PARAMS, 1.0,
+                  thermostat=(seed, dt, kT), timestep=t, tiled=True)
+    assert rt.nl.get_tiles()[3] in (256, 512)
+    assert np.array_equal(rt.force, r.force) and np.array_equal(rt.virial, r.virial)
 
 
 def test_dpd_temperature():

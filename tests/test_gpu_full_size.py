@@ -15,7 +15,9 @@ A = 25 / 40 / 25, gamma 4.5, s 0.5, r_cut 1, kT 1, dt 0.02):
   1e-12 of the oracle (the pair terms are bit-identical; only the order of
   each particle's sum differs),
 * the total force vanishes (every random and dissipative pair force is
-  antisymmetric).
+  antisymmetric),
+* the kernel over the tiled list (neighbors staged in LDS) gives the same
+  bits as over the plain list.
 """
 
 import numpy as np
@@ -132,3 +134,10 @@ def test_dpd_2m_full_size():
     say("[full] dpd rel err force/energy/virial", err)
     assert max(err) < 1e-12
     assert np.abs(r.force[:, :3].sum(axis=0)).max() < 1e-14 * np.abs(r.force[:, :3]).max() * np.sqrt(N)
+    # over the tiled list (neighbors staged in LDS): the same entries in the
+    # same order, so the same bits
+    say("[full] gpu DPD over the tiled list ...")
+    rt = gpu_pair("PotentialPairDPDThermoGeneralWeight", GpuSystem(pos4, L, vel4, tag=tags), 2, c["params"],
+                  c["r_cut"], thermostat=(seed, c["dt"], c["kT"]), timestep=t, tiled=True)
+    assert rt.nl.get_tiles()[3] in (256, 512), "the tiled list was not built"
+    assert np.array_equal(rt.force, r.force) and np.array_equal(rt.virial, r.virial)
