@@ -4,7 +4,9 @@ Mirrors src/pytest/test_pair.py:309-363 and src/pytest/test_pair_aniso.py:113-16
 line for line (snapshot factory, nlist.Cell(buffer=0.4), dummy NVE integrator,
 ``sim.run(0)``, params read back after attach, energies/forces/torques to 4
 decimals), using the harness in place of HOOMD-blue.  Both precision forms run
-where the potential supports them.
+where the potential supports them; Colloid and DPD keep the plain FP64 form
+only, and asking them for the mixed form must fail at attach with that
+message.
 """
 
 import json
@@ -38,8 +40,7 @@ def _precisions(name):
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 @pytest.mark.parametrize("potential_test", PAIR_CASES, ids=lambda c: f"{c['potential']}-L{c['source_line']}")
 def test_energy_and_force(potential_test, precision):
-    if precision not in _precisions(potential_test["potential"]):
-        pytest.skip("plain FP64 form only")
+    refused = precision not in _precisions(potential_test["potential"])
     say(f"[golden] {potential_test['potential']} line {potential_test['source_line']} {precision}")
     r_cut = potential_test["r_cut"]
     r_buff = 0.4
@@ -61,6 +62,10 @@ def test_energy_and_force(potential_test, precision):
     potential.precision = precision
     integrator.forces = [potential]
     sim.operations.integrator = integrator
+    if refused:
+        with pytest.raises(RuntimeError, match="runs in the plain FP64 form only"):
+            sim.run(0)
+        return
     sim.run(0)
     assert potential.params[("A", "A")] == potential_test["params"]
     e = potential_test["energy"]

@@ -100,8 +100,6 @@ def test_iso_fp64_and_mixed(ev, mode):
     pos4[:, :3] -= Lv * np.floor(pos4[:, :3] / Lv + 0.5)
     nt = 2
     r_on = {k: 0.7 * v for k, v in ISO_RCUT[ev].items()} if isinstance(ISO_RCUT[ev], dict) else 0.7 * ISO_RCUT[ev]
-    if ev == 4 and mode != "none":
-        pytest.skip("PotentialPairConservative uses mode none here")
     cls = CONSERVATIVE_DPD if ev == 4 else CLASS[ev]
     say(f"[iso] evaluator={ev} mode={mode} N={pos4.shape[0]}")
     fo, vo, pairs = _oracle_iso(ev, pos4, Lv, nt, ISO_PARAMS[ev], ISO_RCUT[ev], mode, r_on)
