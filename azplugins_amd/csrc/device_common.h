@@ -205,4 +205,40 @@ __device__ __forceinline__ int32_t fixed_delta(int32_t a, int32_t b)
     return (int32_t)((uint32_t)a - (uint32_t)b);
     }
 
+// Stages `count` entries of a tile's neighbor set into LDS: each lane of the
+// workgroup takes slots threadIdx.x, + blockDim.x, ..., K at a time, with the
+// K index loads and then the K data loads all issued before the first store,
+// so a tile's staging waits on two memory latencies per K slots instead of
+// two per slot (the whole workgroup waits for its staging at a barrier).
+// load(j) reads neighbor j's data, store(m, v) writes slot m.
+template<uint32_t K, class Load, class Store>
+__device__ __forceinline__ void stage_slots(uint32_t count, const uint32_t* js, Load&& load, Store&& store)
+    {
+    const uint32_t bs = blockDim.x;
+    for (uint32_t base = threadIdx.x; base < count; base += K * bs)
+        {
+        uint32_t j[K];
+#pragma unroll
+        for (uint32_t k = 0; k < K; ++k)
+            {
+            const uint32_t m = base + k * bs;
+            j[k] = m < count ? js[m] : 0u;
+            }
+        decltype(load(0u)) v[K];
+#pragma unroll
+        for (uint32_t k = 0; k < K; ++k)
+            {
+            if (base + k * bs < count)
+                v[k] = load(j[k]);
+            }
+#pragma unroll
+        for (uint32_t k = 0; k < K; ++k)
+            {
+            const uint32_t m = base + k * bs;
+            if (m < count)
+                store(m, v[k]);
+            }
+        }
+    }
+
 } // namespace azp

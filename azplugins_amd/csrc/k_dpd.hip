@@ -255,13 +255,11 @@ __global__ void __launch_bounds__(512, 4) k_dpd_tiled(const DPDKernelArgs a)
     const uint32_t lb = xcd_logical_block();
     const uint32_t bs = blockDim.x;
     const uint32_t* js = a.tile_jset + (size_t)lb * kTileCap;
-    {
-        const uint32_t m_count = a.tile_count[lb];
-        for (uint32_t m = threadIdx.x; m < m_count; m += bs)
-            s_pos[m] = a.pos[js[m]];
-        for (uint32_t t = threadIdx.x; t < nt * nt; t += bs)
-            s_p[t] = make_dpd_pair(a, t);
-    }
+    stage_slots<8>(
+        a.tile_count[lb], js, [&](uint32_t j) { return a.pos[j]; },
+        [&](uint32_t m, const double4& p) { s_pos[m] = p; });
+    for (uint32_t t = threadIdx.x; t < nt * nt; t += bs)
+        s_p[t] = make_dpd_pair(a, t);
     __syncthreads();
     const uint32_t ie = lb * bs + threadIdx.x;
     if (ie >= a.N)

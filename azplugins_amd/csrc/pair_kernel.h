@@ -423,8 +423,9 @@ __global__ void __launch_bounds__(512) k_pair(const PairKernelArgs a)
         s_pos = reinterpret_cast<int4*>(smem);
         tile_js = a.tile_jset + (size_t)lb * kTileCap;
         tile_m = a.tile_count[lb];
-        for (uint32_t m = threadIdx.x; m < tile_m; m += blockDim.x)
-            s_pos[m] = fixed4(a.pos[tile_js[m]], a.box);
+        stage_slots<8>(
+            tile_m, tile_js, [&](uint32_t j) { return a.pos[j]; },
+            [&](uint32_t m, const double4& p) { s_pos[m] = fixed4(p, a.box); });
         if (threadIdx.x == 0)
             s_pos[tile_m] = make_int4(0, 0, 0, 0);
         }
@@ -434,8 +435,9 @@ __global__ void __launch_bounds__(512) k_pair(const PairKernelArgs a)
         double4* s_pos64 = reinterpret_cast<double4*>(smem);
         tile_js = a.tile_jset + (size_t)lb * kTileCap;
         tile_m = a.tile_count[lb];
-        for (uint32_t m = threadIdx.x; m < tile_m; m += blockDim.x)
-            s_pos64[m] = a.pos[tile_js[m]];
+        stage_slots<8>(
+            tile_m, tile_js, [&](uint32_t j) { return a.pos[j]; },
+            [&](uint32_t m, const double4& p) { s_pos64[m] = p; });
         if (threadIdx.x == 0)
             s_pos64[tile_m] = make_double4(0.0, 0.0, 0.0, 0.0);
         }
