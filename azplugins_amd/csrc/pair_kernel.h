@@ -366,10 +366,18 @@ template<class Eval> static bool tile_fits(uint32_t ntypes, uint32_t slots)
     return tables + (size_t)slots * sizeof(int4) <= kTileLds;
     }
 
+// Bytes of one staged neighbor in the FP64 form over the tiled list: x, y, z
+// with one particle type (so that two workgroups of a tile at the 1M workload
+// fit in a CU's LDS), x, y, z, type otherwise.
+__host__ __device__ constexpr size_t fp64_slot_bytes(bool single)
+    {
+    return single ? 3 * sizeof(double) : sizeof(double4);
+    }
+
 template<class Eval> static bool tile_fits64(uint32_t ntypes, uint32_t slots)
     {
     const size_t tables = ntypes == 1 ? 0 : table_bytes<Eval, double>(ntypes * ntypes);
-    return tables + (size_t)slots * sizeof(double4) <= kTileLds64;
+    return tables + (size_t)slots * fp64_slot_bytes(ntypes == 1) <= kTileLds64;
     }
 
 // Neighbor-list layouts a kernel instance reads.
@@ -392,11 +400,11 @@ __global__ void __launch_bounds__(512) k_pair(const PairKernelArgs a)
     const uint32_t nt = a.ntypes;
     const uint32_t nt2 = nt * nt;
     // tiled: the neighbor positions sit at LDS offset 0 (a list entry,
-    // 16 x rank, is their byte address in the mixed form's 16-byte slots and
-    // half of it in the FP64 form's 32-byte slots) and the type-pair tables
-    // after them
-    using SlotT = std::conditional_t<MIXED, int4, double4>;
-    Prep* s_prep = reinterpret_cast<Prep*>(reinterpret_cast<char*>(smem) + (TILED ? a.tile_slots * sizeof(SlotT) : 0));
+    // 16 x rank, is their byte address in the mixed form's 16-byte slots, and
+    // 3/2 or 2 x it in the FP64 form's 24- or 32-byte slots) and the
+    // type-pair tables after them
+    constexpr size_t kSlotBytes = MIXED ? sizeof(int4) : fp64_slot_bytes(SINGLE);
+    Prep* s_prep = reinterpret_cast<Prep*>(reinterpret_cast<char*>(smem) + (TILED ? a.tile_slots * kSlotBytes : 0));
     TypePairExtra<R>* s_x = reinterpret_cast<TypePairExtra<R>*>(s_prep + nt2);
 
     Prep q0;
@@ -431,15 +439,32 @@ __global__ void __launch_bounds__(512) k_pair(const PairKernelArgs a)
         }
     if constexpr (TILED && !MIXED)
         {
-        // FP64 form: the FP64 positions themselves (32-byte slots)
-        double4* s_pos64 = reinterpret_cast<double4*>(smem);
+        // FP64 form: the FP64 positions themselves (x, y, z in 24-byte slots
+        // with one type, x, y, z, type in 32-byte slots otherwise)
         tile_js = a.tile_jset + (size_t)lb * kTileCap;
         tile_m = a.tile_count[lb];
-        stage_slots<8>(
-            tile_m, tile_js, [&](uint32_t j) { return a.pos[j]; },
-            [&](uint32_t m, const double4& p) { s_pos64[m] = p; });
-        if (threadIdx.x == 0)
-            s_pos64[tile_m] = make_double4(0.0, 0.0, 0.0, 0.0);
+        if constexpr (SINGLE)
+            {
+            double* s_pos3 = reinterpret_cast<double*>(smem);
+            stage_slots<8>(
+                tile_m, tile_js, [&](uint32_t j) { return a.pos[j]; },
+                [&](uint32_t m, const double4& p) {
+                    s_pos3[3 * m] = p.x;
+                    s_pos3[3 * m + 1] = p.y;
+                    s_pos3[3 * m + 2] = p.z;
+                });
+            if (threadIdx.x == 0)
+                s_pos3[3 * tile_m] = s_pos3[3 * tile_m + 1] = s_pos3[3 * tile_m + 2] = 0.0;
+            }
+        else
+            {
+            double4* s_pos64 = reinterpret_cast<double4*>(smem);
+            stage_slots<8>(
+                tile_m, tile_js, [&](uint32_t j) { return a.pos[j]; },
+                [&](uint32_t m, const double4& p) { s_pos64[m] = p; });
+            if (threadIdx.x == 0)
+                s_pos64[tile_m] = make_double4(0.0, 0.0, 0.0, 0.0);
+            }
         }
     if (!SINGLE || TILED)
         __syncthreads();
@@ -488,7 +513,17 @@ __global__ void __launch_bounds__(512) k_pair(const PairKernelArgs a)
             double4 pj[U];
 #pragma unroll
             for (uint32_t u = 0; u < U; ++u)
-                pj[u] = *reinterpret_cast<const double4*>(s_base + 2u * e[u]);
+                {
+                if constexpr (SINGLE)
+                    {
+                    const double* p3 = reinterpret_cast<const double*>(s_base + e[u] + (e[u] >> 1));
+                    pj[u] = make_double4(p3[0], p3[1], p3[2], 0.0);
+                    }
+                else
+                    {
+                    pj[u] = *reinterpret_cast<const double4*>(s_base + 2u * e[u]);
+                    }
+                }
 #pragma unroll
             for (uint32_t u = 0; u < U; ++u)
                 {
@@ -653,7 +688,7 @@ static int launch_t(const PairKernelArgs& ka, unsigned block, unsigned slots, hi
     PairKernelArgs kb = ka;
     kb.tile_slots = TILED ? slots : 0u;
     if (TILED)
-        shmem += (size_t)slots * (MIXED ? sizeof(int4) : sizeof(double4));
+        shmem += (size_t)slots * (MIXED ? sizeof(int4) : fp64_slot_bytes(SINGLE));
     const size_t cap = (TILED && !MIXED) ? kTileLds64 : kTileLds;
     if (shmem > cap)
         return set_error(AZP_ERR_UNSUPPORTED, "too many particle types (%u) for LDS staging", ka.ntypes);

@@ -343,8 +343,8 @@ def _fp64_pair(cls, sysm, nt, params, rc, **kw):
 @pytest.mark.parametrize("mode", ["none", "shift", "xplor"])
 def test_fp64_tiled_equals_plain_and_oracle(mode):
     """The FP64 form over the tiled list reads FP64 positions from LDS (32-byte
-    slots): the same terms in the same order as over the plain list, so the
-    results are bit-identical, and within 1e-12 of the oracle."""
+    slots with two types): the same terms in the same order as over the plain
+    list, so the results are bit-identical, and within 1e-12 of the oracle."""
     L = 16.0
     pos4, _ = jittered_lattice(16, L, 0.15, 2, seed=43)
     Lv = np.array([L, L * 1.02, L])
@@ -376,6 +376,36 @@ def test_fp64_tiled_equals_plain_and_oracle(mode):
         err = max(rel_err(tiled.force[:, :3], fo[:, :3]), rel_err(tiled.force[:, 3], fo[:, 3]),
                   rel_err(tiled.virial, vo))
         say(f"  fp64 tiled vs oracle {err:.2e}")
+        assert err < 1e-12
+
+
+@pytest.mark.parametrize("mode", ["none", "shift", "xplor"])
+def test_fp64_tiled_one_type(mode):
+    """With one particle type the FP64 form stages only x, y, z (24-byte slots,
+    the entry's byte offset times 3/2): bit-identical to the plain list and
+    within 1e-12 of the oracle, for tiles of both sizes."""
+    L = 18.0
+    pos4, _ = jittered_lattice(18, L, 0.15, 1, seed=44)
+    Lv = np.array([L, L * 1.01, L * 0.99])
+    pos4[:, :3] -= Lv * np.floor(pos4[:, :3] / Lv + 0.5)
+    sysm = GpuSystem(pos4, Lv)
+    params = {(0, 0): {"epsilon": 1.0, "sigma": 1.0, "attraction_scale_factor": 0.5}}
+    rc = 2.5
+    r_on = 0.7 * rc if mode == "xplor" else None
+    pairs = oracle.nlist(pos4, Lv, 1, oracle.r_listsq_matrix([rc], 0.4), "cell")
+    fo, vo = oracle.pair_forces(oracle.PERTURBED_LJ, pos4, Lv, pairs, 1,
+                                oracle.pack_matrix(oracle.PERTURBED_LJ, params, 1), np.array([rc * rc]),
+                                np.array([(0.7 * rc) ** 2]), mode)
+    plain = gpu_pair("PotentialPairPerturbedLennardJones", sysm, 1, params, rc, mode=mode, r_on=r_on)
+    for size in (256, 512):
+        tiled = gpu_pair("PotentialPairPerturbedLennardJones", sysm, 1, params, rc, mode=mode, r_on=r_on,
+                         tiled=size)
+        say(f"[tiles fp64 one type] {mode}: tile size {tiled.nl.get_tiles()[3]}")
+        assert tiled.nl.get_tiles()[3] == size
+        assert np.array_equal(plain.force, tiled.force)
+        assert np.array_equal(plain.virial, tiled.virial)
+        err = max(rel_err(tiled.force[:, :3], fo[:, :3]), rel_err(tiled.force[:, 3], fo[:, 3]),
+                  rel_err(tiled.virial, vo))
         assert err < 1e-12
 
 
