@@ -9,6 +9,8 @@
   against the padded column layout: the same entries in the same order, so
   forces, energies, virials and torques are bit-identical, for the isotropic
   FP64 and mixed forms, DPD with its thermostat, and TwoPatchMorse.
+* The DPD launcher over the tiled list (with and without d_nlist) against
+  the column layout of the same build: bit-identical.
 """
 
 import ctypes
@@ -201,3 +203,48 @@ def test_head_list_dpd_and_aniso_bit_identical():
             assert np.array_equal(x[:, :N] if x.shape[0] == 6 else x, y[:, :N] if y.shape[0] == 6 else y)
         assert np.array_equal(results[0][0], ref.force)
         assert np.array_equal(results[0][2][:, :3], ref.torque[:, :3])
+
+
+def test_dpd_launcher_tiled_list_bit_identical():
+    """azp_gpu_compute_dpd_forces given the tiled list (azp_nlist_get_tiles,
+    azp_nlist_get_tile_order; the neighbor positions staged in LDS) gives the
+    same bits as the column layout of the same build, and as the launcher
+    with d_nlist NULL."""
+    from azplugins_amd import _azplugins
+
+    L = 12.0
+    rng = np.random.default_rng(21)
+    N = int(3.0 * L**3)
+    pos4 = np.zeros((N, 4))
+    pos4[:, :3] = rng.uniform(-L / 2, L / 2, size=(N, 3))
+    pos4[:, 3] = rng.integers(0, 2, size=N)
+    vel4 = np.zeros((N, 4))
+    vel4[:, :3] = rng.normal(size=(N, 3))
+    vel4[:, 3] = 1.0
+    sysm = GpuSystem(pos4, np.array([L, L, L]), vel4)
+    rows = [[25.0, 4.5, 0.5], [40.0, 4.5, 0.5], [40.0, 4.5, 0.5], [25.0, 4.5, 0.5]]
+    d_params = _params_dev(_azp.EVAL_DPD_GENERAL_WEIGHT, rows)
+    rcsq = torch.ones(4, dtype=torch.float64, device="cuda")
+    nl = _azplugins.NeighborListGPU(0.4)
+    nl.set_r_cut(2, [1.0] * 4)
+    nl.set_tiled(True)
+    nl.update(sysm.particles())
+    tiles = nl.get_tiles()
+    assert tiles[3] in (256, 512)
+    order = nl.get_tile_order()
+    results = []
+    for form in ("column", "tiled", "tiled without d_nlist"):
+        f, v, _, pitch = _outputs(N)
+        a = raw_args(sysm, nl, rcsq, None, 2, f, v, pitch)
+        a.seed, a.timestep, a.dt, a.kT = 7, 321, 0.02, 1.0
+        if form != "column":
+            a.d_tile_count, a.d_tile_jset, a.d_tile_list, a.tile_size, a.tile_slots = tiles
+            a.d_tile_order, a.d_tile_nn = order
+        if form == "tiled without d_nlist":
+            a.d_nlist = None
+        say(f"[launcher] dpd {form}")
+        _azp.call("azp_gpu_compute_dpd_forces", ctypes.byref(a), _azp.dptr(d_params))
+        results.append(_host(f, v))
+    for r in results[1:]:
+        assert np.array_equal(results[0][0], r[0])
+        assert np.array_equal(results[0][1][:, :N], r[1][:, :N])
