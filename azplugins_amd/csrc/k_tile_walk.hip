@@ -227,7 +227,11 @@ __device__ uint32_t tile_walk_one(const TileWalkArgs& a,
     // The flags are named scalars, not arrays: an array of flags becomes a
     // packed byte vector in VGPRs, while scalars stay lane masks.
     static_assert(kWalkBatch == 4, "the batch below is written out for four candidates");
-    auto batch = [&](const int4 (&pc)[kWalkBatch], uint32_t m0, uint32_t e) {
+    // CHECKED false: the wavefront knows that every lane's four candidates lie
+    // inside its cell and that the cell is not the lane's own (no range or
+    // self test per candidate)
+    auto batch = [&](const int4 (&pc)[kWalkBatch], uint32_t m0, uint32_t e, auto checked) {
+        constexpr bool CHECKED = decltype(checked)::value;
         // offsets from m0: candidate q is in the range when q < rem and is
         // the lane's own particle when q == dself (which wraps to a large
         // value when self < m0)
@@ -237,7 +241,7 @@ __device__ uint32_t tile_walk_one(const TileWalkArgs& a,
             const uint32_t tj = (uint32_t)c.w;
             // with one type, a candidate of an invalid type can only exist
             // when its own lane reports it, which fails the build
-            const bool valid = (q < rem) & (q != dself) & (SINGLE || tj < a.ntypes);
+            const bool valid = (!CHECKED || ((q < rem) & (q != dself))) & (SINGLE || tj < a.ntypes);
             const uint32_t tp = SINGLE ? 0u : ti * a.ntypes + min(tj, a.ntypes - 1);
             const float lo = SINGLE ? lo0 : s.lohi[tp].x * unit2;
             const float hi = SINGLE ? hi0 : s.lohi[tp].y * unit2;
@@ -339,13 +343,24 @@ __device__ uint32_t tile_walk_one(const TileWalkArgs& a,
             load_batch(pb, m0 + kWalkBatch);
             __builtin_amdgcn_sched_barrier(0);
             store_pending();
-            batch(pa, m0, e);
+            // both batches whole for every lane still in its cell, and not
+            // the own cell (stencil entry 13): the tests need no masks
+            if (k != 13u && __ballot(m0 + 2 * kWalkBatch > e) == 0ull)
+                {
+                batch(pa, m0, e, std::false_type{});
+                load_batch(pa, m0 + 2 * kWalkBatch);
+                __builtin_amdgcn_sched_barrier(0);
+                store_pending();
+                batch(pb, m0 + kWalkBatch, e, std::false_type{});
+                continue;
+                }
+            batch(pa, m0, e, std::true_type{});
             if (m0 + kWalkBatch < e)
                 {
                 load_batch(pa, m0 + 2 * kWalkBatch);
                 __builtin_amdgcn_sched_barrier(0);
                 store_pending();
-                batch(pb, m0 + kWalkBatch, e);
+                batch(pb, m0 + kWalkBatch, e, std::true_type{});
                 }
             }
         }
