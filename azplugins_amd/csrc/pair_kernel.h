@@ -239,6 +239,13 @@ __device__ __forceinline__ uint32_t wave_max(uint32_t v)
     return v;
     }
 
+__device__ __forceinline__ uint32_t wave_min(uint32_t v)
+    {
+    for (int off = 32; off > 0; off >>= 1)
+        v = min(v, (uint32_t)__shfl_xor((int)v, off));
+    return v;
+    }
+
 // Accumulators of the mixed form are kept in integer-grid units of the
 // fixed-point positions (dx = n * L / 2^32): sum f n and sum f n_a n_b are
 // formed from the integer displacements (as FP32) and scaled once at the end.
@@ -494,8 +501,10 @@ __global__ void __launch_bounds__(512) k_pair(const PairKernelArgs a)
         const uint32_t col_off = ie * 4u;
         const uint32_t tile_m16 = tile_m << 4; // the padding entry
         const char* s_base = reinterpret_cast<const char*>(smem);
-        for (uint32_t k = 0; k < P; k += U)
-            {
+        // below F every lane of the wavefront has U real entries per step
+        const uint32_t F = __builtin_amdgcn_readfirstlane(wave_min(nn) & ~(U - 1));
+        auto step = [&](uint32_t k, auto padded) {
+            constexpr bool PADDED = decltype(padded)::value;
             uint32_t w[U / 2];
 #pragma unroll
             for (uint32_t u = 0; u < U / 2; ++u)
@@ -532,9 +541,14 @@ __global__ void __launch_bounds__(512) k_pair(const PairKernelArgs a)
                 const double dz = min_image_1d(pi.z - pj[u].z, a.box.L[2], a.box.Linv[2]);
                 const uint32_t tp = SINGLE ? 0u : ti * nt + (uint32_t)pj[u].w;
                 term_fp64<Eval, VIRIAL, XPLOR>(SINGLE ? q0 : s_prep[tp], SINGLE ? x0 : s_x[tp],
-                                               e[u] != tile_m16, dx, dy, dz, acc);
+                                               !PADDED || e[u] != tile_m16, dx, dy, dz, acc);
                 }
-            }
+        };
+        uint32_t k = 0;
+        for (; k < F; k += U)
+            step(k, std::false_type{});
+        for (; k < P; k += U)
+            step(k, std::true_type{});
         if (i < a.N)
             store<VIRIAL>(a, pe, acc);
         }
@@ -578,9 +592,12 @@ __global__ void __launch_bounds__(512) k_pair(const PairKernelArgs a)
         const uint32_t ti = (uint32_t)pi.w;
         const uint32_t col_off = ie * 4u;
         const uint32_t tile_m16 = tile_m << 4; // the padding entry
+        // below F every lane of the wavefront has U real entries per step:
+        // those steps skip the padding test
+        const uint32_t F = __builtin_amdgcn_readfirstlane(wave_min(nn) & ~(U - 1));
         Acc32 part;
-        for (uint32_t k = 0; k < P; k += U)
-            {
+        auto step = [&](uint32_t k, auto padded) {
+            constexpr bool PADDED = decltype(padded)::value;
             uint32_t w[U / 2];
 #pragma unroll
             for (uint32_t u = 0; u < U / 2; ++u)
@@ -604,7 +621,7 @@ __global__ void __launch_bounds__(512) k_pair(const PairKernelArgs a)
                 {
                 const uint32_t tp = SINGLE ? 0u : ti * nt + (uint32_t)pj[u].w;
                 term_mixed<Eval, VIRIAL, XPLOR, CUBIC>(a, SINGLE ? q0 : s_prep[tp], SINGLE ? x0 : s_x[tp],
-                                                       e[u] != tile_m16, pe, e[u] >> 4, tile_js,
+                                                       !PADDED || e[u] != tile_m16, pe, e[u] >> 4, tile_js,
                                                        fixed_delta(pi.x, pj[u].x), fixed_delta(pi.y, pj[u].y),
                                                        fixed_delta(pi.z, pj[u].z), part);
                 if (ACC64)
@@ -618,7 +635,12 @@ __global__ void __launch_bounds__(512) k_pair(const PairKernelArgs a)
                 flush32<VIRIAL>(acc, part);
                 part = Acc32();
                 }
-            }
+        };
+        uint32_t k = 0;
+        for (; k < F; k += U)
+            step(k, std::false_type{});
+        for (; k < P; k += U)
+            step(k, std::true_type{});
         if (i < a.N)
             store_scaled<VIRIAL>(a, pe, acc);
         }
