@@ -230,10 +230,11 @@ __global__ void __launch_bounds__(512, 4) k_dpd(const DPDKernelArgs a)
     }
 
 // DPD over the tiled list (tiles.h): one workgroup per tile stages the FP64
-// positions of the tile's neighbor set in LDS (32 bytes a neighbor), so the
-// scan over every list entry reads its neighbors from LDS instead of
-// gathering them from global memory; the velocities and tags of the in-cut
-// neighbors (about a third of the entries) are still gathered.  Staging the
+// positions of the tile's neighbor set in LDS (32 bytes a neighbor, the
+// fourth word holding the type and the particle index), so the scan over
+// every list entry reads its neighbors from LDS instead of gathering them
+// from global memory; the velocities and tags of the in-cut neighbors (about
+// a third of the entries) are still gathered.  Staging the
 // velocities and tags too (64 bytes a neighbor) left room for one workgroup
 // per CU and measured slower.  A list entry is 16 x the neighbor's rank in
 // the set.  The scan, the in-cut queue (16-bit ranks) and the evaluation are
@@ -255,8 +256,16 @@ __global__ void __launch_bounds__(512, 4) k_dpd_tiled(const DPDKernelArgs a)
     const uint32_t lb = xcd_logical_block();
     const uint32_t bs = blockDim.x;
     const uint32_t* js = a.tile_jset + (size_t)lb * kTileCap;
+    // slot m: the neighbor's x, y, z and, in the fourth word, its type (low
+    // half) and particle index (high half), so the evaluation gathers its
+    // velocity and tag without reading the neighbor set again
     stage_slots<8>(
-        a.tile_count[lb], js, [&](uint32_t j) { return a.pos[j]; },
+        a.tile_count[lb], js,
+        [&](uint32_t j) {
+            double4 p = a.pos[j];
+            p.w = __hiloint2double((int)j, (int)(uint32_t)p.w);
+            return p;
+        },
         [&](uint32_t m, const double4& p) { s_pos[m] = p; });
     for (uint32_t t = threadIdx.x; t < nt * nt; t += bs)
         s_p[t] = make_dpd_pair(a, t);
@@ -283,7 +292,7 @@ __global__ void __launch_bounds__(512, 4) k_dpd_tiled(const DPDKernelArgs a)
         const double dy = min_image_1d(pi.y - pj.y, a.box.L[1], a.box.Linv[1]);
         const double dz = min_image_1d(pi.z - pj.z, a.box.L[2], a.box.Linv[2]);
         const double rsq = dx * dx + dy * dy + dz * dz;
-        return rsq < rc_row[(uint32_t)pj.w * rc_stride];
+        return rsq < rc_row[(uint32_t)__double2loint(pj.w) * rc_stride];
     };
     auto rank_at = [&](uint32_t kk) {
         const uint32_t w = col[(size_t)(kk >> 1) * pitch];
@@ -333,15 +342,15 @@ __global__ void __launch_bounds__(512, 4) k_dpd_tiled(const DPDKernelArgs a)
         for (uint32_t c = 0; c < cnt; ++c)
             {
             const uint32_t r = q_col[c * bs];
-            const uint32_t j = js[r];
             const double4 pj = s_pos[r];
+            const uint32_t j = (uint32_t)__double2hiint(pj.w);
             const double4 vj = a.vel[j];
             const uint32_t tagj = a.tag[j];
             const double dx = min_image_1d(pi.x - pj.x, a.box.L[0], a.box.Linv[0]);
             const double dy = min_image_1d(pi.y - pj.y, a.box.L[1], a.box.Linv[1]);
             const double dz = min_image_1d(pi.z - pj.z, a.box.L[2], a.box.Linv[2]);
             const double rsq = dx * dx + dy * dy + dz * dz;
-            const DPDPair q = s_p[ti * nt + (uint32_t)pj.w];
+            const DPDPair q = s_p[ti * nt + (uint32_t)__double2loint(pj.w)];
             const double rdotv = dx * (vi.x - vj.x) + dy * (vi.y - vj.y) + dz * (vi.z - vj.z);
             double fdivr, fcons, e;
             dpd_eval(q, rsq, a.seed, tagi, tagj, a.timestep, rdotv, fdivr, fcons, e);
