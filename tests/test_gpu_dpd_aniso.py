@@ -151,6 +151,44 @@ def test_dpd_many_body():
     assert np.array_equal(rt.force, r.force) and np.array_equal(rt.virial, r.virial)
 
 
+def test_dpd_three_types_orthorhombic_tiled():
+    """Three types with per-pair cutoffs (1.0, 0.9, 1.2) in an orthorhombic
+    box: the tiled kernel (a type-pair table of nine entries, the type in the
+    staged slot) equals the plain kernel bit for bit and the oracle to 1e-12,
+    and the neighbor sets equal the oracle's."""
+    Lv = np.array([11.0, 12.5, 10.0])
+    rng = np.random.default_rng(31)
+    N = int(3.0 * np.prod(Lv))
+    pos4 = np.zeros((N, 4))
+    pos4[:, :3] = rng.uniform(-0.5, 0.5, size=(N, 3)) * Lv
+    pos4[:, 3] = rng.integers(0, 3, size=N)
+    vel4 = np.zeros((N, 4))
+    vel4[:, :3] = rng.normal(size=(N, 3))
+    vel4[:, 3] = 1.0
+    params, rcut = {}, {}
+    for a in range(3):
+        for b in range(a, 3):
+            params[(a, b)] = {"A": 20.0 + 5.0 * (a + b), "gamma": 4.5 - 0.5 * a, "s": (0.5, 1.0, 2.0)[b]}
+            rcut[(a, b)] = (1.0, 0.9, 1.2)[(a + b) % 3]
+    seed, t, dt, kT = 17, 2024, 0.02, 1.2
+    say(f"[dpd three types] N={N}")
+    sysm = GpuSystem(pos4, Lv, vel4)
+    r = gpu_pair("PotentialPairDPDThermoGeneralWeight", sysm, 3, params, rcut, thermostat=(seed, dt, kT),
+                 timestep=t)
+    rt = gpu_pair("PotentialPairDPDThermoGeneralWeight", sysm, 3, params, rcut, thermostat=(seed, dt, kT),
+                  timestep=t, tiled=True)
+    assert rt.nl.get_tiles()[3] in (256, 512)
+    assert np.array_equal(rt.force, r.force) and np.array_equal(rt.virial, r.virial)
+    rc = np.array([rcut[(min(a, b), max(a, b))] for a in range(3) for b in range(3)])
+    pairs = oracle.nlist(pos4, Lv, 3, oracle.r_listsq_matrix(rc, 0.4), "cell")
+    assert nlist_pairs(r.nl, N) == set(zip(pairs[0].tolist(), pairs[1].tolist()))
+    fo, vo = oracle.dpd_forces(pos4, vel4, np.arange(N), Lv, pairs, 3, oracle.pack_matrix(4, params, 3),
+                               rc * rc, seed, t, dt, kT)
+    assert rel_err(r.force[:, :3], fo[:, :3]) < 1e-12
+    assert rel_err(r.force[:, 3], fo[:, 3]) < 1e-12
+    assert rel_err(r.virial, vo) < 1e-12
+
+
 def test_dpd_temperature():
     """The reference's statistical test (test_pair_dpd.py:13-46) on the GPU."""
     import azplugins_amd.harness as hoomd
