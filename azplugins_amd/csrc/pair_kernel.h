@@ -246,6 +246,24 @@ __device__ __forceinline__ uint32_t wave_min(uint32_t v)
     return v;
     }
 
+// U consecutive entries of one lane's head-list row (U a multiple of 4) as
+// 16-byte loads; a row starts at any 4-byte boundary, and the copy lets the
+// compiler pick the loads that alignment allows.
+template<uint32_t U> __device__ __forceinline__ void head_words(const uint32_t* p, uint32_t (&jj)[U])
+    {
+    static_assert(U % 4 == 0, "whole 16-byte groups");
+#pragma unroll
+    for (uint32_t u = 0; u < U; u += 4)
+        {
+        uint4 w;
+        __builtin_memcpy(&w, p + u, sizeof(w));
+        jj[u] = w.x;
+        jj[u + 1] = w.y;
+        jj[u + 2] = w.z;
+        jj[u + 3] = w.w;
+        }
+    }
+
 // Accumulators of the mixed form are kept in integer-grid units of the
 // fixed-point positions (dx = n * L / 2^32): sum f n and sum f n_a n_b are
 // formed from the integer displacements (as FP32) and scaled once at the end.
@@ -557,16 +575,23 @@ __global__ void __launch_bounds__(512) k_pair(const PairKernelArgs a)
         const double4 pi = a.pos[ie];
         const uint32_t ti = (uint32_t)pi.w;
         const uint32_t* hrow = HEAD ? a.nlist + a.head_list[ie] : nullptr;
-        for (uint32_t k = 0; k < P; k += U)
-            {
+        // head list: below F every lane has U entries left in its row, read as
+        // 16-byte vectors (one lane's row is contiguous; lanes' rows are not)
+        const uint32_t F = HEAD ? __builtin_amdgcn_readfirstlane(wave_min(nn) & ~(U - 1)) : 0u;
+        auto step = [&](uint32_t k, auto whole) {
             uint32_t jj[U];
-#pragma unroll
-            for (uint32_t u = 0; u < U; ++u)
+            if constexpr (HEAD && decltype(whole)::value)
+                head_words<U>(hrow + k, jj);
+            else
                 {
-                if constexpr (HEAD)
-                    jj[u] = k + u < nn ? hrow[k + u] : ie; // rows are not padded
-                else
-                    jj[u] = (a.nlist + (size_t)(k + u) * stride)[ie];
+#pragma unroll
+                for (uint32_t u = 0; u < U; ++u)
+                    {
+                    if constexpr (HEAD)
+                        jj[u] = k + u < nn ? hrow[k + u] : ie; // rows are not padded
+                    else
+                        jj[u] = (a.nlist + (size_t)(k + u) * stride)[ie];
+                    }
                 }
             double4 pj[U];
 #pragma unroll
@@ -582,7 +607,12 @@ __global__ void __launch_bounds__(512) k_pair(const PairKernelArgs a)
                 term_fp64<Eval, VIRIAL, XPLOR>(SINGLE ? q0 : s_prep[tp], SINGLE ? x0 : s_x[tp],
                                                jj[u] != ie, dx, dy, dz, acc);
                 }
-            }
+        };
+        uint32_t k = 0;
+        for (; k < F; k += U)
+            step(k, std::true_type{});
+        for (; k < P; k += U)
+            step(k, std::false_type{});
         if (i < a.N)
             store<VIRIAL>(a, i, acc);
         }
@@ -653,21 +683,28 @@ __global__ void __launch_bounds__(512) k_pair(const PairKernelArgs a)
         const uint32_t col_off = ie * 4u;
         const char* pf_base = reinterpret_cast<const char*>(a.pos_fixed);
         const uint32_t* hrow = HEAD ? a.nlist + a.head_list[ie] : nullptr;
+        // head list: below F every lane has U entries left in its row, read as
+        // 16-byte vectors
+        const uint32_t F = HEAD ? __builtin_amdgcn_readfirstlane(wave_min(nn) & ~(U - 1)) : 0u;
         Acc32 part;
-        for (uint32_t k = 0; k < P; k += U)
-            {
+        auto step = [&](uint32_t k, auto whole) {
             uint32_t jj[U];
-#pragma unroll
-            for (uint32_t u = 0; u < U; ++u)
+            if constexpr (HEAD && decltype(whole)::value)
+                head_words<U>(hrow + k, jj);
+            else
                 {
-                if constexpr (HEAD)
+#pragma unroll
+                for (uint32_t u = 0; u < U; ++u)
                     {
-                    jj[u] = k + u < nn ? hrow[k + u] : ie; // rows are not padded
-                    }
-                else
-                    {
-                    const char* row = reinterpret_cast<const char*>(a.nlist + (size_t)(k + u) * stride);
-                    jj[u] = *reinterpret_cast<const uint32_t*>(row + col_off);
+                    if constexpr (HEAD)
+                        {
+                        jj[u] = k + u < nn ? hrow[k + u] : ie; // rows are not padded
+                        }
+                    else
+                        {
+                        const char* row = reinterpret_cast<const char*>(a.nlist + (size_t)(k + u) * stride);
+                        jj[u] = *reinterpret_cast<const uint32_t*>(row + col_off);
+                        }
                     }
                 }
             int4 pj[U];
@@ -693,7 +730,12 @@ __global__ void __launch_bounds__(512) k_pair(const PairKernelArgs a)
                 flush32<VIRIAL>(acc, part);
                 part = Acc32();
                 }
-            }
+        };
+        uint32_t k = 0;
+        for (; k < F; k += U)
+            step(k, std::true_type{});
+        for (; k < P; k += U)
+            step(k, std::false_type{});
         if (i < a.N)
             store_scaled<VIRIAL>(a, i, acc);
         }
