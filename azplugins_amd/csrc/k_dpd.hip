@@ -159,10 +159,24 @@ __global__ void __launch_bounds__(512, 4) k_dpd(const DPDKernelArgs a)
         uint32_t cnt = 0;
         while (k + 4 <= nn && cnt + 4 <= qlen)
             {
-            const uint32_t j0 = row[(size_t)k * pitch];
-            const uint32_t j1 = row[(size_t)(k + 1) * pitch];
-            const uint32_t j2 = row[(size_t)(k + 2) * pitch];
-            const uint32_t j3 = row[(size_t)(k + 3) * pitch];
+            uint32_t j0, j1, j2, j3;
+            if (pitch == 1)
+                {
+                // consecutive entries (HOOMD's head-list rows): one 16-byte load
+                uint4 w;
+                __builtin_memcpy(&w, row + k, sizeof(w));
+                j0 = w.x;
+                j1 = w.y;
+                j2 = w.z;
+                j3 = w.w;
+                }
+            else
+                {
+                j0 = row[(size_t)k * pitch];
+                j1 = row[(size_t)(k + 1) * pitch];
+                j2 = row[(size_t)(k + 2) * pitch];
+                j3 = row[(size_t)(k + 3) * pitch];
+                }
             const double4 p0 = a.pos[j0];
             const double4 p1 = a.pos[j1];
             const double4 p2 = a.pos[j2];

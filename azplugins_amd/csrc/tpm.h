@@ -262,12 +262,34 @@ __device__ __forceinline__ void tpm_list(uint32_t ie,
     constexpr uint32_t U = 4;
     using PosT = std::conditional_t<MIXED, int4, double4>;
     using OriT = std::conditional_t<DIRS, float4, double4>;
+    // consecutive entries (HOOMD's head-list rows): below F every lane has
+    // U entries left, read as one 16-byte load
+    uint32_t F = 0;
+    if (stride == 1)
+        {
+        uint32_t m = nn;
+        for (int off = 32; off > 0; off >>= 1)
+            m = min(m, (uint32_t)__shfl_xor((int)m, off));
+        F = __builtin_amdgcn_readfirstlane(m & ~(U - 1));
+        }
     for (uint32_t k = 0; k < trips; k += U)
         {
         uint32_t jj[U];
+        if (k < F)
+            {
+            uint4 w;
+            __builtin_memcpy(&w, row + k, sizeof(w));
+            jj[0] = w.x;
+            jj[1] = w.y;
+            jj[2] = w.z;
+            jj[3] = w.w;
+            }
+        else
+            {
 #pragma unroll
-        for (uint32_t u = 0; u < U; ++u)
-            jj[u] = k + u < nn ? row[(size_t)(k + u) * stride] : ie;
+            for (uint32_t u = 0; u < U; ++u)
+                jj[u] = k + u < nn ? row[(size_t)(k + u) * stride] : ie;
+            }
         PosT pj[U];
         OriT oj[U];
 #pragma unroll
