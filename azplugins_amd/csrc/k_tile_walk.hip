@@ -197,9 +197,13 @@ __device__ uint32_t tile_walk_one(const TileWalkArgs& a,
     const int32_t fzi = to_fixed(pi.z, a.box.Linv[2]);
     const float sx = a.fx_scale[0], sy = a.fx_scale[1], sz = a.fx_scale[2];
     const float unit2 = CUBIC ? 1.0f / (sx * sx) : 1.0f; // r^2 -> fixed-point units^2
-    // the lane's 27 neighbor cells (own cell at 13), in stencil_axis order
+    // the lane's 27 neighbor cells in stencil_axis order; the own cell is the
+    // entry with offset 0 on every axis: the middle one, or the first on an
+    // axis one cell wide (a box of exactly 2 r_list), whose other entries
+    // repeat it and are empty
     const uint32_t* stencil = s.stencil + own * 27u;
-    const uint32_t own_first = stencil[13] & 0xffffu;
+    const uint32_t own_k = (a.g.n[0] > 1 ? 1u : 0u) + (a.g.n[1] > 1 ? 3u : 0u) + (a.g.n[2] > 1 ? 9u : 0u);
+    const uint32_t own_first = stencil[own_k] & 0xffffu;
     const uint32_t self = own_first + (slot - s.gs[union_cell_of(s.off, s.U, own_first)]);
     const int4 me = s.cand[self];
     const float rxi = __int_as_float(me.x), ryi = __int_as_float(me.y), rzi = __int_as_float(me.z);
@@ -344,8 +348,8 @@ __device__ uint32_t tile_walk_one(const TileWalkArgs& a,
             __builtin_amdgcn_sched_barrier(0);
             store_pending();
             // both batches whole for every lane still in its cell, and not
-            // the own cell (stencil entry 13): the tests need no masks
-            if (k != 13u && __ballot(m0 + 2 * kWalkBatch > e) == 0ull)
+            // the own cell: the tests need no masks
+            if (k != own_k && __ballot(m0 + 2 * kWalkBatch > e) == 0ull)
                 {
                 batch(pa, m0, e, std::false_type{});
                 load_batch(pa, m0 + 2 * kWalkBatch);

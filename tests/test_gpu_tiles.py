@@ -272,12 +272,15 @@ def test_fused_walk_relative_offsets_large_box():
     assert np.array_equal(plain.virial, tiled.virial)
 
 
-@pytest.mark.parametrize("Lv", [(26.0, 22.0, 19.0), (70.0, 64.0, 58.0)])
+@pytest.mark.parametrize("Lv", [(26.0, 22.0, 19.0), (70.0, 64.0, 58.0), (26.0, 22.0, 6.8)])
 def test_fused_walk_non_cubic_box(Lv):
     """Orthorhombic boxes: the per-tile walk in fixed point with per-axis
     scales (small box) and in FP32 offsets (large box); lists equal the plain
     walk's, the pair set equals the oracle's (small box), and the mixed forces
-    over the tiled list equal the plain-list ones and the FP64 oracle to 1e-5."""
+    over the tiled list equal the plain-list ones and the FP64 oracle to 1e-5.
+    The third box is twice r_list in z, the smallest box allowed: one cell
+    thick, where a particle's own cell is the only stencil entry on that
+    axis."""
     Lv = np.array(Lv)
     rng = np.random.default_rng(int(Lv[0]))
     n = [int(round(l / 1.056)) for l in Lv]
