@@ -105,6 +105,24 @@ class PairArgs(ctypes.Structure):
     ]
 
 
+class StepArgs(ctypes.Structure):
+    """azp_step_args: the MD step that follows the pair force in
+    azp_pair_update_compute_step."""
+
+    _fields_ = [
+        ("d_pos", ctypes.c_void_p),
+        ("d_vel", ctypes.c_void_p),
+        ("d_image", ctypes.c_void_p),
+        ("d_tag", ctypes.c_void_p),
+        ("dt", ctypes.c_double),
+        ("langevin", ctypes.c_int),
+        ("gamma", ctypes.c_double),
+        ("kT", ctypes.c_double),
+        ("seed", ctypes.c_uint16),
+        ("timestep", ctypes.c_uint64),
+    ]
+
+
 class BatchArrays(ctypes.Structure):
     _fields_ = [
         ("d_pos", ctypes.c_void_p),
@@ -216,6 +234,8 @@ SIGNATURES = [
     ("azp_pair_compute", _i, [_vp, _vp, ctypes.POINTER(Particles), ctypes.POINTER(Outputs), _u64, _i, _vp]),
     ("azp_pair_update_compute", _i, [_vp, _vp, ctypes.POINTER(Particles), ctypes.POINTER(Outputs), _u64, _i, _i,
                                      ctypes.POINTER(_i), _vp]),
+    ("azp_pair_update_compute_step", _i, [_vp, _vp, ctypes.POINTER(Particles), ctypes.POINTER(Outputs), _u64, _i,
+                                          _i, ctypes.POINTER(_i), ctypes.POINTER(StepArgs), _vp]),
     ("azp_batch_create", _i, [ctypes.POINTER(_vp), _u32, ctypes.POINTER(_u32), ctypes.POINTER(Box), _u32, _i]),
     ("azp_batch_destroy", _i, [_vp]),
     ("azp_batch_size", _i, [_vp, ctypes.POINTER(_u32), ctypes.POINTER(_u32)]),

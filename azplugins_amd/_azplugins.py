@@ -43,6 +43,12 @@ class PotentialPairGPUBase:
     def handle(self):
         return self._h
 
+    @property
+    def takes_step(self):
+        """Whether update_compute_step serves this force: the isotropic
+        drivers (DPD and the anisotropic driver integrate otherwise)."""
+        return self.driver in (0, 2)
+
     def setParams(self, ti, tj, values):
         arr = _azp.dbl_array(values)
         _azp.call("azp_pair_set_params", self._h, ti, tj, arr)
@@ -79,6 +85,16 @@ class PotentialPairGPUBase:
         _azp.call("azp_pair_update_compute", self._h, nlist_handle, ctypes.byref(particles), ctypes.byref(outputs),
                   int(timestep), 1 if compute_virial else 0, 1 if force_rebuild else 0, ctypes.byref(rebuilt),
                   _azp.stream_ptr(stream))
+        return bool(rebuilt.value)
+
+    def update_compute_step(self, nlist_handle, particles, outputs, timestep, step, compute_virial=True,
+                            force_rebuild=False, stream=None):
+        """azp_pair_update_compute_step: update, force, then the MD step
+        `step` (an _azp.StepArgs) with the next update's rebuild check."""
+        rebuilt = ctypes.c_int()
+        _azp.call("azp_pair_update_compute_step", self._h, nlist_handle, ctypes.byref(particles),
+                  ctypes.byref(outputs), int(timestep), 1 if compute_virial else 0, 1 if force_rebuild else 0,
+                  ctypes.byref(rebuilt), ctypes.byref(step), _azp.stream_ptr(stream))
         return bool(rebuilt.value)
 
     def compute(self, nlist_handle, particles, outputs, timestep, compute_virial=True, stream=None):

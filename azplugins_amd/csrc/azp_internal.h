@@ -22,6 +22,17 @@ int launcher_heap_scratch(uint32_t N, int4** out);
 // neighbor set of `slots` entries in one workgroup's LDS.
 bool dpd_tile_fits(uint32_t ntypes, uint32_t slots, uint32_t tile_size);
 
+struct StepKernelArgs;
+
+// The list side of a step's rebuild check (k_nlist.hip): the positions of
+// the last build, the flag word the step reports into (cleared here on
+// stream s) and the threshold; after the launch, which positions that flag
+// checked.  nlist_update_checked is azp_nlist_update that reads such a check
+// of p->d_pos instead of checking again.
+int nlist_step_check(azp_nlist* nl, hipStream_t s, StepKernelArgs* k);
+void nlist_step_checked(azp_nlist* nl, const double* d_pos);
+int nlist_update_checked(azp_nlist* nl, const azp_particles* p, int force_rebuild, int* rebuilt, void* stream);
+
 // Records a message for azp_last_error() and returns `code`.
 int set_error(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
 void clear_error();
@@ -135,5 +146,8 @@ inline unsigned grid_for(size_t n, unsigned block)
     {
     return (unsigned)((n + block - 1) / block);
     }
+
+// The step after a pair force, with the next rebuild check (k_integrate.hip).
+int launch_step(const double4* force, uint32_t N, const DevBox& box, const StepKernelArgs& step, hipStream_t s);
 
 } // namespace azp

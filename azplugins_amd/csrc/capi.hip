@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "azp_internal.h"
+#include "step.h"
 
 namespace azp
 {
@@ -533,4 +534,49 @@ extern "C" int azp_pair_update_compute(azp_pair* pp,
     {
     AZP_TRY(azp_nlist_update(nl, p, force_rebuild, rebuilt, stream));
     return azp_pair_compute(pp, nl, p, out, timestep, compute_virial, stream);
+    }
+
+extern "C" int azp_pair_update_compute_step(azp_pair* pp,
+                                            azp_nlist* nl,
+                                            const azp_particles* p,
+                                            const azp_outputs* out,
+                                            uint64_t timestep,
+                                            int compute_virial,
+                                            int force_rebuild,
+                                            int* rebuilt,
+                                            const azp_step_args* step,
+                                            void* stream)
+    {
+    if (!pp || !nl || !p || !out || !step)
+        return set_error(AZP_ERR_INVALID, "null argument");
+    if (pp->driver != AZP_DRIVER_PAIR && pp->driver != AZP_DRIVER_CONSERVATIVE)
+        return set_error(AZP_ERR_UNSUPPORTED,
+                         "the step follows isotropic pair forces only (the DPD and anisotropic drivers have their "
+                         "own integration)");
+    if (p->N > 0 && (!p->d_pos || !step->d_vel || !out->d_force || (step->langevin && !step->d_tag)))
+        return set_error(AZP_ERR_INVALID, "the step needs positions, velocities, forces and (Langevin) tags");
+    if (step->d_pos != p->d_pos)
+        return set_error(AZP_ERR_INVALID, "step->d_pos must be the positions the forces read (p->d_pos)");
+    if (!(step->dt > 0.0))
+        return set_error(AZP_ERR_INVALID, "dt must be positive");
+    AZP_TRY(nlist_update_checked(nl, p, force_rebuild, rebuilt, stream));
+    AZP_TRY(azp_pair_compute(pp, nl, p, out, timestep, compute_virial, stream));
+    if (p->N == 0)
+        return AZP_OK;
+    hipStream_t s = (hipStream_t)stream;
+    StepKernelArgs k{};
+    k.pos = reinterpret_cast<double4*>(step->d_pos);
+    k.vel = reinterpret_cast<double4*>(step->d_vel);
+    k.image = reinterpret_cast<int4*>(step->d_image);
+    k.tag = step->d_tag;
+    AZP_TRY(nlist_step_check(nl, s, &k));
+    k.dt = step->dt;
+    k.langevin = step->langevin;
+    k.gamma = step->gamma;
+    k.kT = step->kT;
+    k.seed = step->seed;
+    k.timestep = step->timestep;
+    AZP_TRY(launch_step(reinterpret_cast<const double4*>(out->d_force), p->N, make_box(p->box), k, s));
+    nlist_step_checked(nl, p->d_pos);
+    return AZP_OK;
     }

@@ -8,23 +8,10 @@
 // Deterministic FP64 reductions: fixed grid, one partial per workgroup,
 // partials summed in a fixed order by a second kernel.
 #include "device_common.h"
+#include "step.h"
 
 namespace azp
 {
-
-__device__ __forceinline__ void wrap_1d(double& x, int& img, double L)
-    {
-    if (x >= 0.5 * L)
-        {
-        x -= L;
-        ++img;
-        }
-    else if (x < -0.5 * L)
-        {
-        x += L;
-        --img;
-        }
-    }
 
 __global__ void k_nve_one(double4* __restrict__ pos,
                           double4* __restrict__ vel,
@@ -128,50 +115,34 @@ __global__ void k_vv_two_one(double4* __restrict__ pos,
         return;
     double4 v = vel[i];
     const double4 f = force[i];
-    double fx = f.x, fy = f.y, fz = f.z;
-    if (langevin)
-        {
-        uint32_t k0, k1;
-        hoomd_seed_key(kRngLangevin, timestep, seed, k0, k1);
-        u32x4 c;
-        c.v[0] = tag[i];
-        c.v[1] = 0u;
-        c.v[2] = 0u;
-        c.v[3] = 0u;
-        const u32x4 r0 = philox4x32_10(c, k0, k1);
-        c.v[3] = 1u;
-        const u32x4 r1 = philox4x32_10(c, k0, k1);
-        u32x4 a, b, d;
-        a.v[0] = r0.v[0]; a.v[1] = r0.v[1];
-        b.v[0] = r0.v[2]; b.v[1] = r0.v[3];
-        d.v[0] = r1.v[0]; d.v[1] = r1.v[1];
-        const double coeff = sqrt(6.0 * gamma * kT / dt);
-        fx += -gamma * v.x + coeff * uniform_m1_1(a);
-        fy += -gamma * v.y + coeff * uniform_m1_1(b);
-        fz += -gamma * v.z + coeff * uniform_m1_1(d);
-        }
-    const double minv = 1.0 / v.w;
-    const double ax = fx * minv, ay = fy * minv, az = fz * minv;
-    // second half of step t
-    v.x += 0.5 * dt * ax;
-    v.y += 0.5 * dt * ay;
-    v.z += 0.5 * dt * az;
-    // first half of step t + 1
-    v.x += 0.5 * dt * ax;
-    v.y += 0.5 * dt * ay;
-    v.z += 0.5 * dt * az;
     double4 p = pos[i];
-    p.x += dt * v.x;
-    p.y += dt * v.y;
-    p.z += dt * v.z;
     int4 im = image ? image[i] : make_int4(0, 0, 0, 0);
-    wrap_1d(p.x, im.x, box.L[0]);
-    wrap_1d(p.y, im.y, box.L[1]);
-    wrap_1d(p.z, im.z, box.L[2]);
+    vv_two_one_particle(p, v, im, f.x, f.y, f.z, langevin ? tag[i] : 0u, box, dt, langevin, gamma, kT, seed,
+                        timestep);
     pos[i] = p;
     vel[i] = v;
     if (image)
         image[i] = im;
+    }
+
+// The step of k_vv_two_one from forces already computed, with the list's
+// rebuild check of the new positions (so the next update needs no
+// displacement kernel).
+__global__ void k_step(const double4* __restrict__ force, uint32_t N, DevBox box, StepKernelArgs s)
+    {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = i < N;
+    step_particle(s, box, i, live, live ? force[i] : make_double4(0.0, 0.0, 0.0, 0.0));
+    }
+
+int launch_step(const double4* force, uint32_t N, const DevBox& box, const StepKernelArgs& step, hipStream_t s)
+    {
+    if (N == 0)
+        return AZP_OK;
+    const unsigned B = 256;
+    hipLaunchKernelGGL(k_step, dim3(grid_for(N, B)), dim3(B), 0, s, force, N, box, step);
+    AZP_LAUNCH_CHECK();
+    return AZP_OK;
     }
 
 // Spherical tops: L += dt tau / 2, then q <- exp(dt omega / 2) q with omega = L / I.

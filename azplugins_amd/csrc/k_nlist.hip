@@ -33,6 +33,7 @@
 #include <vector>
 
 #include "cells.h"
+#include "step.h"
 #include "tiles.h"
 
 namespace azp
@@ -359,18 +360,9 @@ __global__ void k_displacement(const double4* __restrict__ pos,
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N)
         return;
-    const double4 p = pos[i];
-    const double4 q = last[i];
-    if (!(isfinite(p.x) && isfinite(p.y) && isfinite(p.z)))
-        {
-        atomicOr(flag, 2u);
-        return;
-        }
-    const double dx = min_image_1d(p.x - q.x, box.L[0], box.Linv[0]);
-    const double dy = min_image_1d(p.y - q.y, box.L[1], box.Linv[1]);
-    const double dz = min_image_1d(p.z - q.z, box.L[2], box.Linv[2]);
-    if (rsq3_nofma(dx, dy, dz) > thresh_sq)
-        atomicOr(flag, 1u);
+    const uint32_t bits = displaced(pos[i], last[i], box, thresh_sq);
+    if (bits != 0u)
+        atomicOr(flag, bits);
     }
 
 __global__ void k_copy_d4(const double4* __restrict__ in, double4* __restrict__ out, uint32_t N)
@@ -424,6 +416,9 @@ struct azp_nlist
     azp_box built_box = {0, 0, 0};
     bool have_list = false;
     uint64_t n_builds = 0;
+    // flags[1] holds the rebuild check of these positions, made by the step
+    // after the last force (k_step); the next update_compute_step reads it
+    const double* checked_pos = nullptr;
 
     ~azp_nlist()
         {
@@ -449,6 +444,7 @@ static int nlist_build(azp_nlist* nl, const azp_particles* p, hipStream_t s)
     nl->have_list = false;
     nl->built_N = 0;
     nl->tiles.valid = false;
+    nl->checked_pos = nullptr;
     const uint32_t N = p->N;
     const DevBox box = make_box(p->box);
     const double rmax = nl->r_list_max();
@@ -719,7 +715,11 @@ extern "C" int azp_nlist_set_r_cut(azp_nlist* nl, uint32_t ntypes, const double*
 // Value the rebuild flag never takes (it holds bits 1 and 2 only).
 constexpr uint32_t kFlagPending = 0xa5a5a5a5u;
 
-extern "C" int azp_nlist_update(azp_nlist* nl, const azp_particles* p, int force_rebuild, int* rebuilt, void* stream)
+// azp_nlist_update; with use_checked, a rebuild check of p->d_pos that a
+// step made after the last update (nlist_step_checked) stands in for the
+// displacement kernel.  Any update consumes that check.
+static int nlist_update(azp_nlist* nl, const azp_particles* p, int force_rebuild, int* rebuilt, void* stream,
+                        bool use_checked)
     {
     if (rebuilt)
         *rebuilt = 0;
@@ -770,15 +770,21 @@ extern "C" int azp_nlist_update(azp_nlist* nl, const azp_particles* p, int force
     bool need = force_rebuild || !nl->have_list || nl->params_dirty || p->N != nl->built_N
                 || p->box.Lx != nl->built_box.Lx || p->box.Ly != nl->built_box.Ly
                 || p->box.Lz != nl->built_box.Lz;
+    // the step after the last force may have checked these positions already
+    const bool prechecked = use_checked && nl->checked_pos != nullptr && nl->checked_pos == p->d_pos;
+    nl->checked_pos = nullptr;
     if (!need && p->N > 0)
         {
-        const unsigned B = 256;
-        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, nl->flags.ptr + 1, 0u, (size_t)1);
-        const double half = 0.5 * nl->r_buff;
-        hipLaunchKernelGGL(k_displacement, dim3(grid_for(p->N, B)), dim3(B), 0, s,
-                           reinterpret_cast<const double4*>(p->d_pos), nl->last_pos.ptr, p->N,
-                           make_box(p->box), half * half, nl->flags.ptr + 1);
-        AZP_LAUNCH_CHECK();
+        if (!prechecked)
+            {
+            const unsigned B = 256;
+            hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, nl->flags.ptr + 1, 0u, (size_t)1);
+            const double half = 0.5 * nl->r_buff;
+            hipLaunchKernelGGL(k_displacement, dim3(grid_for(p->N, B)), dim3(B), 0, s,
+                               reinterpret_cast<const double4*>(p->d_pos), nl->last_pos.ptr, p->N,
+                               make_box(p->box), half * half, nl->flags.ptr + 1);
+            AZP_LAUNCH_CHECK();
+            }
         // The host polls the pinned word for the copied flag (a sentinel until
         // it lands) instead of waiting for the stream: the step's critical
         // path then skips the wake-up of a blocking synchronization.  After
@@ -825,6 +831,11 @@ extern "C" int azp_nlist_update(azp_nlist* nl, const azp_particles* p, int force
     return AZP_OK;
     }
 
+extern "C" int azp_nlist_update(azp_nlist* nl, const azp_particles* p, int force_rebuild, int* rebuilt, void* stream)
+    {
+    return nlist_update(nl, p, force_rebuild, rebuilt, stream, false);
+    }
+
 // Expands the tiled list into the plain one on stream s if that has not
 // happened since the last build.
 static int nlist_expand(const azp_nlist* cnl, hipStream_t s)
@@ -839,6 +850,29 @@ static int nlist_expand(const azp_nlist* cnl, hipStream_t s)
 
 namespace azp
 {
+int nlist_step_check(azp_nlist* nl, hipStream_t s, StepKernelArgs* k)
+    {
+    if (!nl || !nl->have_list)
+        return set_error(AZP_ERR_STATE, "neighbor list not built");
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, nl->flags.ptr + 1, 0u, (size_t)1);
+    AZP_LAUNCH_CHECK();
+    k->last = nl->last_pos.ptr;
+    k->flag = nl->flags.ptr + 1;
+    const double half = 0.5 * nl->r_buff;
+    k->thresh_sq = half * half;
+    return AZP_OK;
+    }
+
+void nlist_step_checked(azp_nlist* nl, const double* d_pos)
+    {
+    nl->checked_pos = d_pos;
+    }
+
+int nlist_update_checked(azp_nlist* nl, const azp_particles* p, int force_rebuild, int* rebuilt, void* stream)
+    {
+    return nlist_update(nl, p, force_rebuild, rebuilt, stream, true);
+    }
+
 // The list arrays for work enqueued on stream s.  With need_plain false the
 // plain list is returned only if it is current (else null: a tiled consumer).
 int nlist_get_on_stream(const azp_nlist* nl,

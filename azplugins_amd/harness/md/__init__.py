@@ -81,11 +81,26 @@ class Integrator:
     # Steps whose second half runs straight into the next step's first half
     # (nothing acts between them) fuse the two in one kernel per particle.
     fuse_half_steps = True
+    # With one isotropic pair force and one translational method, those fused
+    # half steps go to the force's library call, whose step kernel also makes
+    # the next step's rebuild check (no displacement kernel of its own).
+    step_with_force = True
+
+    def _steps_with_force(self):
+        return (self.step_with_force and self.fuse_half_steps and len(self.forces) == 1
+                and len(self.methods) == 1 and not self.integrate_rotational_dof
+                and hasattr(self.forces[0], "_takes_step") and self.forces[0]._takes_step())
 
     def _update(self, timestep, fuse_next=False):
         if not self._one_done:
             for m in self.methods:
                 m._step_one(self, timestep)
+        if fuse_next and self._steps_with_force():
+            f = self.forces[0]
+            f._compute_step(timestep + 1, self.methods[0], self)
+            self._net_force, self._net_virial, self._net_torque = f._force, f._virial, f._torque
+            self._one_done = True
+            return
         self._compute_net_force(timestep + 1)
         if fuse_next and self.fuse_half_steps and len(self.methods) == 1:
             self.methods[0]._step_two_one(self, timestep + 1)

@@ -145,5 +145,28 @@ class Pair(Force):
                               compute_virial=virial)
         self._virial_current = bool(virial)
 
+    def _takes_step(self):
+        return self._cpp is not None and self._cpp.takes_step
+
+    def _compute_step(self, timestep, method, integ):
+        """_compute(timestep) followed by method._step_two_one(integ,
+        timestep) in one library call, whose step kernel also makes the next
+        update's rebuild check (bit-identical to the separate calls)."""
+        from .... import _azp
+
+        self._thermostat(timestep)
+        virial = self._sim._needs_virial()
+        nl = self.nlist
+        nl._needs_update(timestep)  # pushes the list settings when due
+        st = self._sim.state
+        lang, gamma, _gamma_r, kT = method._langevin(timestep)
+        step = _azp.StepArgs(d_pos=st.pos.data_ptr(), d_vel=st.vel.data_ptr(), d_image=st.image.data_ptr(),
+                             d_tag=st.tag.data_ptr(), dt=integ.dt, langevin=lang, gamma=gamma, kT=kT,
+                             seed=int(self._sim.seed) & 0xFFFF, timestep=int(timestep))
+        self._cpp.update_compute_step(nl.handle, st.particles(), self._outputs(), timestep, step,
+                                      compute_virial=virial, force_rebuild=nl._take_force_flag())
+        nl._last = timestep
+        self._virial_current = bool(virial)
+
 
 from . import aniso  # noqa: E402,F401

@@ -515,6 +515,43 @@ int azp_pair_update_compute(azp_pair* pp,
                             int* rebuilt,
                             void* stream);
 
+/* No counterpart in the reference, whose integrators run after the force
+ * compute (TwoStepConstantVolume / TwoStepLangevin around PotentialPair).
+ * azp_pair_update_compute followed by a caller's MD step in one call, for
+ * drivers PAIR and CONSERVATIVE: once the forces are stored, one kernel
+ * takes the second half step of `timestep` and the first half step of the
+ * next (velocity Verlet; Langevin drag and uniform noise keyed by seed,
+ * timestep and tag), updating positions, velocities and images in place
+ * with the FP64 operations of azp_gpu_integrate_vv_two_one (bit-identical
+ * trajectories), and makes the list's rebuild check of the new positions.
+ * The next azp_pair_update_compute_step with the same position array reads
+ * that check instead of launching its own; any other list update discards
+ * it, and the caller must not write the positions in between except through
+ * an update that forces a rebuild. */
+typedef struct
+    {
+    double* d_pos;         /* the array p->d_pos points to (updated) */
+    double* d_vel;         /* double4[N] vx, vy, vz, mass (updated) */
+    int32_t* d_image;      /* int4[N] image counts (updated; may be NULL) */
+    const uint32_t* d_tag; /* tags (Langevin noise; may be NULL for velocity Verlet) */
+    double dt;
+    int langevin;          /* 0: velocity Verlet, 1: Langevin */
+    double gamma;          /* Langevin drag */
+    double kT;
+    uint16_t seed;
+    uint64_t timestep;     /* the step whose second half is taken */
+    } azp_step_args;
+int azp_pair_update_compute_step(azp_pair* pp,
+                                 azp_nlist* nl,
+                                 const azp_particles* p,
+                                 const azp_outputs* out,
+                                 uint64_t timestep,
+                                 int compute_virial,
+                                 int force_rebuild,
+                                 int* rebuilt,
+                                 const azp_step_args* step,
+                                 void* stream);
+
 /* ---- batched replicas (no single reference counterpart: R independent
  * AnisoPotentialPair<AnisoPairEvaluatorTwoPatchMorse> systems,
  * AnisoPotentialPairGPUKernel.cu.inc:21-25, each with its own periodic box,

@@ -105,6 +105,56 @@ def test_fused_half_steps_are_bit_identical(method):
         assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("precision,method", [("mixed", "langevin"), ("mixed", "nve"), ("fp64", "langevin"),
+                                              ("fp64", "nve")])
+def test_step_with_force_is_bit_identical(precision, method):
+    """With one isotropic pair force and one translational method, the fused
+    half steps go to the force's library call, whose step kernel also makes
+    the next step's rebuild check.  Trajectories and the steps that rebuild
+    the list equal those of the separate calls bit for bit, through rebuilds
+    and sorter reorders."""
+    import azplugins_amd.harness as hoomd
+
+    L = 16.0
+    pos4, rng = jittered_lattice(16, L, 0.1, 1, seed=37)
+    pos4 = pos4[rng.permutation(pos4.shape[0])]
+    runs = []
+    for joined in (True, False):
+        sim, plj = _sim(pos4, L, precision)
+        integ = sim.operations.integrator
+        if method == "langevin":
+            integ.methods = [hoomd.md.methods.Langevin(hoomd.filter.All(), kT=1.2)]
+        integ.step_with_force = joined
+        sim.operations.tuners[0].trigger = hoomd.trigger.Periodic(23)
+        sim.state.thermalize_particle_momenta(hoomd.filter.All(), kT=1.2)
+        sim.run(0)
+        cpp, rebuilt = plj._cpp, []
+        calls = {"step": 0}
+
+        def counted(fn, key=None):
+            def wrapped(*a, **k):
+                r = fn(*a, **k)
+                rebuilt.append(bool(r))
+                if key:
+                    calls[key] += 1
+                return r
+            return wrapped
+
+        cpp.update_compute = counted(cpp.update_compute)
+        cpp.update_compute_step = counted(cpp.update_compute_step, "step")
+        sim.run(60)
+        sim.run(5)
+        assert (calls["step"] > 50) == joined
+        snap = sim.state.get_snapshot()
+        runs.append((snap.particles.position.copy(), snap.particles.velocity.copy(), snap.particles.image.copy(),
+                     np.array(rebuilt)))
+    say(f"[harness] step with the force ({precision}, {method}): {int(runs[0][3].sum())} rebuilds in "
+        f"{runs[0][3].size} updates, max |dx| {np.abs(runs[0][0] - runs[1][0]).max():.3e}")
+    assert runs[0][3].sum() >= 3
+    for a, b in zip(runs[0], runs[1]):
+        assert np.array_equal(a, b)
+
+
 def test_buffer_tuner_in_md():
     """The buffer tuner picks one of its grid values during a run; the forces
     with the tuned buffer equal those of a fresh list."""
