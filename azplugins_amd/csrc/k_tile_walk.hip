@@ -112,6 +112,14 @@ __device__ __forceinline__ uint32_t union_cell_of(const uint32_t* off, uint32_t 
     return lo;
     }
 
+// Largest value over the 64 lanes of the wavefront (every lane gets it).
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v)
+    {
+    for (int off = 32; off > 0; off >>= 1)
+        v = max(v, (uint32_t)__shfl_xor((int)v, off));
+    return v;
+    }
+
 // Exclusive scan of one value per thread over the T-thread workgroup.
 template<uint32_t T> __device__ __forceinline__ uint32_t tile_scan(uint32_t v, uint32_t* s_wave, uint32_t& total)
     {
@@ -474,25 +482,60 @@ template<uint32_t T, bool SINGLE> __global__ void __launch_bounds__(T) k_tile_wa
     for (uint32_t k = U + tid; k < P2; k += T)
         s_ucell[k] = ~0ull;
     __syncthreads();
-    for (uint32_t size = 2; size <= P2; size <<= 1)
-        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1)
-            {
-            for (uint32_t e = tid; e < P2; e += T)
+    if (P2 <= T)
+        {
+        // one key per thread; stages with a stride below 64 exchange keys
+        // within the wavefront (no barrier), the others through LDS
+        uint64_t key = tid < P2 ? s_ucell[tid] : ~0ull;
+        for (uint32_t size = 2; size <= P2; size <<= 1)
+            for (uint32_t stride = size >> 1; stride > 0; stride >>= 1)
                 {
-                const uint32_t p = e ^ stride;
-                if (p > e)
+                uint64_t other;
+                if (stride >= 64u)
                     {
-                    const bool up = (e & size) == 0u;
-                    const uint64_t x = s_ucell[e], y = s_ucell[p];
-                    if ((x > y) == up)
+                    __syncthreads();
+                    if (tid < P2)
+                        s_ucell[tid] = key;
+                    __syncthreads();
+                    other = tid < P2 ? s_ucell[tid ^ stride] : ~0ull;
+                    }
+                else
+                    {
+                    const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)key, (int)stride);
+                    const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(key >> 32), (int)stride);
+                    other = ((uint64_t)hi << 32) | lo;
+                    }
+                // ascending blocks keep the smaller key at the lower position
+                const bool lower = (tid & stride) == 0u, up = (tid & size) == 0u;
+                key = (lower == up) ? min(key, other) : max(key, other);
+                }
+        __syncthreads();
+        if (tid < P2)
+            s_ucell[tid] = key;
+        __syncthreads();
+        }
+    else
+        {
+        for (uint32_t size = 2; size <= P2; size <<= 1)
+            for (uint32_t stride = size >> 1; stride > 0; stride >>= 1)
+                {
+                for (uint32_t e = tid; e < P2; e += T)
+                    {
+                    const uint32_t p = e ^ stride;
+                    if (p > e)
                         {
-                        s_ucell[e] = y;
-                        s_ucell[p] = x;
+                        const bool up = (e & size) == 0u;
+                        const uint64_t x = s_ucell[e], y = s_ucell[p];
+                        if ((x > y) == up)
+                            {
+                            s_ucell[e] = y;
+                            s_ucell[p] = x;
+                            }
                         }
                     }
+                __syncthreads();
                 }
-            __syncthreads();
-            }
+        }
 
     // 2. LDS offsets of the union cells' members; the anchor of the FP32
     // offsets is the tile's middle particle
@@ -682,45 +725,55 @@ template<uint32_t T, bool SINGLE> __global__ void __launch_bounds__(T) k_tile_wa
 
     // 5. columns ordered by list length (longest first, ties by slot), so the
     // pair kernel's 64-lane wavefronts run lists of nearly equal length: a
-    // bitonic sort of (length, slot) keys in the staging space (free now)
+    // bitonic sort of (length, slot) keys, one per thread, whose stages with
+    // a stride below 64 exchange keys within the wavefront (no barrier); the
+    // staging space (free now) carries the others and the slots' particles
+    // and lengths
     const uint32_t nn = active ? min(count, a.max_neighbors) : 0u;
     uint32_t* s_sort = reinterpret_cast<uint32_t*>(s_cand);
-    uint32_t* s_where = s_sort + T;
-    s_sort[tid] = (min(nn, 0xffffu) << 16) | (0xffffu - tid);
-    __syncthreads();
+    uint32_t* s_who = s_sort + T;
+    uint32_t* s_len = s_who + T;
+    s_who[tid] = i;
+    s_len[tid] = nn;
+    uint32_t key = (min(nn, 0xffffu) << 16) | (0xffffu - tid);
     for (uint32_t size = 2; size <= T; size <<= 1)
         for (uint32_t stride = size >> 1; stride > 0; stride >>= 1)
             {
-            const uint32_t p = tid ^ stride;
-            if (p > tid)
+            uint32_t other;
+            if (stride >= 64u)
                 {
-                const bool down = (tid & size) == 0u; // descending overall
-                const uint32_t x = s_sort[tid], y = s_sort[p];
-                if ((x < y) == down)
-                    {
-                    s_sort[tid] = y;
-                    s_sort[p] = x;
-                    }
+                __syncthreads();
+                s_sort[tid] = key;
+                __syncthreads();
+                other = s_sort[tid ^ stride];
                 }
-            __syncthreads();
+            else
+                other = (uint32_t)__shfl_xor((int)key, (int)stride);
+            // the lower position of a pair keeps the larger key in a
+            // descending block, the smaller in an ascending one
+            const bool lower = (tid & stride) == 0u, down = (tid & size) == 0u;
+            key = (lower == down) ? max(key, other) : min(key, other);
             }
-    s_where[0xffffu - (s_sort[tid] & 0xffffu)] = tid;
+    // column t T + tid holds the slot of the tid-th key: its writer is this
+    // thread, so consecutive lanes store consecutive columns
+    const uint32_t cslot = 0xffffu - (key & 0xffffu);
     __syncthreads();
-    const uint32_t place = s_where[tid];
-    const uint32_t col = t * T + place;
-    // the column wavefront's longest list: its first (longest) column's
-    const uint32_t P = (((s_sort[place & ~63u] >> 16) + 7u) & ~7u);
-    if (!active)
+    const uint32_t ci = s_who[cslot], cnn = s_len[cslot];
+    const bool cactive = t * T + cslot < a.N;
+    // the column wavefront's longest list, padded to whole steps of 8
+    const uint32_t P = (wave_max_u32(cactive ? cnn : 0u) + 7u) & ~7u;
+    if (!cactive)
         return;
-    a.tile_nn[col] = nn;
-    a.tile_order[col] = i;
-    a.tile_col[i] = col;
+    const uint32_t col = t * T + tid;
+    a.tile_nn[col] = cnn;
+    a.tile_order[col] = ci;
+    a.tile_col[ci] = col;
     // the row is reread in 16-byte chunks (8 entries), four in flight
-    const uint4* row = reinterpret_cast<const uint4*>(a.rows + (size_t)slot * a.max_neighbors);
-    auto chunk_load = [&](uint32_t kc) { return kc < nn ? row[kc / 8] : make_uint4(0u, 0u, 0u, 0u); };
+    const uint4* row = reinterpret_cast<const uint4*>(a.rows + (size_t)(t * T + cslot) * a.max_neighbors);
+    auto chunk_load = [&](uint32_t kc) { return kc < cnn ? row[kc / 8] : make_uint4(0u, 0u, 0u, 0u); };
     auto rank_of_entry = [&](uint32_t k, uint32_t word, uint32_t half) {
         const uint32_t m = (word >> (16u * half)) & 0xffffu;
-        return (k < nn ? (uint32_t)(s_mark[m] & ~kUsed) : M) << 4; // stored as 16 x rank
+        return (k < cnn ? (uint32_t)(s_mark[m] & ~kUsed) : M) << 4; // stored as 16 x rank
     };
     auto chunk_store = [&](uint32_t kc, uint4 ch) {
         if (kc >= P)
