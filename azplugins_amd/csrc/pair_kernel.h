@@ -329,9 +329,13 @@ __device__ __forceinline__ void term_mixed(const PairKernelArgs& a,
         rsq = (fx * fx + fy * fy + fz * fz) * a.fx_scale2[0];
     else
         rsq = fx * fx * a.fx_scale2[0] + fy * fy * a.fx_scale2[1] + fz * fz * a.fx_scale2[2];
-    bool in = valid && rsq < x.rcsq_lo;
+    // two compares give the three outcomes (inside, the FP64 band, outside) as
+    // lane masks from the compare builtins (float <)
+    const uint64_t below_lo = __builtin_amdgcn_fcmpf(rsq, x.rcsq_lo, 4);
+    const uint64_t below_hi = __builtin_amdgcn_fcmpf(rsq, x.rcsq_hi, 4);
+    bool in = valid && __builtin_amdgcn_inverse_ballot_w64(below_lo);
     double r64 = -1.0;
-    if (valid && !in && rsq < x.rcsq_hi)
+    if (valid && __builtin_amdgcn_inverse_ballot_w64(below_hi & ~below_lo))
         {
         // FP32 cannot decide this one: apply the FP64 cutoff test
         const uint32_t j = jmap ? jmap[jkey] : jkey;
