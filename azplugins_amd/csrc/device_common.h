@@ -13,8 +13,7 @@ namespace azp
 
 // ---------------------------------------------------------------------------
 // Minimum image in FP64: img = rint(d * Linv); d -= L * img.
-// Evaluated without contraction so that the neighbor predicate and the DPD
-// pair geometry are bit-identical to the host oracle.
+// Evaluated without contraction, as the host oracle evaluates it.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double min_image_1d(double d, double L, double Linv)
     {
@@ -22,6 +21,82 @@ __device__ __forceinline__ double min_image_1d(double d, double L, double Linv)
     const double img = rint(d * Linv);
     return d - L * img;
     }
+
+// The same minimum image with one rounding: for positions inside the box
+// |rint(d / L)| <= 2, so L rint(d / L) is exact and the fused form rounds
+// d - L rint(d / L) as the two-step form does; the pair and DPD kernels use
+// it, bit-identical to the oracle's geometry.
+__device__ __forceinline__ double min_image_fused(double d, double L, double Linv)
+    {
+    return __builtin_fma(-L, rint(d * Linv), d);
+    }
+
+// The largest x, y, z and -x, -y, -z (in FP32) of the positions a tile kernel
+// stages, reduced over its workgroup; wrap_axes() then tells, per axis,
+// whether a difference of two of them may need the periodic image.
+struct TileExtent
+    {
+    float m[6];
+    __device__ TileExtent()
+        {
+#pragma unroll
+        for (uint32_t c = 0; c < 6; ++c)
+            m[c] = -__builtin_huge_valf();
+        }
+    __device__ void add(double x, double y, double z)
+        {
+        const float fx = (float)x, fy = (float)y, fz = (float)z;
+        m[0] = fmaxf(m[0], fx);
+        m[1] = fmaxf(m[1], fy);
+        m[2] = fmaxf(m[2], fz);
+        m[3] = fmaxf(m[3], -fx);
+        m[4] = fmaxf(m[4], -fy);
+        m[5] = fmaxf(m[5], -fz);
+        }
+    // each wavefront's maxima to s_ext[wave * 8 + c]; before a barrier
+    __device__ void store_wave(float* s_ext) const
+        {
+#pragma unroll
+        for (uint32_t c = 0; c < 6; ++c)
+            {
+            float v = m[c];
+            for (int off = 32; off > 0; off >>= 1)
+                v = fmaxf(v, __shfl_xor(v, off));
+            if ((threadIdx.x & 63u) == 0)
+                s_ext[(threadIdx.x >> 6) * 8u + c] = v;
+            }
+        }
+    };
+
+// After the barrier: bit a set when the workgroup's positions span 0.45 L or
+// more on axis a.  A clear bit means every difference d of two of them has
+// |d / L| < 0.46, rint(d / L) = 0, and the minimum image on that axis is d
+// (up to the sign of a zero difference).  Wave-uniform (a scalar register).
+__device__ __forceinline__ uint32_t wrap_axes(const float* s_ext, const double* L)
+    {
+    float m[6];
+#pragma unroll
+    for (uint32_t c = 0; c < 6; ++c)
+        m[c] = -__builtin_huge_valf();
+    for (uint32_t w = 0; w < blockDim.x / 64u; ++w)
+        {
+#pragma unroll
+        for (uint32_t c = 0; c < 6; ++c)
+            m[c] = fmaxf(m[c], s_ext[w * 8u + c]);
+        }
+    uint32_t wrap = 0u;
+#pragma unroll
+    for (uint32_t c = 0; c < 3; ++c)
+        {
+        if (!(m[c] + m[c + 3] < 0.45f * (float)L[c]))
+            wrap |= 1u << c;
+        }
+    return __builtin_amdgcn_readfirstlane(wrap);
+    }
+
+// Bytes of LDS the per-wavefront maxima take (8 floats for each of up to 8
+// wavefronts).
+constexpr size_t kExtentBytes = 8 * 8 * sizeof(float);
 
 __device__ __forceinline__ double rsq3_nofma(double x, double y, double z)
     {
@@ -54,9 +129,18 @@ __device__ __forceinline__ uint32_t xcd_logical_block()
 // Precision-generic math.  FP64 keeps the IEEE operations of the reference
 // expressions; FP32 uses the hardware rcp/sqrt/exp/log units (about 1 ulp).
 // ---------------------------------------------------------------------------
+// 1 / x correctly rounded: the hardware reciprocal, two Newton steps and a
+// final correction, the steps of the compiler's IEEE division of 1 by x
+// without its operand scaling (exact powers of two, which change no bit of
+// the result for a normal x); the fixup step gives 1/0, 1/inf and NaN their
+// IEEE values.
 __device__ __forceinline__ double f_rcp(double x)
     {
-    return 1.0 / x;
+    double r = __builtin_amdgcn_rcp(x);
+    r = __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
+    r = __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
+    const double q = __builtin_fma(__builtin_fma(-x, r, 1.0), r, r);
+    return __builtin_amdgcn_div_fixup(q, x, 1.0);
     }
 __device__ __forceinline__ float f_rcp(float x)
     {

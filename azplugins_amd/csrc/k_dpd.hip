@@ -146,9 +146,9 @@ __global__ void __launch_bounds__(512, 4) k_dpd(const DPDKernelArgs a)
     double v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0, v5 = 0;
     const uint32_t nn = a.n_neigh[i];
     auto in_cut = [&](const double4& pj) {
-        const double dx = min_image_1d(pi.x - pj.x, a.box.L[0], a.box.Linv[0]);
-        const double dy = min_image_1d(pi.y - pj.y, a.box.L[1], a.box.Linv[1]);
-        const double dz = min_image_1d(pi.z - pj.z, a.box.L[2], a.box.Linv[2]);
+        const double dx = min_image_fused(pi.x - pj.x, a.box.L[0], a.box.Linv[0]);
+        const double dy = min_image_fused(pi.y - pj.y, a.box.L[1], a.box.Linv[1]);
+        const double dz = min_image_fused(pi.z - pj.z, a.box.L[2], a.box.Linv[2]);
         const double rsq = dx * dx + dy * dy + dz * dz;
         return rsq < rc_row[(uint32_t)pj.w * rc_stride];
     };
@@ -206,9 +206,9 @@ __global__ void __launch_bounds__(512, 4) k_dpd(const DPDKernelArgs a)
             const double4 pj = a.pos[j];
             const double4 vj = a.vel[j];
             const uint32_t tagj = a.tag[j];
-            const double dx = min_image_1d(pi.x - pj.x, a.box.L[0], a.box.Linv[0]);
-            const double dy = min_image_1d(pi.y - pj.y, a.box.L[1], a.box.Linv[1]);
-            const double dz = min_image_1d(pi.z - pj.z, a.box.L[2], a.box.Linv[2]);
+            const double dx = min_image_fused(pi.x - pj.x, a.box.L[0], a.box.Linv[0]);
+            const double dy = min_image_fused(pi.y - pj.y, a.box.L[1], a.box.Linv[1]);
+            const double dz = min_image_fused(pi.z - pj.z, a.box.L[2], a.box.Linv[2]);
             const double rsq = dx * dx + dy * dy + dz * dz;
             const DPDPair q = LDS_TABLE ? s_p[ti * nt + (uint32_t)pj.w] : make_dpd_pair(a, ti * nt + (uint32_t)pj.w);
             const double rdotv = dx * (vi.x - vj.x) + dy * (vi.y - vj.y) + dz * (vi.z - vj.z);
@@ -302,9 +302,9 @@ __global__ void __launch_bounds__(512, 4) k_dpd_tiled(const DPDKernelArgs a)
     double fx = 0, fy = 0, fz = 0, pe_sum = 0;
     double v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0, v5 = 0;
     auto in_cut = [&](const double4& pj) {
-        const double dx = min_image_1d(pi.x - pj.x, a.box.L[0], a.box.Linv[0]);
-        const double dy = min_image_1d(pi.y - pj.y, a.box.L[1], a.box.Linv[1]);
-        const double dz = min_image_1d(pi.z - pj.z, a.box.L[2], a.box.Linv[2]);
+        const double dx = min_image_fused(pi.x - pj.x, a.box.L[0], a.box.Linv[0]);
+        const double dy = min_image_fused(pi.y - pj.y, a.box.L[1], a.box.Linv[1]);
+        const double dz = min_image_fused(pi.z - pj.z, a.box.L[2], a.box.Linv[2]);
         const double rsq = dx * dx + dy * dy + dz * dz;
         return rsq < rc_row[(uint32_t)__double2loint(pj.w) * rc_stride];
     };
@@ -360,9 +360,9 @@ __global__ void __launch_bounds__(512, 4) k_dpd_tiled(const DPDKernelArgs a)
             const uint32_t j = (uint32_t)__double2hiint(pj.w);
             const double4 vj = a.vel[j];
             const uint32_t tagj = a.tag[j];
-            const double dx = min_image_1d(pi.x - pj.x, a.box.L[0], a.box.Linv[0]);
-            const double dy = min_image_1d(pi.y - pj.y, a.box.L[1], a.box.Linv[1]);
-            const double dz = min_image_1d(pi.z - pj.z, a.box.L[2], a.box.Linv[2]);
+            const double dx = min_image_fused(pi.x - pj.x, a.box.L[0], a.box.Linv[0]);
+            const double dy = min_image_fused(pi.y - pj.y, a.box.L[1], a.box.Linv[1]);
+            const double dz = min_image_fused(pi.z - pj.z, a.box.L[2], a.box.Linv[2]);
             const double rsq = dx * dx + dy * dy + dz * dz;
             const DPDPair q = s_p[ti * nt + (uint32_t)__double2loint(pj.w)];
             const double rdotv = dx * (vi.x - vj.x) + dy * (vi.y - vj.y) + dz * (vi.z - vj.z);

@@ -406,8 +406,9 @@ __host__ __device__ constexpr size_t fp64_slot_bytes(bool single)
 template<class Eval> static bool tile_fits64(uint32_t ntypes, uint32_t slots)
     {
     const size_t tables = ntypes == 1 ? 0 : table_bytes<Eval, double>(ntypes * ntypes);
-    return tables + (size_t)slots * fp64_slot_bytes(ntypes == 1) <= kTileLds64;
+    return tables + kExtentBytes + (size_t)slots * fp64_slot_bytes(ntypes == 1) <= kTileLds64;
     }
+
 
 // Neighbor-list layouts a kernel instance reads.
 constexpr int kLayoutColumn = 0; // nlist[k * pitch + i], padded with i
@@ -433,7 +434,10 @@ __global__ void __launch_bounds__(512) k_pair(const PairKernelArgs a)
     // 3/2 or 2 x it in the FP64 form's 24- or 32-byte slots) and the
     // type-pair tables after them
     constexpr size_t kSlotBytes = MIXED ? sizeof(int4) : fp64_slot_bytes(SINGLE);
-    Prep* s_prep = reinterpret_cast<Prep*>(reinterpret_cast<char*>(smem) + (TILED ? a.tile_slots * kSlotBytes : 0));
+    constexpr size_t kExtent = (TILED && !MIXED) ? kExtentBytes : 0;
+    [[maybe_unused]] float* s_ext = reinterpret_cast<float*>(reinterpret_cast<char*>(smem) + (TILED ? a.tile_slots * kSlotBytes : 0));
+    Prep* s_prep = reinterpret_cast<Prep*>(reinterpret_cast<char*>(smem) + (TILED ? a.tile_slots * kSlotBytes : 0)
+                                           + kExtent);
     TypePairExtra<R>* s_x = reinterpret_cast<TypePairExtra<R>*>(s_prep + nt2);
 
     Prep q0;
@@ -472,6 +476,9 @@ __global__ void __launch_bounds__(512) k_pair(const PairKernelArgs a)
         // with one type, x, y, z, type in 32-byte slots otherwise)
         tile_js = a.tile_jset + (size_t)lb * kTileCap;
         tile_m = a.tile_count[lb];
+        // extent of the staged positions and of the tile's own particles
+        TileExtent ext;
+        auto track = [&](const double4& p) { ext.add(p.x, p.y, p.z); };
         if constexpr (SINGLE)
             {
             double* s_pos3 = reinterpret_cast<double*>(smem);
@@ -481,6 +488,7 @@ __global__ void __launch_bounds__(512) k_pair(const PairKernelArgs a)
                     s_pos3[3 * m] = p.x;
                     s_pos3[3 * m + 1] = p.y;
                     s_pos3[3 * m + 2] = p.z;
+                    track(p);
                 });
             if (threadIdx.x == 0)
                 s_pos3[3 * tile_m] = s_pos3[3 * tile_m + 1] = s_pos3[3 * tile_m + 2] = 0.0;
@@ -490,13 +498,26 @@ __global__ void __launch_bounds__(512) k_pair(const PairKernelArgs a)
             double4* s_pos64 = reinterpret_cast<double4*>(smem);
             stage_slots<8>(
                 tile_m, tile_js, [&](uint32_t j) { return a.pos[j]; },
-                [&](uint32_t m, const double4& p) { s_pos64[m] = p; });
+                [&](uint32_t m, const double4& p) {
+                    s_pos64[m] = p;
+                    track(p);
+                });
             if (threadIdx.x == 0)
                 s_pos64[tile_m] = make_double4(0.0, 0.0, 0.0, 0.0);
             }
+        const uint32_t i0 = lb * blockDim.x + threadIdx.x;
+        if (i0 < a.N)
+            track(a.pos[a.tile_order != nullptr ? a.tile_order[i0] : i0]);
+        ext.store_wave(s_ext);
         }
     if (!SINGLE || TILED)
         __syncthreads();
+    // FP64 over the tiled list, bit a set: some pair of the tile may need
+    // the periodic image on axis a (the tile's positions span 0.45 L or more
+    // on it); on the other axes the minimum image is the difference itself
+    [[maybe_unused]] uint32_t wrap = 7u;
+    if constexpr (TILED && !MIXED)
+        wrap = wrap_axes(s_ext, a.box.L);
 
     const uint32_t i = lb * blockDim.x + threadIdx.x;
     // wavefronts entirely past N leave together; lanes past N in the last
@@ -555,15 +576,39 @@ __global__ void __launch_bounds__(512) k_pair(const PairKernelArgs a)
                     pj[u] = *reinterpret_cast<const double4*>(s_base + 2u * e[u]);
                     }
                 }
+            double dx[U], dy[U], dz[U];
 #pragma unroll
             for (uint32_t u = 0; u < U; ++u)
                 {
-                const double dx = min_image_1d(pi.x - pj[u].x, a.box.L[0], a.box.Linv[0]);
-                const double dy = min_image_1d(pi.y - pj[u].y, a.box.L[1], a.box.Linv[1]);
-                const double dz = min_image_1d(pi.z - pj[u].z, a.box.L[2], a.box.Linv[2]);
+                dx[u] = pi.x - pj[u].x;
+                dy[u] = pi.y - pj[u].y;
+                dz[u] = pi.z - pj[u].z;
+                }
+            // the images, on the axes the tile needs them (a scalar branch)
+            if (wrap & 1u)
+                {
+#pragma unroll
+                for (uint32_t u = 0; u < U; ++u)
+                    dx[u] = min_image_fused(dx[u], a.box.L[0], a.box.Linv[0]);
+                }
+            if (wrap & 2u)
+                {
+#pragma unroll
+                for (uint32_t u = 0; u < U; ++u)
+                    dy[u] = min_image_fused(dy[u], a.box.L[1], a.box.Linv[1]);
+                }
+            if (wrap & 4u)
+                {
+#pragma unroll
+                for (uint32_t u = 0; u < U; ++u)
+                    dz[u] = min_image_fused(dz[u], a.box.L[2], a.box.Linv[2]);
+                }
+#pragma unroll
+            for (uint32_t u = 0; u < U; ++u)
+                {
                 const uint32_t tp = SINGLE ? 0u : ti * nt + (uint32_t)pj[u].w;
                 term_fp64<Eval, VIRIAL, XPLOR>(SINGLE ? q0 : s_prep[tp], SINGLE ? x0 : s_x[tp],
-                                               !PADDED || e[u] != tile_m16, dx, dy, dz, acc);
+                                               !PADDED || e[u] != tile_m16, dx[u], dy[u], dz[u], acc);
                 }
         };
         uint32_t k = 0;
@@ -604,9 +649,9 @@ __global__ void __launch_bounds__(512) k_pair(const PairKernelArgs a)
 #pragma unroll
             for (uint32_t u = 0; u < U; ++u)
                 {
-                const double dx = min_image_1d(pi.x - pj[u].x, a.box.L[0], a.box.Linv[0]);
-                const double dy = min_image_1d(pi.y - pj[u].y, a.box.L[1], a.box.Linv[1]);
-                const double dz = min_image_1d(pi.z - pj[u].z, a.box.L[2], a.box.Linv[2]);
+                const double dx = min_image_fused(pi.x - pj[u].x, a.box.L[0], a.box.Linv[0]);
+                const double dy = min_image_fused(pi.y - pj[u].y, a.box.L[1], a.box.Linv[1]);
+                const double dz = min_image_fused(pi.z - pj[u].z, a.box.L[2], a.box.Linv[2]);
                 const uint32_t tp = SINGLE ? 0u : ti * nt + (uint32_t)pj[u].w;
                 term_fp64<Eval, VIRIAL, XPLOR>(SINGLE ? q0 : s_prep[tp], SINGLE ? x0 : s_x[tp],
                                                jj[u] != ie, dx, dy, dz, acc);
@@ -756,7 +801,7 @@ static int launch_t(const PairKernelArgs& ka, unsigned block, unsigned slots, hi
     PairKernelArgs kb = ka;
     kb.tile_slots = TILED ? slots : 0u;
     if (TILED)
-        shmem += (size_t)slots * (MIXED ? sizeof(int4) : fp64_slot_bytes(SINGLE));
+        shmem += (size_t)slots * (MIXED ? sizeof(int4) : fp64_slot_bytes(SINGLE)) + (MIXED ? 0 : kExtentBytes);
     const size_t cap = (TILED && !MIXED) ? kTileLds64 : kTileLds;
     if (shmem > cap)
         return set_error(AZP_ERR_UNSUPPORTED, "too many particle types (%u) for LDS staging", ka.ntypes);

@@ -120,6 +120,32 @@ def test_dpd_many_types_bit_exact(tiled):
     assert np.array_equal(r.virial.view(np.uint64), vo.view(np.uint64))
 
 
+@pytest.mark.parametrize("shift", [(0.0, 0.0, 0.0), (0.5, 0.0, 0.0), (0.5, 0.5, 0.5)])
+def test_dpd_tiled_images_per_axis(shift):
+    """The DPD kernels take the minimum image as one fused d - L rint(d / L),
+    which rounds as the oracle's two steps do.  Isolated pairs filling a
+    region 40 wide in a box of 100 (four tiles) sit at its centre (no pair
+    crosses a face), straddle the x face, or straddle every face: over the
+    tiled list the forces and virials equal the oracle's bit for bit in each
+    case."""
+    L = 100.0
+    pos4, vel4, tags = _isolated_pairs(1000, 40.0, 1.0, 19)
+    Lv = np.array([L, L, L])
+    pos4[:, :3] += Lv * np.array(shift)
+    pos4[:, :3] -= Lv * np.floor(pos4[:, :3] / Lv + 0.5)
+    seed, t, dt, kT = 3, 123457, 0.02, 1.0
+    say(f"[dpd images] shift {shift}")
+    r = gpu_pair("PotentialPairDPDThermoGeneralWeight", GpuSystem(pos4, Lv, vel4, tag=tags), 2,
+                 DPD_C3, 1.0, thermostat=(seed, dt, kT), timestep=t, tiled=True)
+    assert r.nl.get_tiles()[3] != 0
+    pairs = oracle.nlist(pos4, Lv, 2, oracle.r_listsq_matrix([1.0] * 4, 0.4), "brute")
+    assert len(pairs[0]) == 1000
+    fo, vo = oracle.dpd_forces(pos4, vel4, tags, Lv, pairs, 2, oracle.pack_matrix(4, DPD_C3, 2),
+                               np.ones(4), seed, t, dt, kT, portable_pow=True)
+    assert np.array_equal(r.force.view(np.uint64), fo.view(np.uint64))
+    assert np.array_equal(r.virial.view(np.uint64), vo.view(np.uint64))
+
+
 def test_dpd_many_body():
     L = 12.0
     rng = np.random.default_rng(5)

@@ -426,6 +426,42 @@ def test_fp64_tiled_colloid():
     assert np.array_equal(plain.virial, tiled.virial)
 
 
+@pytest.mark.parametrize("ntypes", [1, 2])
+@pytest.mark.parametrize("shift", [(0.0, 0.0, 0.0), (0.5, 0.0, 0.0), (0.5, 0.5, 0.5)])
+def test_fp64_tiled_images_per_axis(shift, ntypes):
+    """The FP64 form over the tiled list takes the periodic image only on the
+    axes where the tile's positions span 0.45 L or more.  A 12^3 lattice blob
+    in a box of 30 spans 12.3 < 13.5 on every axis at the centre (no image
+    anywhere), and straddles the x face (shift L/2 on x) or every face (shift
+    L/2 on all axes): in each case the forces and virials equal the plain
+    list's bit for bit, and the oracle's to 1e-12."""
+    L = 30.0
+    Lv = np.array([L, L, L])
+    pos4, rng = jittered_lattice(12, 12.0, 0.15, ntypes, seed=45)
+    pos4[:, :3] += Lv * np.array(shift)
+    pos4[:, :3] -= Lv * np.floor(pos4[:, :3] / Lv + 0.5)
+    sysm = GpuSystem(pos4, Lv)
+    plj = {(0, 0): {"epsilon": 1.0, "sigma": 1.0, "attraction_scale_factor": 0.5},
+           (0, 1): {"epsilon": 0.8, "sigma": 0.9, "attraction_scale_factor": 0.3},
+           (1, 1): {"epsilon": 1.2, "sigma": 0.95, "attraction_scale_factor": 0.7}}
+    params = {k: v for k, v in plj.items() if max(k) < ntypes}
+    rc = 2.5
+    plain, tiled = _fp64_pair("PotentialPairPerturbedLennardJones", sysm, ntypes, params, rc, mode="shift")
+    say(f"[tiles fp64 images] shift {shift} ntypes {ntypes}: tile size {tiled.nl.get_tiles()[3]}")
+    assert tiled.nl.get_tiles()[3] in (256, 512)
+    assert np.array_equal(plain.force, tiled.force)
+    assert np.array_equal(plain.virial, tiled.virial)
+    rcm = np.full(ntypes * ntypes, rc)
+    pairs = oracle.nlist(pos4, Lv, ntypes, oracle.r_listsq_matrix(rcm, 0.4), "cell")
+    fo, vo = oracle.pair_forces(oracle.PERTURBED_LJ, pos4, Lv, pairs, ntypes,
+                                oracle.pack_matrix(oracle.PERTURBED_LJ, params, ntypes), rcm * rcm,
+                                (0.7 * rcm) ** 2, "shift")
+    err = max(rel_err(tiled.force[:, :3], fo[:, :3]), rel_err(tiled.force[:, 3], fo[:, 3]),
+              rel_err(tiled.virial, vo))
+    say(f"  fp64 tiled vs oracle {err:.2e}")
+    assert err < 1e-12
+
+
 def test_mixed_acc64_tiled_equals_plain():
     """FP32 pair arithmetic with every entry's terms added to the FP64
     accumulators: tiled and plain lists give the same bits; the force error
