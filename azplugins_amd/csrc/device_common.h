@@ -164,7 +164,7 @@ __device__ __forceinline__ float f_sqrt(float x)
     }
 __device__ __forceinline__ double f_rsqrt(double x)
     {
-    return 1.0 / sqrt(x);
+    return f_rcp(sqrt(x)); // the same bits as 1.0 / sqrt(x)
     }
 __device__ __forceinline__ float f_rsqrt(float x)
     {

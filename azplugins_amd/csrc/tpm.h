@@ -63,8 +63,10 @@ __device__ __forceinline__ void tpm_eval(const TPMPrep<R>& q,
                                          R t_i[3],
                                          R& e)
     {
+    // reciprocals through f_rcp: the hardware reciprocal in FP32 (about one
+    // ulp), the correctly rounded 1/x in FP64
     const R rinv = f_rsqrt(rsq);
-    const R r = R(1) / rinv;
+    const R r = f_rcp(rinv);
     const R u[3] = {dr[0] * rinv, dr[1] * rinv, dr[2] * rinv};
     R UMorse = R(-1) * q.M_d;
     R dUMorse_dr = R(0);
@@ -77,10 +79,10 @@ __device__ __forceinline__ void tpm_eval(const TPMPrep<R>& q,
         }
     const R gi = u[0] * n_i[0] + u[1] * n_i[1] + u[2] * n_i[2];
     const R gie = f_exp(-q.omega * (gi * gi - q.alpha));
-    const R Oi = R(1) / (R(1) + gie);
+    const R Oi = f_rcp(R(1) + gie);
     const R gj = u[0] * n_j[0] + u[1] * n_j[1] + u[2] * n_j[2];
     const R gje = f_exp(-q.omega * (gj * gj - q.alpha));
-    const R Oj = R(1) / (R(1) + gje);
+    const R Oj = f_rcp(R(1) + gje);
     e = UMorse * Oi * Oj;
     const R dU_dr = dUMorse_dr * Oi * Oj;
     const R dOi = R(2) * q.omega * gi * gie * Oi * Oi;
