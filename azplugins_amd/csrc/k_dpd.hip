@@ -73,8 +73,9 @@ __device__ __forceinline__ void dpd_eval(const DPDPair& q,
                                          double& fdivr_cons,
                                          double& e)
     {
-    const double rinv = 1.0 / sqrt(rsq);
-    const double r = 1.0 / rinv;
+    // f_rcp is the correctly rounded 1/x: the same bits as the divisions
+    const double rinv = f_rcp(sqrt(rsq));
+    const double r = f_rcp(rinv);
     const double alpha = dpd_alpha(seed, timestep, ti, tj);
     fdivr = q.A * (rinv - q.rcutinv);
     fdivr_cons = fdivr;
