@@ -66,7 +66,11 @@ def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    # untimed steps before the timed ones: the GPU's clocks ramp up over the
+    # first tens of milliseconds of load (at 1M the step takes 200.7 us after
+    # 5 warm-up steps, 192.5 after 50 and 187.5 after 200, the sustained rate
+    # an MD run sees)
+    ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tps", action="store_true")
     ap.add_argument("--tile", type=int, default=1, choices=(0, 1, 256, 512),
