@@ -167,6 +167,36 @@ def test_empty_and_isolated():
     assert nlist_pairs(r.nl, 3) == set()
 
 
+@pytest.mark.parametrize("tiled", [False, True])
+def test_coincident_particles_match_the_oracle(tiled):
+    """Two particles at one point: r^2 = 0 and the FP64 reciprocal gives
+    1/0 = +inf as IEEE division does, so the pair's energy is +inf and its
+    force and virial are NaN, as in the oracle.  Every other value is finite
+    and equals the oracle's to 1e-12."""
+    pos4, _ = jittered_lattice(8, 9.0, 0.1, 1, seed=51)
+    pos4 = np.concatenate([pos4, pos4[:1]])
+    N = pos4.shape[0]
+    Lv = np.array([9.0, 9.0, 9.0])
+    params = {(0, 0): ISO_PARAMS[2][(0, 0)]}
+    rc = 2.5
+    r = gpu_pair("PotentialPairPerturbedLennardJones", GpuSystem(pos4, Lv), 1, params, rc, tiled=tiled)
+    pairs = oracle.nlist(pos4, Lv, 1, oracle.r_listsq_matrix([rc], 0.4), "cell")
+    fo, vo = oracle.pair_forces(oracle.PERTURBED_LJ, pos4, Lv, pairs, 1,
+                                oracle.pack_matrix(oracle.PERTURBED_LJ, params, 1), np.array([rc * rc]))
+    say(f"[coincident] tiled={tiled}: energy {r.force[0, 3]}, force {r.force[0, :3]}")
+    assert np.isposinf(r.force[0, 3]) and np.isposinf(r.force[N - 1, 3])
+    assert np.all(np.isnan(r.force[[0, N - 1], :3]))
+    for got, ref in ((r.force, fo), (r.virial, vo)):
+        assert np.array_equal(np.isnan(got), np.isnan(ref))
+        assert np.array_equal(np.isposinf(got), np.isposinf(ref))
+        assert np.array_equal(np.isneginf(got), np.isneginf(ref))
+    fin = np.isfinite(fo).all(axis=1)
+    assert fin.sum() == N - 2
+    assert rel_err(r.force[fin], fo[fin]) < 1e-12
+    vfin = np.isfinite(vo).all(axis=0)
+    assert rel_err(r.virial[:, vfin], vo[:, vfin]) < 1e-12
+
+
 def test_invalid_inputs_are_errors():
     """Non-finite positions and type indices without parameters are reported
     as errors by the neighbor-list build and the rebuild check (no silent
