@@ -396,7 +396,7 @@ struct azp_nlist
     DeviceArray<uint32_t> rank_cell; // its inverse
     int rank_grid[3] = {0, 0, 0};
     DeviceArray<uint32_t> flags; // [0] overflow, [1] rebuild, [2] bad type, [3] tile misfit, [4] largest tile set,
-                                 // [5] tiles walked in FP32 offsets
+                                 // [5] tiles walked in FP32 offsets, [6] the step kernel's check
     uint32_t* h_flags = nullptr;  // pinned, 8 words ([4], [5] tile build flags)
 
     bool want_tiles = false;
@@ -416,9 +416,13 @@ struct azp_nlist
     azp_box built_box = {0, 0, 0};
     bool have_list = false;
     uint64_t n_builds = 0;
-    // flags[1] holds the rebuild check of these positions, made by the step
-    // after the last force (k_step); the next update_compute_step reads it
+    // flags[6] holds the rebuild check of these positions, made by the step
+    // after the last force (k_step) as (checked_seq << 2) | bits; the next
+    // update_compute_step reads it.  Each step check takes a new number, so
+    // the word is cleared only on first use and when the numbers wrap.
     const double* checked_pos = nullptr;
+    uint32_t step_seq = 0;
+    uint32_t checked_seq = 0;
 
     ~azp_nlist()
         {
@@ -462,7 +466,7 @@ static int nlist_build(azp_nlist* nl, const azp_particles* p, hipStream_t s)
     AZP_TRY(nl->rank_of.reserve(N));
     AZP_TRY(nl->n_neigh.reserve(N));
     AZP_TRY(nl->last_pos.reserve(N));
-    AZP_TRY(nl->flags.reserve(6));
+    AZP_TRY(nl->flags.reserve(7));
 
     if (g.n[0] != nl->rank_grid[0] || g.n[1] != nl->rank_grid[1] || g.n[2] != nl->rank_grid[2])
         {
@@ -789,10 +793,11 @@ static int nlist_update(azp_nlist* nl, const azp_particles* p, int force_rebuild
         // it lands) instead of waiting for the stream: the step's critical
         // path then skips the wake-up of a blocking synchronization.  After
         // 2 ms it falls back to synchronizing the stream.
-        volatile uint32_t* hflag = nl->h_flags + 1;
+        const uint32_t w = prechecked ? 6u : 1u;
+        volatile uint32_t* hflag = nl->h_flags + w;
         *hflag = kFlagPending;
         std::atomic_thread_fence(std::memory_order_seq_cst);
-        AZP_HIP(hipMemcpyAsync(nl->h_flags + 1, nl->flags.ptr + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        AZP_HIP(hipMemcpyAsync(nl->h_flags + w, nl->flags.ptr + w, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         const auto t0 = std::chrono::steady_clock::now();
         while (*hflag == kFlagPending)
             {
@@ -803,9 +808,12 @@ static int nlist_update(azp_nlist* nl, const azp_particles* p, int force_rebuild
                 }
             }
         std::atomic_thread_fence(std::memory_order_seq_cst);
-        if (nl->h_flags[1] & 2u)
+        uint32_t bits = nl->h_flags[w];
+        if (prechecked)
+            bits = (bits >> 2) == nl->checked_seq ? (bits & 3u) : 0u;
+        if (bits & 2u)
             return set_error(AZP_ERR_INVALID, "a particle position is not finite (NaN or inf)");
-        need = nl->h_flags[1] != 0;
+        need = bits != 0;
         }
     nl->params_dirty = false;
     if (!need)
@@ -854,10 +862,18 @@ int nlist_step_check(azp_nlist* nl, hipStream_t s, StepKernelArgs* k)
     {
     if (!nl || !nl->have_list)
         return set_error(AZP_ERR_STATE, "neighbor list not built");
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, nl->flags.ptr + 1, 0u, (size_t)1);
-    AZP_LAUNCH_CHECK();
+    // a new check number per step; the word is cleared on first use and
+    // before the numbers would pass 2^30 (the flag holds seq << 2)
+    if (nl->step_seq == 0 || nl->step_seq >= (1u << 30) - 1u)
+        {
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, nl->flags.ptr + 6, 0u, (size_t)1);
+        AZP_LAUNCH_CHECK();
+        nl->step_seq = 0;
+        }
+    ++nl->step_seq;
     k->last = nl->last_pos.ptr;
-    k->flag = nl->flags.ptr + 1;
+    k->flag = nl->flags.ptr + 6;
+    k->seq = nl->step_seq;
     const double half = 0.5 * nl->r_buff;
     k->thresh_sq = half * half;
     return AZP_OK;
@@ -866,6 +882,7 @@ int nlist_step_check(azp_nlist* nl, hipStream_t s, StepKernelArgs* k)
 void nlist_step_checked(azp_nlist* nl, const double* d_pos)
     {
     nl->checked_pos = d_pos;
+    nl->checked_seq = nl->step_seq;
     }
 
 int nlist_update_checked(azp_nlist* nl, const azp_particles* p, int force_rebuild, int* rebuilt, void* stream)

@@ -112,7 +112,8 @@ struct StepKernelArgs
     int4* image;          // may be null
     const uint32_t* tag;  // Langevin keys (may be null for velocity Verlet)
     const double4* last;  // positions at the last list build
-    uint32_t* flag;       // rebuild check of the new positions (bits as displaced())
+    uint32_t* flag;       // rebuild check of the new positions: (seq << 2) | bits as displaced(), by atomicMax
+    uint32_t seq;         // this step's check number (never 0; the word needs no clearing between steps)
     double thresh_sq;
     double dt;
     int langevin;
@@ -147,7 +148,7 @@ __device__ __forceinline__ void step_particle(const StepKernelArgs& s,
         }
     const uint64_t moved = __ballot(bits & 1u), bad = __ballot(bits & 2u);
     if ((moved | bad) != 0ull && (threadIdx.x & 63u) == (uint32_t)__builtin_ctzll(moved | bad))
-        atomicOr(s.flag, (moved ? 1u : 0u) | (bad ? 2u : 0u));
+        atomicMax(s.flag, (s.seq << 2) | (moved ? 1u : 0u) | (bad ? 2u : 0u));
     }
 
 } // namespace azp
